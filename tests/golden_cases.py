@@ -1,0 +1,41 @@
+"""Loader for the golden fixtures written by tests/golden/make_golden.py."""
+import os
+
+import numpy as np
+
+from laboetie_amd.lbin import LbIn, read_geom_in
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FULL_CASES = ["cfg1_slit", "bcc12_ads", "cyl9_tau08", "chromat_2disks", "random_3d", "shipped_geom"]
+
+
+def load(name):
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+class Case:
+    """One reference run: its lb.in, geometry and dumped outputs."""
+
+    def __init__(self, name):
+        self.name = name
+        self.g = load(name)
+        self.lb = LbIn(str(self.g["lb_in"]))
+        self.dims = tuple(int(v) for v in self.g["dims"])
+        self.N = int(np.prod(self.dims))
+        self.label = self.lb.int("geometryLabel", 0)
+        self.tau = self.lb.dp("relaxation_time", 1.0)
+        self.rho0 = self.lb.dp("initialSolventDensity", 1.0)
+        self.fext = self.lb.dp3("f_ext", [0.0, 0.0, 0.0])
+        self.target = self.lb.dp("target_error", 1e-10)
+        self.mp_steps = self.lb.int("maximum_moment_propagation_steps", 0)
+        self.Db = self.lb.dp("tracer_Db", 0.0)
+        self.ka = self.lb.dp("tracer_ka", 0.0)
+        self.kd = self.lb.dp("tracer_kd", 0.0)
+        self.solid_ijk = read_geom_in(str(self.g["geom_in"]))
+        self.nature = self.g["nature"]
+        self.interfacial = self.g["interfacial"]
+
+    @property
+    def j(self):
+        return np.stack([self.g["jx"], self.g["jy"], self.g["jz"]])
