@@ -1,0 +1,216 @@
+"""Benchmark of the laboetie hot path on one MI355X.
+
+Workload (one bench "step" = one full solve, inputs resident in HBM):
+BCC cell of touching solid spheres (geometryLabel = 3), lattice 128^3,
+periodic, density 1, tau 1, f_ext = 1e-6 along z; D3Q19 equilibration until
+max|j - j_old| <= 1e-10 (the reference's two-stage loop), then for each
+adsorption free energy dF in {0, -1, -2} kT (kd = 0.01, ka = kd exp(-dF))
+10000 moment-propagation steps of a tracer with Db = 0.01, giving VACF and
+D_eff.  FP64 throughout.
+
+metric: lattice-node updates per second over LB and MP steps together
+(lx*ly*lz per step, the usual MLUPS convention), whole job.  The JSON line
+also carries the roofline of the dominant kernel (algorithmic HBM bytes per
+launch / measured mean launch duration, HIP events on the library's stream)
+and the reference program timed on this machine's host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import numpy as np  # noqa: E402
+
+from laboetie_amd import lbx  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
+LB_BYTES_PER_FLUID = 19 * 8 * 2                 # read + write of 19 FP64 populations
+MP_BYTES_PER_FLUID = 19 * 8 + 8 + 2 * 3 * 8     # coefficients (n, rho) + P read/write
+MP_BYTES_PER_ADS_INTERFACIAL = 2 * 3 * 8        # Pa read + write
+
+WORKLOAD = dict(label=3, dims=(128, 128, 128), tau=1.0, rho0=1.0, fext=(0.0, 0.0, 1e-6), target=1e-10,
+                Db=0.01, kd=0.01, dF=(0.0, -1.0, -2.0), mp_steps=10000)
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_bcc128.txt")
+
+
+def solve(L, w):
+    """One full solve; returns (lb_steps, mp_steps, D_eff per dF)."""
+    L.init_rest(w["rho0"])
+    r = L.equilibrate(w["fext"], w["target"], trace_cap=1)
+    deffs = []
+    mp_steps = 0
+    for dF in w["dF"]:
+        ka = w["kd"] * np.exp(-dF)
+        L.mp_init(w["fext"], w["Db"], ka, w["kd"])
+        m = L.mp_run(w["mp_steps"])
+        mp_steps += m["steps"]
+        deffs.append(L.mp_deff().tolist())
+    L.sync()
+    return r["steps"], mp_steps, deffs
+
+
+def read_traffic():
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), as committed in profiles/traffic_bcc128.txt lines
+    `<kernel> <bytes_per_launch>`; None when absent."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return {}
+    out = {}
+    for ln in open(TRAFFIC_FILE):
+        ln = ln.split("#")[0].split()
+        if len(ln) == 2:
+            out[ln[0]] = float(ln[1])
+    return out
+
+
+def lb_in_text(w, fext, mp_steps, ka, print_frequency=1):
+    lx, ly, lz = w["dims"]
+    return (f"lx = {lx}\nly = {ly}\nlz = {lz}\ngeometryLabel = {w['label']}\n"
+            f"relaxation_time = {w['tau']}\ninitialSolventDensity = {w['rho0']}\n"
+            f"f_ext = {fext[0]} {fext[1]} {fext[2]}\ntarget_error = {w['target']}\n"
+            f"print_frequency = {print_frequency}\nmaximum_moment_propagation_steps = {mp_steps}\n"
+            f"tracer_Db = {w['Db']}\ntracer_ka = {ka}\ntracer_kd = {w['kd']}\n")
+
+
+def _sample_growth(exe, d, fname, window, threads, max_wait):
+    """Run the reference in d until `fname` has grown for `window` seconds;
+    return steps/second from its line count."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    path = os.path.join(d, "output", fname)
+    p = subprocess.Popen([exe], cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    t_start = time.time()
+    first = None
+    last = None
+    try:
+        while time.time() - t_start < max_wait:
+            time.sleep(0.25)
+            if p.poll() is not None:
+                break
+            if not os.path.exists(path):
+                continue
+            with open(path, "rb") as fh:
+                n = sum(1 for _ in fh)
+            now = time.time()
+            if first is None and n >= 2:
+                first = (now, n)
+            elif first is not None:
+                last = (now, n)
+                if now - first[0] >= window:
+                    break
+    finally:
+        if p.poll() is None:
+            p.kill()
+            p.wait()
+    if first is None or last is None or last[1] <= first[1]:
+        return None
+    return (last[1] - first[1]) / (last[0] - first[0])
+
+
+def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0):
+    """The reference program (oracle/_ref/laboetie_fast: flang -O3 -fopenmp
+    -funroll-loops, the reference Makefile's flags) on this machine's host
+    cores, timed on a bounded sample of the same workload: the LB step rate
+    from the growth of output/l2err.dat over `window` seconds of the 128^3
+    BCC run, the MP step rate from output/vacf.dat of a run whose flow is at
+    rest (f_ext = 0, so equilibration ends at t = 4; MP cost does not depend
+    on the flow).  Whole-job MLUPS = N (LB + MP steps) / (LB/rate + MP/rate)."""
+    exe = os.path.join(HERE, "oracle", "_ref", "laboetie_fast")
+    if not os.path.exists(exe):
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    N = int(np.prod(w["dims"]))
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, w["fext"], 0, 0.0))
+        lb_rate = _sample_growth(exe, d, "l2err.dat", window, threads, max_wait=240)
+    with tempfile.TemporaryDirectory() as d:
+        ka = w["kd"] * np.exp(1.0)
+        open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, (0.0, 0.0, 0.0), 10 ** 7, ka))
+        mp_rate = _sample_growth(exe, d, "vacf.dat", window, threads, max_wait=240)
+    if not lb_rate or not mp_rate:
+        return None
+    t_job = lb_steps / lb_rate + mp_steps_total / mp_rate
+    return {"value": round(N * (lb_steps + mp_steps_total) / t_job / 1e6, 4), "unit": "MLUPS",
+            "cores": threads, "kind": "reference",
+            "sample": (f"reference laboetie (flang -O3 -fopenmp -funroll-loops), OMP_NUM_THREADS={threads}: "
+                       f"LB {lb_rate:.3f} steps/s over {window:.0f} s of the 128^3 BCC run, MP "
+                       f"{mp_rate:.3f} steps/s over {window:.0f} s (dF=-1 tracer), combined over this job's "
+                       f"{lb_steps} LB + {mp_steps_total} MP steps"),
+            "lb_mlups": round(N * lb_rate / 1e6, 4), "mp_mlups": round(N * mp_rate / 1e6, 4)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed full solves")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOAD
+    lx, ly, lz = w["dims"]
+    N = lx * ly * lz
+    nat = lbx.build_geometry(w["label"], lx, ly, lz)
+    itf = lbx.detect_interfacial(lx, ly, lz, nat)
+    nfi = int(((nat == 0) & (itf == 1)).sum())
+    with lbx.Lattice(w["dims"], nat, w["tau"]) as L:
+        nf = L.fluid_nodes
+        for _ in range(args.warmup):
+            solve(L, w)
+        L.timing(True)
+        L.sync()
+        t0 = time.perf_counter()
+        results = [solve(L, w) for _ in range(args.steps)]
+        L.sync()
+        dt = time.perf_counter() - t0
+        lb_ms, lb_n = L.kernel_time(lbx.KERNEL_LB)
+        mp_ms, mp_n = L.kernel_time(lbx.KERNEL_MP)
+    lb_steps, mp_steps, deffs = results[-1]
+    updates = N * sum(r[0] + r[1] for r in results)
+    value = updates / dt / 1e6
+    lb_avg_s = lb_ms / lb_n / 1e3
+    mp_avg_s = mp_ms / mp_n / 1e3
+    lb_bytes = LB_BYTES_PER_FLUID * nf
+    mp_bytes = MP_BYTES_PER_FLUID * nf + MP_BYTES_PER_ADS_INTERFACIAL * nfi * 2 / 3  # 2 of 3 dF adsorb
+    lb_gbs = lb_bytes / lb_avg_s / 1e9
+    mp_gbs = mp_bytes / mp_avg_s / 1e9
+    traffic = read_traffic()
+    dominant = "lb_step" if lb_ms >= mp_ms else "mp_step"
+    roof_lb = {"bound": "hbm", "achieved": round(lb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(lb_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("k_lb_step"),
+               "kernel": "k_lb_step", "bytes_per_launch": lb_bytes, "avg_launch_us": round(lb_avg_s * 1e6, 2)}
+    roof_mp = {"bound": "hbm", "achieved": round(mp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(mp_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("k_mp_step"),
+               "kernel": "k_mp_step", "bytes_per_launch": round(mp_bytes), "avg_launch_us": round(mp_avg_s * 1e6, 2)}
+    line = {
+        "metric": "D3Q19+MP MLUPS & HBM roofline %, FP64 vs FP32-shift; p50/p99 solve latency",
+        "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "BCC touching spheres 128^3 (geometryLabel 3), FP64: equilibrate to "
+                               "max|dj|<=1e-10 then 3 x 10000 adsorbing MP steps (dF 0,-1,-2 kT)",
+                   "lattice": [lx, ly, lz], "fluid_nodes": nf, "lb_steps": lb_steps, "mp_steps": mp_steps,
+                   "precision": "fp64"},
+        "roofline": roof_lb if dominant == "lb_step" else roof_mp,
+        "roofline_lb": roof_lb, "roofline_mp": roof_mp,
+        "lb_mlups": round(N * lb_steps * args.steps / (lb_ms / 1e3) / 1e6, 1),
+        "mp_mlups": round(N * mp_steps * args.steps / (mp_ms / 1e3) / 1e6, 1),
+        "deff": deffs,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(w, lb_steps, mp_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

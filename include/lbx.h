@@ -1,0 +1,179 @@
+/* lbx: MI355X (gfx950) lattice-Boltzmann + moment-propagation hot path of
+ * laboetie (maxlevesque/laboetie), as a C-ABI shared library (liblbx.so).
+ *
+ * The reference has no FFI: its hot path is a set of Fortran subroutines on
+ * module globals.  Each entry point below replaces one of them (cited), so a
+ * Fortran shim with the reference's unchanged interfaces can forward to it
+ * through iso_c_binding (see laboetie_amd/fortran).
+ *
+ * Conventions
+ *  - Arrays are host arrays in the reference's Fortran column-major layouts,
+ *    flattened (x fastest, N = lx*ly*lz):
+ *      scalar field   f[i + lx*(j + ly*k)]            n(lx,ly,lz)
+ *      population     n[node + N*l], l = 0..18         n(lx,ly,lz,19)
+ *      moment field   P[a + 3*node], a = 0..2          P(3,lx,ly,lz)
+ *    l = 0..18 stands for the reference's velocity index 1..19
+ *    (module_lbmodel.f90:217-237).
+ *  - nature: int8, 0 fluid, 1 solid (module_system.f90:32).
+ *  - Every function returns an int status (LBX_OK = 0); the reference's fatal
+ *    stops become status codes the caller maps back to the same messages.
+ *  - A context owns its device memory and one HIP stream; it is not
+ *    thread-safe (one caller at a time), like the reference's routines.
+ *  - Copies in and out are synchronous unless stated; state between calls
+ *    stays resident in HBM.
+ */
+#ifndef LBX_H
+#define LBX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define LBX_OK            0
+#define LBX_ERR_NEG_POP   1  /* "In equilibration, the population n(x,y,z,vel) < 0" equilibration.f90:248 */
+#define LBX_ERR_RESTPART  2  /* "somewhere restpart is negative" module_moment_propagation.f90:249,257 */
+#define LBX_ERR_TAU       3  /* "relaxation_time must be > 0.5" module_collision.f90:39 */
+#define LBX_ERR_NOMEM     4  /* device allocation failed / HBM admission refused */
+#define LBX_ERR_ARG       5  /* invalid argument (also the reference's input stops) */
+#define LBX_ERR_DEVICE    6  /* HIP runtime error or no device */
+#define LBX_ERR_STATE     7  /* call out of order (e.g. mp step before mp init) */
+
+/* precision forms */
+#define LBX_PREC_FP64        0  /* populations and arithmetic in FP64 (parity form) */
+#define LBX_PREC_FP32_SHIFT  1  /* populations stored as FP32 f_l - w_l*rho0, FP32 collision,
+                                   FP64 moments/reductions/VACF/D_eff */
+
+/* lbx_equilibrate exit reasons */
+#define LBX_EXIT_CONVERGED 1    /* second convergence, equilibration.f90:373-374 */
+#define LBX_EXIT_CAP       2    /* max_steps reached (extension: the reference has no cap) */
+
+typedef struct lbx_ctx lbx_ctx;
+
+/* Message of a status code (static string). */
+const char* lbx_status_string(int status);
+
+/* Library version, e.g. 100 for 1.0.0. */
+int lbx_version(void);
+
+/* D3Q19 tables exactly as module_lbmodel.f90:217-237,273-313 builds them:
+ * c[3*l+a], a0 = w, a1 = w/cs^2, a2 = w/(2 cs^4), inv (0-based). */
+int lbx_d3q19_tables(int32_t c[57], double a0[19], double a1[19], double a2[19], int32_t inv[19]);
+
+/* ------------------------------------------------------------- geometry
+ * Host-side builders of node%nature, replacing supercell_definition.f90:107-169
+ * for geometryLabel -1 (no solid), 0 (custom list, module_geometry.f90:380-426:
+ * solid_ijk holds npts 1-based (i,j,k) triples), 1 (slit, :158-166),
+ * 2 (cylinder along z, :253-277), 3 (BCC of touching spheres, :206-245).
+ * LBX_ERR_ARG on the reference's stops (shape checks, index out of range,
+ * all nodes solid). */
+int lbx_build_geometry(int32_t label, int32_t lx, int32_t ly, int32_t lz,
+                       const int32_t* solid_ijk, int64_t npts, int8_t* nature);
+
+/* detectInterfacialNodes, supercell_definition.f90:194-226. */
+int lbx_detect_interfacial(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, int8_t* interfacial);
+
+/* --------------------------------------------------------------- context */
+
+/* HBM bytes a context of this shape needs (admission control for batching
+ * and serving); with_mp adds the moment-propagation arrays. */
+int64_t lbx_footprint_bytes(int32_t lx, int32_t ly, int32_t lz, int32_t precision, int32_t with_mp);
+
+/* Create a context on the current HIP device holding one lattice: nature
+ * (N int8), relaxation time tau (must be >= 0.5, module_collision.f90:37-41).
+ * Allocates everything the LB phase needs; the MP arrays come with
+ * lbx_mp_init. */
+int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau,
+               int32_t precision, lbx_ctx** out);
+int lbx_destroy(lbx_ctx* ctx);
+
+/* Populations n = rho0*w_l at fluid nodes, 0 at solids; density rho0 at
+ * fluid, 0 at solid; j = 0 (init_simu.f90:31-39, equilibration.f90:59-80). */
+int lbx_init_rest(lbx_ctx* ctx, double rho0);
+
+/* Arbitrary state: n (post-streaming populations, (lx,ly,lz,19)), and the
+ * density and flux that the next collision uses (collide's arguments). */
+int lbx_set_state(lbx_ctx* ctx, const double* n, const double* rho,
+                  const double* jx, const double* jy, const double* jz);
+
+/* The equilibration time loop, equilibration.f90:143-491 (without the
+ * compensate_f_ext branch): per step collide, bounce-back, streaming,
+ * negativity check, density and flux, l2err = max|j - j_old|; stage 1 runs
+ * with no force until l2err <= target_error (t > 2), then fext acts at every
+ * fluid node; the run exits at the next convergence.  max_steps > 0 caps the
+ * number of steps (extension).  l2err_trace (may be NULL) receives the first
+ * trace_cap values of the per-step l2err (what l2err.dat holds).
+ * Returns LBX_ERR_NEG_POP with *steps_done = the failing step. */
+int lbx_equilibrate(lbx_ctx* ctx, double target_error, const double fext[3], int64_t max_steps,
+                    double* l2err_trace, int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason);
+
+/* State after the last step (equilibration.f90:551-555): n is the
+ * post-streaming population, rho = SUM(n,4), j = F/2 + sum n c.  Any pointer
+ * may be NULL. */
+int lbx_get_state(lbx_ctx* ctx, double* n, double* rho, double* jx, double* jy, double* jz);
+
+/* ------------------------------------------------- moment propagation
+ * moment_propagation::init (module_moment_propagation.f90:30-160) after
+ * update_tracer_population (drop_tracers.f90:85-105): rebuilds the tracer
+ * population from the resident (rho, j) and fext, computes Pstat, VACF(0)
+ * and P(r,1).  Tracers are neutral (tracer_z = 0, tracer_Ds = 0 are the only
+ * values the reference accepts).  Db > eps; ka, kd >= 0; adsorption is on
+ * when ka/kd > eps. */
+int lbx_mp_init(lbx_ctx* ctx, const double fext[3], double Db, double ka, double kd, double vacf0[3]);
+
+/* Same, from host density and flux fields (node%solventDensity/solventFlux)
+ * instead of the resident flow. */
+int lbx_mp_init_from_flow(lbx_ctx* ctx, const double* rho, const double* jx, const double* jy,
+                          const double* jz, const double fext[3], double Db, double ka, double kd,
+                          double vacf0[3]);
+
+/* nsteps calls of moment_propagation::propagate (module_moment_propagation.f90
+ * :164-289).  vacf_trace (may be NULL) receives VACF(it) for each step taken,
+ * 3 per step.  With stop_on_convergence, stops after the first step whose
+ * VACF passes the test of :284-288 and sets *converged.  LBX_ERR_RESTPART
+ * when a rest probability is below epsilon (the reference stops in the
+ * first propagate call). */
+int lbx_mp_run(lbx_ctx* ctx, int64_t nsteps, int32_t stop_on_convergence, double* vacf_trace,
+               int64_t* steps_done, int32_t* converged);
+
+/* Current P(now) and, with adsorption, Pa(now), (3,lx,ly,lz); either may be NULL. */
+int lbx_mp_get(lbx_ctx* ctx, double* P, double* Pa);
+
+/* Effective diffusion coefficient of the steps run so far (not in the
+ * reference: drop_tracers.f90:53-54 is commented out):
+ * D_a = VACF_a(0)/2 + sum_{t>=1} VACF_a(t), FP64 with compensated summation. */
+int lbx_mp_deff(lbx_ctx* ctx, double deff[3]);
+
+/* Release the MP arrays (deallocate_propagated_quantity,
+ * module_moment_propagation.f90:350-353). */
+int lbx_mp_free(lbx_ctx* ctx);
+
+/* ------------------------------------------------------- stateless call
+ * collide(n, density, jx, jy, jz, f_ext_x, f_ext_y, f_ext_z),
+ * module_collision.f90:15-130: one BGK + Guo-force collision of every fluid
+ * node of host arrays, on the device; n is updated in place. */
+int lbx_collide(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double* n,
+                const double* density, const double* jx, const double* jy, const double* jz,
+                const double* f_ext_x, const double* f_ext_y, const double* f_ext_z);
+
+/* ------------------------------------------------------------- timing
+ * Average device duration (ms) of the launches of one kernel family since
+ * the last reset, measured with HIP events on the context's stream around
+ * back-to-back launch runs.  which: 0 = LB step, 1 = MP step. */
+#define LBX_KERNEL_LB 0
+#define LBX_KERNEL_MP 1
+int lbx_timing_enable(lbx_ctx* ctx, int32_t on);
+int lbx_kernel_time(lbx_ctx* ctx, int32_t which, double* total_ms, int64_t* launches);
+
+/* Fluid-node count of the context (the MFLUPS denominator). */
+int64_t lbx_fluid_nodes(lbx_ctx* ctx);
+
+/* Block until the context's stream is idle. */
+int lbx_sync(lbx_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

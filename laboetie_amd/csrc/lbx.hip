@@ -1,0 +1,1132 @@
+// liblbx: MI355X (gfx950) kernels and C-ABI of the laboetie hot path.
+//
+// Device layout of one lattice (N = lx*ly*lz nodes, x fastest, planes of Np
+// elements, Np = N rounded up to 64):
+//   f[2][19][Np]  post-collision populations, ping-pong (step t reads
+//                 f[(t-1)&1], writes f[t&1])
+//   j[3][Np]      flux of the last completed step (the next step's j_old)
+//   rho[Np]       density (written at set-up and when a run ends)
+//   mask[N]       per-node word: bit l (1..18) = pull source r - c_l is solid,
+//                 bit 30 = interfacial, bit 31 = solid node
+// Moment propagation adds win[18][Np] (incoming scattering probabilities),
+// rem[Np], ustar[3][Np], P[2][3][Np] and, with adsorption, Pa[2][3][Np].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/lbx.h"
+#include "lbx_device.h"
+#include "lbx_tables.h"
+
+#pragma clang fp contract(off)
+
+using namespace lbx;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRing = 1024;          // l2err ring slots (>= 2 chunks)
+constexpr int kChunk = 256;          // LB steps launched between host checks
+constexpr int kMpChunk = 512;        // MP steps per host check
+constexpr unsigned long long kNone = ~0ull;
+
+struct Ctrl {
+    int32_t stop;                // LB: a kernel saw the previous step converge
+    int32_t restpart;            // MP: a rest probability < eps (set by mp prep)
+    int64_t exit_step;           // LB: that converged step
+    unsigned long long negpop;   // LB: first step with a negative population
+    int32_t mp_stop;             // MP: VACF convergence reached
+    uint32_t ticket;             // last-block reduction ticket
+    int64_t mp_conv_step;
+    double vacf0[3];
+    double deff_s[3], deff_c[3]; // Neumaier running sums of the VACF integral
+};
+
+__device__ __forceinline__ int32_t vload(const int32_t* p) { return *(const volatile int32_t*)p; }
+__device__ __forceinline__ unsigned long long vload(const unsigned long long* p) {
+    return *(const volatile unsigned long long*)p;
+}
+__device__ __forceinline__ double vload(const double* p) { return *(const volatile double*)p; }
+
+// ---------------------------------------------------------------- LB kernels
+
+struct StepArgs {
+    const double* __restrict__ fin;
+    double* __restrict__ fout;
+    double* __restrict__ j;
+    const uint32_t* __restrict__ mask;
+    unsigned long long* l2ring;
+    Ctrl* ctrl;
+    uint32_t lx, ly, lz, N;
+    uint64_t Np;
+    double A, B, C;
+    double fmx, fmy, fmz;   // F(t)/2, the force of step t in j(t) (equilibration.f90:293-295)
+    double fcx, fcy, fcz;   // F(t+1), the force of the collision that opens step t+1
+    int64_t t;
+    int64_t check_after;    // stop check on l2err(t-1) only when t-1 > check_after
+    double target;
+    int32_t write_l2;       // 0 for a redo of a step whose l2err is already known
+};
+
+// One time step, fused: pull-streaming with half-way bounce-back of the
+// post-collision populations of step t (equilibration.f90:204-243), the
+// negativity check (:248), density (:254) and flux (:293-300), the l2err
+// contribution (:339-344), then the collision that opens step t+1
+// (module_collision.f90:87-108).
+__global__ __launch_bounds__(kBlock) void k_lb_step(StepArgs a) {
+    if (vload(&a.ctrl->stop) || vload(&a.ctrl->negpop) != kNone) return;
+    if (a.t - 1 > a.check_after) {
+        const double prev = __longlong_as_double((long long)vload(&a.l2ring[(a.t - 1) % kRing]));
+        if (prev <= a.target) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                a.ctrl->exit_step = a.t - 1;
+                a.ctrl->stop = 1;
+            }
+            return;
+        }
+    }
+    if (a.write_l2 && blockIdx.x == 0 && threadIdx.x == 0) a.l2ring[(a.t + 1) % kRing] = 0ull;
+
+    const uint32_t r = xcd_tile() * kBlock + threadIdx.x;
+    double l2 = 0.0;
+    bool neg = false;
+    if (r < a.N) {
+        const uint32_t m = a.mask[r];
+        if (!(m & kSolidBit)) {
+            Nbr nb;
+            nb.init(r, a.lx, a.ly, a.lz);
+            double f[kQ];
+            static_for<0, kQ>([&](auto I) {
+                constexpr int L = I;
+                const uint32_t src = nb.at<-kCX[L], -kCY[L], -kCZ[L]>();
+                const bool bb = (m >> L) & 1u;
+                const uint64_t off = bb ? (uint64_t)kInv[L] * a.Np + r : (uint64_t)L * a.Np + src;
+                f[L] = a.fin[off];
+            });
+            static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < 0.0); });
+            double rho = f[0];
+            static_for<1, kQ>([&](auto I) { rho = rho + f[I]; });
+            double jx = a.fmx, jy = a.fmy, jz = a.fmz;
+            static_for<0, kQ>([&](auto I) {
+                constexpr int L = I;
+                if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(f[L]);
+                if constexpr (kCY[L] != 0) jy = jy + sgn<kCY[L]>(f[L]);
+                if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(f[L]);
+            });
+            double* jp = a.j + r;
+            if (a.write_l2) {
+                const double dx = fabs(jx - jp[0]), dy = fabs(jy - jp[a.Np]), dz = fabs(jz - jp[2 * a.Np]);
+                l2 = fmax(fmax(dx, dy), dz);
+            }
+            jp[0] = jx;
+            jp[a.Np] = jy;
+            jp[2 * a.Np] = jz;
+            const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
+            static_for<0, kQ>([&](auto I) {
+                constexpr int L = I;
+                a.fout[(uint64_t)L * a.Np + r] =
+                    collide_pop<L>(f[L], rho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy, a.fcz, a.A, a.B, a.C);
+            });
+        }
+    }
+    if (a.write_l2) {
+        l2 = wave_max(l2);
+        if ((threadIdx.x & 63) == 0 && l2 > 0.0)
+            atomicMax(&a.l2ring[a.t % kRing], (unsigned long long)__double_as_longlong(l2));
+    }
+    if (__ballot(neg) != 0ull && (threadIdx.x & 63) == 0) atomicMin(&a.ctrl->negpop, (unsigned long long)a.t);
+}
+
+// Streaming only: the post-streaming state f_s(t) of the step whose input is
+// fin, with its density; solids get n = 0, rho = 0, j = 0.  Writes the
+// reference-layout populations into nout (plane stride Np).
+__global__ __launch_bounds__(kBlock) void k_extract(const double* __restrict__ fin, double* __restrict__ nout,
+                                                    double* __restrict__ rho_out, double* __restrict__ j,
+                                                    const uint32_t* __restrict__ mask, uint32_t lx, uint32_t ly,
+                                                    uint32_t lz, uint32_t N, uint64_t Np) {
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= N) return;
+    const uint32_t m = mask[r];
+    if (m & kSolidBit) {
+        for (int l = 0; l < kQ; ++l) nout[(uint64_t)l * Np + r] = 0.0;
+        rho_out[r] = 0.0;
+        j[r] = 0.0;
+        j[Np + r] = 0.0;
+        j[2 * Np + r] = 0.0;
+        return;
+    }
+    Nbr nb;
+    nb.init(r, lx, ly, lz);
+    double rho = 0.0;
+    static_for<0, kQ>([&](auto I) {
+        constexpr int L = I;
+        const uint32_t src = nb.at<-kCX[L], -kCY[L], -kCZ[L]>();
+        const bool bb = (m >> L) & 1u;
+        const double v = fin[bb ? (uint64_t)kInv[L] * Np + r : (uint64_t)L * Np + src];
+        nout[(uint64_t)L * Np + r] = v;
+        rho = (L == 0) ? v : rho + v;
+    });
+    rho_out[r] = rho;
+}
+
+// collide() on whole fields (module_collision.f90:15-130): fluid nodes only.
+// F is per node (fx/fy/fz arrays) or uniform (arrays NULL).  Solid nodes are
+// copied (in place: untouched).
+__global__ __launch_bounds__(kBlock) void k_collide_field(const double* fin, double* fout,
+                                                          const double* __restrict__ rho,
+                                                          const double* __restrict__ j,
+                                                          const double* __restrict__ fxa,
+                                                          const double* __restrict__ fya,
+                                                          const double* __restrict__ fza,
+                                                          double fx0, double fy0, double fz0,
+                                                          const uint32_t* __restrict__ mask, uint32_t N,
+                                                          uint64_t Np, double A, double B, double C) {
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= N) return;
+    const bool solid = mask[r] & kSolidBit;
+    if (solid) {
+        if (fin != fout)
+            for (int l = 0; l < kQ; ++l) fout[(uint64_t)l * Np + r] = fin[(uint64_t)l * Np + r];
+        return;
+    }
+    const double d = rho[r], jx = j[r], jy = j[Np + r], jz = j[2 * Np + r];
+    const double fx = fxa ? fxa[r] : fx0, fy = fya ? fya[r] : fy0, fz = fza ? fza[r] : fz0;
+    const double ux = jx / d, uy = jy / d, uz = jz / d;
+    static_for<0, kQ>([&](auto I) {
+        constexpr int L = I;
+        const double v = fin[(uint64_t)L * Np + r];
+        fout[(uint64_t)L * Np + r] = collide_pop<L>(v, d, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
+    });
+}
+
+// Rest state, init_simu.f90:31-39: n = rho*a0 (rho = rho0 at fluid, 0 at solid).
+__global__ __launch_bounds__(kBlock) void k_init_rest(double* __restrict__ f, double* __restrict__ rho,
+                                                      double* __restrict__ j, const uint32_t* __restrict__ mask,
+                                                      uint32_t N, uint64_t Np, double rho0) {
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= N) return;
+    const double d = (mask[r] & kSolidBit) ? 0.0 : rho0;
+    static_for<0, kQ>([&](auto I) {
+        constexpr int L = I;
+        f[(uint64_t)L * Np + r] = d * a0(L);
+    });
+    rho[r] = d;
+    j[r] = 0.0;
+    j[Np + r] = 0.0;
+    j[2 * Np + r] = 0.0;
+}
+
+// ---------------------------------------------------------------- MP kernels
+
+// Last-block reduction of per-block triples into one triple, in block order.
+// Returns true in thread 0 of the last block, with the total in out[].
+__device__ __forceinline__ bool last_block_sum3(double v[3], double* partials, uint32_t* ticket,
+                                                double out[3], double* lds) {
+    __shared__ bool is_last;
+    block_sum3<kBlock>(v, lds);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x + 0] = v[0];
+        partials[3 * blockIdx.x + 1] = v[1];
+        partials[3 * blockIdx.x + 2] = v[2];
+        __threadfence();
+        is_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double s[3] = {0.0, 0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        for (int c = 0; c < 3; ++c) {
+            const double x = __hip_atomic_load(&partials[3 * b + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s[c] = (b == threadIdx.x) ? x : s[c] + x;
+        }
+    }
+    block_sum3<kBlock>(s, lds);
+    if (threadIdx.x == 0) {
+        out[0] = s[0];
+        out[1] = s[1];
+        out[2] = s[2];
+        *ticket = 0u;
+    }
+    return threadIdx.x == 0;
+}
+
+struct MpPrepArgs {
+    const double* __restrict__ rho;
+    const double* __restrict__ j;
+    const uint32_t* __restrict__ mask;
+    double* __restrict__ win;
+    double* __restrict__ rem;
+    double* __restrict__ ustar;
+    double* __restrict__ P0;
+    double* partials;
+    Ctrl* ctrl;
+    uint32_t lx, ly, lz, N;
+    uint64_t Np;
+    double fx, fy, fz;   // lb.in f_ext (drop_tracers.f90:92)
+    double lambda, bw, ka, eps;
+    int32_t ads;
+};
+
+// update_tracer_population (drop_tracers.f90:97-100) fused with
+// moment_propagation::init (module_moment_propagation.f90:105-137) and the
+// static part of propagate (:218-249): per fluid node, the scattering
+// probabilities s(r,l) = n_l/rho - w_l + lambda w_l / 2 towards fluid
+// neighbours, the rest probability rem (minus ka at adsorbing interfacial
+// nodes), u* = sum s c, the incoming probabilities win(r,l) = s(r+c_l, inv l),
+// P(r,1) and the VACF(0) contribution.
+__global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs a) {
+    __shared__ double lds[kBlock / 64 * 3];
+    const uint32_t r = xcd_tile() * kBlock + threadIdx.x;
+    double vp[3] = {0.0, 0.0, 0.0};
+    if (r < a.N) {
+        const uint32_t m = a.mask[r];
+        if (!(m & kSolidBit)) {
+            Nbr nb;
+            nb.init(r, a.lx, a.ly, a.lz);
+            const double rr = a.rho[r];
+            const double vx = a.j[r] + a.fx, vy = a.j[a.Np + r] + a.fy, vz = a.j[2 * a.Np + r] + a.fz;
+            double us[3] = {0.0, 0.0, 0.0};
+            double p1[3] = {0.0, 0.0, 0.0};
+            double remv = 1.0;
+            static_for<1, kQ>([&](auto I) {
+                constexpr int L = I;
+                constexpr int Li = kInv[L];
+                constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
+                // r + c_L is fluid iff the pull source of direction inv(L) is fluid
+                if (!((m >> Li) & 1u)) {
+                    const double nt = a0(L) * rr + a1(L) * dot3<cx, cy, cz>(vx, vy, vz);
+                    const double s = nt / rr - a0(L) + a.lambda * a0(L) * 0.5;
+                    remv = remv - s;
+                    if constexpr (cx != 0) us[0] = us[0] + sgn<cx>(s);
+                    if constexpr (cy != 0) us[1] = us[1] + sgn<cy>(s);
+                    if constexpr (cz != 0) us[2] = us[2] + sgn<cz>(s);
+                    if constexpr (cx != 0) vp[0] = vp[0] + a.bw * s;
+                    if constexpr (cy != 0) vp[1] = vp[1] + a.bw * s;
+                    if constexpr (cz != 0) vp[2] = vp[2] + a.bw * s;
+                    const uint32_t q = nb.at<cx, cy, cz>();
+                    const double rn = a.rho[q];
+                    const double wx = a.j[q] + a.fx, wy = a.j[a.Np + q] + a.fy, wz = a.j[2 * a.Np + q] + a.fz;
+                    const double ntn = a0(Li) * rn + a1(Li) * dot3<-cx, -cy, -cz>(wx, wy, wz);
+                    const double sp = ntn / rn - a0(Li) + a.lambda * a0(Li) * 0.5;
+                    a.win[(uint64_t)(L - 1) * a.Np + r] = sp;
+                    if constexpr (cx != 0) p1[0] = p1[0] + sgn<-cx>(sp) * a.bw;
+                    if constexpr (cy != 0) p1[1] = p1[1] + sgn<-cy>(sp) * a.bw;
+                    if constexpr (cz != 0) p1[2] = p1[2] + sgn<-cz>(sp) * a.bw;
+                } else {
+                    a.win[(uint64_t)(L - 1) * a.Np + r] = 0.0;
+                }
+            });
+            if (a.ads && (m & kInterfacialBit)) remv = remv - a.ka;
+            if (remv < a.eps) a.ctrl->restpart = 1;
+            a.rem[r] = remv;
+            for (int c = 0; c < 3; ++c) {
+                a.ustar[c * a.Np + r] = us[c];
+                a.P0[c * a.Np + r] = p1[c];
+            }
+        }
+    }
+    double tot[3];
+    if (last_block_sum3(vp, a.partials, &a.ctrl->ticket, tot, lds)) {
+        for (int c = 0; c < 3; ++c) {
+            a.ctrl->vacf0[c] = tot[c];
+            a.ctrl->deff_s[c] = 0.5 * tot[c];
+            a.ctrl->deff_c[c] = 0.0;
+        }
+    }
+}
+
+struct MpStepArgs {
+    const double* __restrict__ win;
+    const double* __restrict__ rem;
+    const double* __restrict__ ustar;
+    const uint32_t* __restrict__ mask;
+    const double* __restrict__ Pn;
+    double* __restrict__ Px;
+    const double* __restrict__ Pan;
+    double* __restrict__ Pax;
+    double* partials;
+    double* vtrace;   // 3 doubles for this step
+    Ctrl* ctrl;
+    uint32_t lx, ly, lz, N;
+    uint64_t Np;
+    double ka, kd, one_minus_kd;
+    int32_t ads;
+    int64_t it;
+    int32_t check_conv;
+    double lim1, lim2;
+};
+
+// One moment-propagation step, module_moment_propagation.f90:207-253:
+// P(r,t+1) = sum_l P(r+c_l,t) s(r+c_l, inv l) + rem(r) P(r,t) [+ kd Pa],
+// Pa(r,t+1) = (1-kd) Pa(r,t) + ka P(r,t) at adsorbing interfacial nodes,
+// VACF(t) = sum_r P(r,t) . u*(r).
+__global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs a) {
+    __shared__ double lds[kBlock / 64 * 3];
+    if (vload(&a.ctrl->mp_stop)) return;
+    const uint32_t r = xcd_tile() * kBlock + threadIdx.x;
+    double vp[3] = {0.0, 0.0, 0.0};
+    if (r < a.N) {
+        const uint32_t m = a.mask[r];
+        if (!(m & kSolidBit)) {
+            Nbr nb;
+            nb.init(r, a.lx, a.ly, a.lz);
+            double acc[3] = {0.0, 0.0, 0.0};
+            static_for<1, kQ>([&](auto I) {
+                constexpr int L = I;
+                if (!((m >> kInv[L]) & 1u)) {
+                    const double w = a.win[(uint64_t)(L - 1) * a.Np + r];
+                    const uint32_t q = nb.at<kCX[L], kCY[L], kCZ[L]>();
+                    acc[0] = acc[0] + a.Pn[q] * w;
+                    acc[1] = acc[1] + a.Pn[a.Np + q] * w;
+                    acc[2] = acc[2] + a.Pn[2 * a.Np + q] * w;
+                }
+            });
+            const double rm = a.rem[r];
+            double p[3];
+            for (int c = 0; c < 3; ++c) {
+                p[c] = a.Pn[c * a.Np + r];
+                vp[c] = p[c] * a.ustar[c * a.Np + r];
+            }
+            if (!a.ads || !(m & kInterfacialBit)) {
+                for (int c = 0; c < 3; ++c) a.Px[c * a.Np + r] = acc[c] + rm * p[c];
+            } else {
+                for (int c = 0; c < 3; ++c) {
+                    const double pa = a.Pan[c * a.Np + r];
+                    a.Px[c * a.Np + r] = (acc[c] + rm * p[c]) + pa * a.kd;
+                    a.Pax[c * a.Np + r] = pa * a.one_minus_kd + p[c] * a.ka;
+                }
+            }
+        }
+    }
+    double tot[3];
+    if (last_block_sum3(vp, a.partials, &a.ctrl->ticket, tot, lds)) {
+        for (int c = 0; c < 3; ++c) {
+            a.vtrace[c] = tot[c];
+            // Neumaier update of D = VACF(0)/2 + sum VACF(t)
+            const double s = a.ctrl->deff_s[c], x = tot[c], u = s + x;
+            a.ctrl->deff_c[c] += (fabs(s) >= fabs(x)) ? (s - u) + x : (x - u) + s;
+            a.ctrl->deff_s[c] = u;
+        }
+        if (a.check_conv && a.it > 2) {
+            bool conv = true;
+            for (int c = 0; c < 3; ++c) conv = conv && fabs(tot[c]) < a.lim1 && fabs(tot[c]) < a.lim2;
+            if (conv) {
+                a.ctrl->mp_conv_step = a.it;
+                a.ctrl->mp_stop = 1;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ====================================================================== host
+
+struct lbx_ctx {
+    int32_t lx = 0, ly = 0, lz = 0;
+    uint32_t N = 0;
+    uint64_t Np = 0;
+    int32_t prec = LBX_PREC_FP64;
+    double tau = 1.0, A = 0, B = 0, C = 0;
+    int64_t nf = 0, nfi = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double* f[2] = {nullptr, nullptr};
+    double* j = nullptr;
+    double* rho = nullptr;
+    uint32_t* mask = nullptr;
+    Ctrl* ctrl = nullptr;
+    unsigned long long* l2ring = nullptr;
+    int pre = 0;  // buffer holding the post-streaming state
+    // moment propagation
+    bool mp_ready = false;
+    double* win = nullptr;
+    double* rem = nullptr;
+    double* ustar = nullptr;
+    double* P[2] = {nullptr, nullptr};
+    double* Pa[2] = {nullptr, nullptr};
+    double* partials = nullptr;
+    double* vtrace = nullptr;
+    int mp_now = 0;
+    int64_t mp_it = 0;
+    double Db = 0, ka = 0, kd = 0, lambda = 0, bw = 0;
+    int32_t ads = 0;
+    bool mp_stopped = false;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double t_ms[2] = {0, 0};
+    int64_t t_n[2] = {0, 0};
+    std::vector<unsigned long long> l2host;
+};
+
+namespace {
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return LBX_OK;
+    return e == hipErrorOutOfMemory ? LBX_ERR_NOMEM : LBX_ERR_DEVICE;
+}
+int hip_status(int st) { return st; }  // an lbx status passes through
+
+#define LBX_TRY(expr)                            \
+    do {                                         \
+        int _st = hip_status(expr);              \
+        if (_st != LBX_OK) return _st;           \
+    } while (0)
+
+uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+template <typename T>
+int dalloc(T** p, uint64_t count) {
+    *p = nullptr;
+    if (count == 0) return LBX_OK;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return e == hipErrorOutOfMemory ? LBX_ERR_NOMEM : LBX_ERR_DEVICE;
+    }
+    return LBX_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_mp(lbx_ctx* c) {
+    dfree(c->win);
+    dfree(c->rem);
+    dfree(c->ustar);
+    dfree(c->P[0]);
+    dfree(c->P[1]);
+    dfree(c->Pa[0]);
+    dfree(c->Pa[1]);
+    dfree(c->partials);
+    dfree(c->vtrace);
+    c->mp_ready = false;
+}
+
+void free_all(lbx_ctx* c) {
+    free_mp(c);
+    dfree(c->f[0]);
+    dfree(c->f[1]);
+    dfree(c->j);
+    dfree(c->rho);
+    dfree(c->mask);
+    dfree(c->ctrl);
+    dfree(c->l2ring);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e), e = nullptr;
+    if (c->stream) (void)hipStreamDestroy(c->stream), c->stream = nullptr;
+}
+
+// A = 1-1/tau, B = 1/tau, C = 1-1/(2 tau), module_collision.f90:99-101
+void relaxation_coeffs(double tau, double* A, double* B, double* C) {
+    *A = 1.0 - 1.0 / tau;
+    *B = 1.0 / tau;
+    *C = 1.0 - 1.0 / (2.0 * tau);
+}
+
+std::vector<uint32_t> build_mask(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature,
+                                 const int8_t* interfacial) {
+    const uint64_t N = (uint64_t)lx * ly * lz;
+    std::vector<uint32_t> m(N, 0u);
+    auto wrap = [](int32_t i, int32_t n) { return i < 0 ? n - 1 : (i >= n ? 0 : i); };
+    for (int32_t k = 0; k < lz; ++k)
+        for (int32_t jj = 0; jj < ly; ++jj)
+            for (int32_t i = 0; i < lx; ++i) {
+                const uint64_t r = i + (uint64_t)lx * (jj + (uint64_t)ly * k);
+                uint32_t w = 0;
+                if (nature[r] != 0) w |= kSolidBit;
+                if (interfacial[r]) w |= kInterfacialBit;
+                for (int l = 1; l < kQ; ++l) {
+                    const uint64_t s = wrap(i - kCX[l], lx) + (uint64_t)lx * (wrap(jj - kCY[l], ly) +
+                                                                              (uint64_t)ly * wrap(k - kCZ[l], lz));
+                    if (nature[s] != 0) w |= 1u << l;
+                }
+                m[r] = w;
+            }
+    return m;
+}
+
+int reset_ctrl(lbx_ctx* c) {
+    Ctrl h{};
+    h.negpop = kNone;
+    return hip_status(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+}
+
+int read_ctrl(lbx_ctx* c, Ctrl* h) {
+    LBX_TRY(hipMemcpyAsync(h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+    return hip_status(hipStreamSynchronize(c->stream));
+}
+
+// Kernel timing: events on the context's stream bracket a run of back-to-back
+// launches (begin before the first, stop right after the last); the elapsed
+// time is collected once the host has synchronised anyway.
+void timer_begin(lbx_ctx* c) {
+    if (c->timing) (void)hipEventRecord(c->ev[0], c->stream);
+}
+void timer_stop(lbx_ctx* c) {
+    if (c->timing) (void)hipEventRecord(c->ev[1], c->stream);
+}
+void timer_end(lbx_ctx* c, int which, int64_t launches) {
+    if (!c->timing || launches <= 0) return;
+    (void)hipEventSynchronize(c->ev[1]);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
+        c->t_ms[which] += ms;
+        c->t_n[which] += launches;
+    }
+}
+
+StepArgs step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3], int64_t check_after,
+                   double target, int write_l2) {
+    StepArgs a;
+    a.fin = c->f[(t - 1) & 1];
+    a.fout = c->f[t & 1];
+    a.j = c->j;
+    a.mask = c->mask;
+    a.l2ring = c->l2ring;
+    a.ctrl = c->ctrl;
+    a.lx = c->lx;
+    a.ly = c->ly;
+    a.lz = c->lz;
+    a.N = c->N;
+    a.Np = c->Np;
+    a.A = c->A;
+    a.B = c->B;
+    a.C = c->C;
+    a.fmx = fm[0] / 2.0;
+    a.fmy = fm[1] / 2.0;
+    a.fmz = fm[2] / 2.0;
+    a.fcx = fc[0];
+    a.fcy = fc[1];
+    a.fcz = fc[2];
+    a.t = t;
+    a.check_after = check_after;
+    a.target = target;
+    a.write_l2 = write_l2;
+    return a;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C-ABI
+
+extern "C" {
+
+const char* lbx_status_string(int status) {
+    switch (status) {
+    case LBX_OK: return "ok";
+    case LBX_ERR_NEG_POP: return "In equilibration, the population n(x,y,z,vel) < 0";
+    case LBX_ERR_RESTPART: return "somewhere restpart is negative";
+    case LBX_ERR_TAU: return "relaxation_time must be > 0.5";
+    case LBX_ERR_NOMEM: return "device memory exhausted (HBM admission refused)";
+    case LBX_ERR_ARG: return "invalid argument";
+    case LBX_ERR_DEVICE: return "HIP runtime error";
+    case LBX_ERR_STATE: return "call out of order";
+    default: return "unknown status";
+    }
+}
+
+int lbx_version(void) { return 100; }
+
+int lbx_d3q19_tables(int32_t c[57], double a0o[19], double a1o[19], double a2o[19], int32_t inv[19]) {
+    for (int l = 0; l < kQ; ++l) {
+        if (c) {
+            c[3 * l] = kCX[l];
+            c[3 * l + 1] = kCY[l];
+            c[3 * l + 2] = kCZ[l];
+        }
+        if (a0o) a0o[l] = a0(l);
+        if (a1o) a1o[l] = a1(l);
+        if (a2o) a2o[l] = a2(l);
+        if (inv) inv[l] = kInv[l];
+    }
+    return LBX_OK;
+}
+
+int64_t lbx_footprint_bytes(int32_t lx, int32_t ly, int32_t lz, int32_t precision, int32_t with_mp) {
+    if (lx <= 0 || ly <= 0 || lz <= 0) return -1;
+    const uint64_t N = (uint64_t)lx * ly * lz, Np = (N + 63) / 64 * 64;
+    const uint64_t pop = precision == LBX_PREC_FP32_SHIFT ? 4 : 8;
+    uint64_t b = 2 * kQ * Np * pop + 4 * Np * 8 + N * 4 + sizeof(Ctrl) + kRing * 8;
+    if (with_mp) b += (18 + 1 + 3 + 12) * Np * 8 + nblocks(N) * 3 * 8 + kMpChunk * 3 * 8;
+    return (int64_t)b;
+}
+
+int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, int32_t precision,
+               lbx_ctx** out) {
+    if (!out) return LBX_ERR_ARG;
+    *out = nullptr;
+    if (lx <= 0 || ly <= 0 || lz <= 0 || !nature) return LBX_ERR_ARG;
+    if ((uint64_t)lx * ly * lz >= (1ull << 31)) return LBX_ERR_ARG;
+    if (precision != LBX_PREC_FP64) return LBX_ERR_ARG;
+    if (!(tau >= 0.5)) return LBX_ERR_TAU;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LBX_ERR_DEVICE;
+    lbx_ctx* c = new (std::nothrow) lbx_ctx();
+    if (!c) return LBX_ERR_NOMEM;
+    c->lx = lx;
+    c->ly = ly;
+    c->lz = lz;
+    c->N = (uint32_t)((uint64_t)lx * ly * lz);
+    c->Np = ((uint64_t)c->N + 63) / 64 * 64;
+    c->prec = precision;
+    c->tau = tau;
+    relaxation_coeffs(tau, &c->A, &c->B, &c->C);
+    (void)hipGetDevice(&c->device);
+    std::vector<int8_t> itf(c->N);
+    lbx_detect_interfacial(lx, ly, lz, nature, itf.data());
+    for (uint32_t r = 0; r < c->N; ++r) {
+        c->nf += nature[r] == 0;
+        c->nfi += nature[r] == 0 && itf[r];
+    }
+    if (c->nf == 0) {
+        delete c;
+        return LBX_ERR_ARG;
+    }
+    std::vector<uint32_t> m = build_mask(lx, ly, lz, nature, itf.data());
+    int st = LBX_OK;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = LBX_ERR_DEVICE;
+    if (!st) st = dalloc(&c->f[0], kQ * c->Np);
+    if (!st) st = dalloc(&c->f[1], kQ * c->Np);
+    if (!st) st = dalloc(&c->j, 3 * c->Np);
+    if (!st) st = dalloc(&c->rho, c->Np);
+    if (!st) st = dalloc(&c->mask, (uint64_t)c->N);
+    if (!st) st = dalloc(&c->ctrl, 1);
+    if (!st) st = dalloc(&c->l2ring, kRing);
+    if (!st && (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess))
+        st = LBX_ERR_DEVICE;
+    if (!st) st = hip_status(hipMemcpyAsync(c->mask, m.data(), c->N * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                            c->stream));
+    if (!st) st = hip_status(hipMemsetAsync(c->f[0], 0, kQ * c->Np * sizeof(double), c->stream));
+    if (!st) st = hip_status(hipMemsetAsync(c->f[1], 0, kQ * c->Np * sizeof(double), c->stream));
+    if (!st) st = hip_status(hipMemsetAsync(c->j, 0, 3 * c->Np * sizeof(double), c->stream));
+    if (!st) st = hip_status(hipMemsetAsync(c->rho, 0, c->Np * sizeof(double), c->stream));
+    if (!st) st = reset_ctrl(c);
+    if (!st) st = hip_status(hipStreamSynchronize(c->stream));
+    if (st) {
+        free_all(c);
+        delete c;
+        return st;
+    }
+    c->l2host.resize(kRing);
+    *out = c;
+    return LBX_OK;
+}
+
+int lbx_destroy(lbx_ctx* c) {
+    if (!c) return LBX_ERR_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return LBX_OK;
+}
+
+int lbx_init_rest(lbx_ctx* c, double rho0) {
+    if (!c) return LBX_ERR_ARG;
+    hipLaunchKernelGGL(k_init_rest, dim3(nblocks(c->N)), dim3(kBlock), 0, c->stream, c->f[0], c->rho, c->j,
+                       c->mask, c->N, c->Np, rho0);
+    c->pre = 0;
+    LBX_TRY(hipGetLastError());
+    return hip_status(hipStreamSynchronize(c->stream));
+}
+
+int lbx_set_state(lbx_ctx* c, const double* n, const double* rho, const double* jx, const double* jy,
+                  const double* jz) {
+    if (!c || !n || !rho || !jx || !jy || !jz) return LBX_ERR_ARG;
+    const size_t plane = (size_t)c->N * sizeof(double), pitch = (size_t)c->Np * sizeof(double);
+    LBX_TRY(hipMemcpy2DAsync(c->f[0], pitch, n, plane, plane, kQ, hipMemcpyHostToDevice, c->stream));
+    LBX_TRY(hipMemcpyAsync(c->rho, rho, plane, hipMemcpyHostToDevice, c->stream));
+    LBX_TRY(hipMemcpyAsync(c->j, jx, plane, hipMemcpyHostToDevice, c->stream));
+    LBX_TRY(hipMemcpyAsync(c->j + c->Np, jy, plane, hipMemcpyHostToDevice, c->stream));
+    LBX_TRY(hipMemcpyAsync(c->j + 2 * c->Np, jz, plane, hipMemcpyHostToDevice, c->stream));
+    c->pre = 0;
+    return hip_status(hipStreamSynchronize(c->stream));
+}
+
+int lbx_get_state(lbx_ctx* c, double* n, double* rho, double* jx, double* jy, double* jz) {
+    if (!c) return LBX_ERR_ARG;
+    const size_t plane = (size_t)c->N * sizeof(double), pitch = (size_t)c->Np * sizeof(double);
+    if (n) LBX_TRY(hipMemcpy2DAsync(n, plane, c->f[c->pre], pitch, plane, kQ, hipMemcpyDeviceToHost, c->stream));
+    if (rho) LBX_TRY(hipMemcpyAsync(rho, c->rho, plane, hipMemcpyDeviceToHost, c->stream));
+    if (jx) LBX_TRY(hipMemcpyAsync(jx, c->j, plane, hipMemcpyDeviceToHost, c->stream));
+    if (jy) LBX_TRY(hipMemcpyAsync(jy, c->j + c->Np, plane, hipMemcpyDeviceToHost, c->stream));
+    if (jz) LBX_TRY(hipMemcpyAsync(jz, c->j + 2 * c->Np, plane, hipMemcpyDeviceToHost, c->stream));
+    return hip_status(hipStreamSynchronize(c->stream));
+}
+
+int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64_t max_steps,
+                    double* l2err_trace, int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason) {
+    if (!c || !fext) return LBX_ERR_ARG;
+    if (steps_done) *steps_done = 0;
+    if (exit_reason) *exit_reason = 0;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    const double F[3] = {fext[0], fext[1], fext[2]};
+    // opening collision of step 1 (stage 1: no force), in place into f[0]
+    hipLaunchKernelGGL(k_collide_field, dim3(nblocks(c->N)), dim3(kBlock), 0, c->stream, c->f[c->pre], c->f[0],
+                       c->rho, c->j, nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, c->mask, c->N, c->Np, c->A, c->B,
+                       c->C);
+    LBX_TRY(hipGetLastError());
+    LBX_TRY(reset_ctrl(c));
+    LBX_TRY(hipMemsetAsync(c->l2ring, 0, kRing * sizeof(unsigned long long), c->stream));
+
+    int stage = 1;
+    int64_t t = 0;            // last completed step
+    int64_t check_after = 2;  // stop tests apply to steps > check_after (t > 2, equilibration.f90:346)
+    int status = LBX_OK;
+    int32_t reason = 0;
+    const unsigned gb = nblocks(c->N);
+    for (;;) {
+        int64_t S = kChunk;
+        if (max_steps > 0) S = std::min<int64_t>(S, max_steps - t);
+        if (S <= 0) {
+            reason = LBX_EXIT_CAP;
+            break;
+        }
+        const double* Fs = stage == 1 ? zero : F;
+        timer_begin(c);
+        for (int64_t k = 1; k <= S; ++k) {
+            StepArgs a = step_args(c, t + k, Fs, Fs, check_after, target_error, 1);
+            hipLaunchKernelGGL(k_lb_step, dim3(gb), dim3(kBlock), 0, c->stream, a);
+        }
+        timer_stop(c);
+        LBX_TRY(hipGetLastError());
+        Ctrl h;
+        LBX_TRY(hipMemcpyAsync(c->l2host.data(), c->l2ring, kRing * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, c->stream));
+        LBX_TRY(read_ctrl(c, &h));
+        int64_t last = t + S;  // steps actually executed in this chunk
+        if (h.negpop != kNone) last = std::min<int64_t>(last, (int64_t)h.negpop);
+        if (h.stop) last = std::min<int64_t>(last, h.exit_step);
+        timer_end(c, LBX_KERNEL_LB, last - t);
+        // l2err of the steps that completed
+        const int64_t last_l2 = (h.negpop != kNone && (int64_t)h.negpop <= last) ? last - 1 : last;
+        if (l2err_trace)
+            for (int64_t s = t + 1; s <= last_l2 && s <= trace_cap; ++s)
+                l2err_trace[s - 1] = __builtin_bit_cast(double, c->l2host[s % kRing]);
+        if (h.negpop != kNone) {
+            status = LBX_ERR_NEG_POP;
+            t = (int64_t)h.negpop;
+            break;
+        }
+        if (h.stop) {
+            const int64_t e = h.exit_step;
+            if (stage == 1) {
+                // the first convergence switches the force on at the end of step e
+                // (equilibration.f90:377-386): redo the collision that opened e+1
+                StepArgs a = step_args(c, e, zero, F, INT64_MAX, target_error, 0);
+                Ctrl clear{};
+                clear.negpop = kNone;
+                LBX_TRY(hipMemcpyAsync(c->ctrl, &clear, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+                hipLaunchKernelGGL(k_lb_step, dim3(gb), dim3(kBlock), 0, c->stream, a);
+                LBX_TRY(hipGetLastError());
+                stage = 2;
+                t = e;
+                check_after = e;  // l2err(e) belongs to stage 1
+                continue;
+            }
+            t = e;
+            reason = LBX_EXIT_CONVERGED;
+            break;
+        }
+        t += S;
+        if (max_steps > 0 && t >= max_steps) {
+            // a convergence at the capped step is still reported as such
+            const double l2 = __builtin_bit_cast(double, c->l2host[t % kRing]);
+            reason = (stage == 2 && t > check_after && l2 <= target_error) ? LBX_EXIT_CONVERGED : LBX_EXIT_CAP;
+            break;
+        }
+    }
+    if (steps_done) *steps_done = t;
+    if (exit_reason) *exit_reason = reason;
+    if (t > 0) {
+        // post-streaming state of step t into the other buffer (f[t&1] holds
+        // the unused opening collision of step t+1), density alongside
+        const int src = (t - 1) & 1, dst = t & 1;
+        hipLaunchKernelGGL(k_extract, dim3(gb), dim3(kBlock), 0, c->stream, c->f[src], c->f[dst], c->rho, c->j,
+                           c->mask, c->lx, c->ly, c->lz, c->N, c->Np);
+        LBX_TRY(hipGetLastError());
+        c->pre = dst;
+    }
+    LBX_TRY(hipStreamSynchronize(c->stream));
+    return status;
+}
+
+static int mp_setup(lbx_ctx* c, const double fext[3], double Db, double ka, double kd, double vacf0[3]) {
+    const double eps = DBL_EPSILON;
+    if (ka < -eps || kd < -eps) return LBX_ERR_ARG;  // module_moment_propagation.f90:43-44
+    if (Db <= eps) return LBX_ERR_ARG;                // drop_tracers.f90:89
+    free_mp(c);
+    int st = LBX_OK;
+    const uint32_t gb = nblocks(c->N);
+    if (!st) st = dalloc(&c->win, 18 * c->Np);
+    if (!st) st = dalloc(&c->rem, c->Np);
+    if (!st) st = dalloc(&c->ustar, 3 * c->Np);
+    if (!st) st = dalloc(&c->P[0], 3 * c->Np);
+    if (!st) st = dalloc(&c->P[1], 3 * c->Np);
+    if (!st) st = dalloc(&c->partials, 3 * (uint64_t)gb);
+    if (!st) st = dalloc(&c->vtrace, 3 * (uint64_t)kMpChunk);
+    const double K = (std::fabs(kd) <= eps) ? 0.0 : ka / kd;  // :46-50
+    c->ads = std::fabs(K) > eps;                                // :52-56
+    if (!st && c->ads) st = dalloc(&c->Pa[0], 3 * c->Np);
+    if (!st && c->ads) st = dalloc(&c->Pa[1], 3 * c->Np);
+    if (st) {
+        free_mp(c);
+        return st;
+    }
+    for (int b = 0; b < 2; ++b) {
+        LBX_TRY(hipMemsetAsync(c->P[b], 0, 3 * c->Np * sizeof(double), c->stream));
+        if (c->ads) LBX_TRY(hipMemsetAsync(c->Pa[b], 0, 3 * c->Np * sizeof(double), c->stream));
+    }
+    LBX_TRY(reset_ctrl(c));
+    c->Db = Db;
+    c->ka = ka;
+    c->kd = kd;
+    c->lambda = 4.0 * Db / kKbt;                               // calc_lambda, :331-337
+    const double Pstat = (double)c->nf + K * (double)c->nfi;   // :97
+    c->bw = 1.0 / Pstat;
+    MpPrepArgs a;
+    a.rho = c->rho;
+    a.j = c->j;
+    a.mask = c->mask;
+    a.win = c->win;
+    a.rem = c->rem;
+    a.ustar = c->ustar;
+    a.P0 = c->P[0];
+    a.partials = c->partials;
+    a.ctrl = c->ctrl;
+    a.lx = c->lx;
+    a.ly = c->ly;
+    a.lz = c->lz;
+    a.N = c->N;
+    a.Np = c->Np;
+    a.fx = fext[0];
+    a.fy = fext[1];
+    a.fz = fext[2];
+    a.lambda = c->lambda;
+    a.bw = c->bw;
+    a.ka = ka;
+    a.eps = eps;
+    a.ads = c->ads;
+    hipLaunchKernelGGL(k_mp_prep, dim3(gb), dim3(kBlock), 0, c->stream, a);
+    LBX_TRY(hipGetLastError());
+    Ctrl h;
+    LBX_TRY(read_ctrl(c, &h));
+    if (vacf0)
+        for (int k = 0; k < 3; ++k) vacf0[k] = h.vacf0[k];
+    c->mp_ready = true;
+    c->mp_now = 0;
+    c->mp_it = 0;
+    c->mp_stopped = false;
+    return LBX_OK;
+}
+
+int lbx_mp_init(lbx_ctx* c, const double fext[3], double Db, double ka, double kd, double vacf0[3]) {
+    if (!c || !fext) return LBX_ERR_ARG;
+    return mp_setup(c, fext, Db, ka, kd, vacf0);
+}
+
+int lbx_mp_init_from_flow(lbx_ctx* c, const double* rho, const double* jx, const double* jy, const double* jz,
+                          const double fext[3], double Db, double ka, double kd, double vacf0[3]) {
+    if (!c || !rho || !jx || !jy || !jz || !fext) return LBX_ERR_ARG;
+    const size_t plane = (size_t)c->N * sizeof(double);
+    LBX_TRY(hipMemcpyAsync(c->rho, rho, plane, hipMemcpyHostToDevice, c->stream));
+    LBX_TRY(hipMemcpyAsync(c->j, jx, plane, hipMemcpyHostToDevice, c->stream));
+    LBX_TRY(hipMemcpyAsync(c->j + c->Np, jy, plane, hipMemcpyHostToDevice, c->stream));
+    LBX_TRY(hipMemcpyAsync(c->j + 2 * c->Np, jz, plane, hipMemcpyHostToDevice, c->stream));
+    return mp_setup(c, fext, Db, ka, kd, vacf0);
+}
+
+int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* vacf_trace, int64_t* steps_done,
+               int32_t* converged) {
+    if (!c || nsteps < 0) return LBX_ERR_ARG;
+    if (steps_done) *steps_done = 0;
+    if (converged) *converged = 0;
+    if (!c->mp_ready) return LBX_ERR_STATE;
+    Ctrl h;
+    LBX_TRY(read_ctrl(c, &h));
+    if (h.restpart) return LBX_ERR_RESTPART;  // the reference stops in the first propagate call
+    if (c->mp_stopped) return LBX_OK;
+    const uint32_t gb = nblocks(c->N);
+    const double lim1 = 1.0 / (2.0 * (double)c->lx * (double)c->ly * (double)c->lz / c->Db);  // :284
+    const double lim2 = (double)1.e-12f;  // a default-real literal in the reference
+    int64_t done = 0;
+    std::vector<double> vt(3 * kMpChunk);
+    while (done < nsteps) {
+        const int64_t S = std::min<int64_t>(kMpChunk, nsteps - done);
+        timer_begin(c);
+        for (int64_t k = 0; k < S; ++k) {
+            const int now = c->mp_now ^ (int)(k & 1);
+            MpStepArgs a;
+            a.win = c->win;
+            a.rem = c->rem;
+            a.ustar = c->ustar;
+            a.mask = c->mask;
+            a.Pn = c->P[now];
+            a.Px = c->P[now ^ 1];
+            a.Pan = c->ads ? c->Pa[now] : nullptr;
+            a.Pax = c->ads ? c->Pa[now ^ 1] : nullptr;
+            a.partials = c->partials;
+            a.vtrace = c->vtrace + 3 * k;
+            a.ctrl = c->ctrl;
+            a.lx = c->lx;
+            a.ly = c->ly;
+            a.lz = c->lz;
+            a.N = c->N;
+            a.Np = c->Np;
+            a.ka = c->ka;
+            a.kd = c->kd;
+            a.one_minus_kd = 1.0 - c->kd;
+            a.ads = c->ads;
+            a.it = c->mp_it + k + 1;
+            a.check_conv = stop_on_convergence;
+            a.lim1 = lim1;
+            a.lim2 = lim2;
+            hipLaunchKernelGGL(k_mp_step, dim3(gb), dim3(kBlock), 0, c->stream, a);
+        }
+        timer_stop(c);
+        LBX_TRY(hipGetLastError());
+        LBX_TRY(hipMemcpyAsync(vt.data(), c->vtrace, 3 * S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        LBX_TRY(read_ctrl(c, &h));
+        int64_t ran = S;
+        if (h.mp_stop) ran = std::min<int64_t>(S, h.mp_conv_step - c->mp_it);
+        timer_end(c, LBX_KERNEL_MP, ran);
+        if (vacf_trace) std::memcpy(vacf_trace + 3 * done, vt.data(), 3 * ran * sizeof(double));
+        c->mp_it += ran;
+        if (ran & 1) c->mp_now ^= 1;
+        done += ran;
+        if (h.mp_stop) {
+            c->mp_stopped = true;
+            if (converged) *converged = 1;
+            break;
+        }
+    }
+    if (steps_done) *steps_done = done;
+    return LBX_OK;
+}
+
+int lbx_mp_get(lbx_ctx* c, double* P, double* Pa) {
+    if (!c) return LBX_ERR_ARG;
+    if (!c->mp_ready) return LBX_ERR_STATE;
+    std::vector<double> tmp(3 * (size_t)c->Np);
+    auto fetch = [&](const double* src, double* dst) -> int {
+        LBX_TRY(hipMemcpyAsync(tmp.data(), src, 3 * c->Np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        LBX_TRY(hipStreamSynchronize(c->stream));
+        for (uint32_t r = 0; r < c->N; ++r)
+            for (int a = 0; a < 3; ++a) dst[3 * (size_t)r + a] = tmp[a * c->Np + r];
+        return LBX_OK;
+    };
+    if (P) {
+        int st = fetch(c->P[c->mp_now], P);
+        if (st) return st;
+    }
+    if (Pa) {
+        if (c->ads) {
+            int st = fetch(c->Pa[c->mp_now], Pa);
+            if (st) return st;
+        } else {
+            std::memset(Pa, 0, 3 * (size_t)c->N * sizeof(double));
+        }
+    }
+    return LBX_OK;
+}
+
+int lbx_mp_deff(lbx_ctx* c, double deff[3]) {
+    if (!c || !deff) return LBX_ERR_ARG;
+    if (!c->mp_ready) return LBX_ERR_STATE;
+    Ctrl h;
+    LBX_TRY(read_ctrl(c, &h));
+    for (int a = 0; a < 3; ++a) deff[a] = h.deff_s[a] + h.deff_c[a];
+    return LBX_OK;
+}
+
+int lbx_mp_free(lbx_ctx* c) {
+    if (!c) return LBX_ERR_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    free_mp(c);
+    return LBX_OK;
+}
+
+int lbx_collide(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double* n,
+                const double* density, const double* jx, const double* jy, const double* jz,
+                const double* f_ext_x, const double* f_ext_y, const double* f_ext_z) {
+    if (lx <= 0 || ly <= 0 || lz <= 0 || !nature || !n || !density || !jx || !jy || !jz || !f_ext_x ||
+        !f_ext_y || !f_ext_z)
+        return LBX_ERR_ARG;
+    if (!(tau >= 0.5)) return LBX_ERR_TAU;
+    const uint64_t N = (uint64_t)lx * ly * lz;
+    if (N >= (1ull << 31)) return LBX_ERR_ARG;
+    std::vector<uint32_t> m(N);
+    for (uint64_t r = 0; r < N; ++r) m[r] = nature[r] ? kSolidBit : 0u;
+    double *dn = nullptr, *dr = nullptr, *dj = nullptr, *dfx = nullptr;
+    uint32_t* dm = nullptr;
+    int st = dalloc(&dn, kQ * N);
+    if (!st) st = dalloc(&dr, N);
+    if (!st) st = dalloc(&dj, 3 * N);
+    if (!st) st = dalloc(&dfx, 3 * N);
+    if (!st) st = dalloc(&dm, N);
+    hipStream_t s = nullptr;
+    if (!st) st = hip_status(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const size_t b = N * sizeof(double);
+    if (!st) st = hip_status(hipMemcpyAsync(dn, n, kQ * b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dr, density, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dj, jx, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dj + N, jy, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dj + 2 * N, jz, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dfx, f_ext_x, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dfx + N, f_ext_y, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dfx + 2 * N, f_ext_z, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dm, m.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (!st) {
+        double A, B, C;
+        relaxation_coeffs(tau, &A, &B, &C);
+        hipLaunchKernelGGL(k_collide_field, dim3(nblocks(N)), dim3(kBlock), 0, s, dn, dn, dr, dj, dfx, dfx + N,
+                           dfx + 2 * N, 0.0, 0.0, 0.0, dm, (uint32_t)N, N, A, B, C);
+        st = hip_status(hipGetLastError());
+    }
+    if (!st) st = hip_status(hipMemcpyAsync(n, dn, kQ * b, hipMemcpyDeviceToHost, s));
+    if (!st) st = hip_status(hipStreamSynchronize(s));
+    if (s) (void)hipStreamDestroy(s);
+    dfree(dn);
+    dfree(dr);
+    dfree(dj);
+    dfree(dfx);
+    dfree(dm);
+    return st;
+}
+
+int lbx_timing_enable(lbx_ctx* c, int32_t on) {
+    if (!c) return LBX_ERR_ARG;
+    c->timing = on != 0;
+    c->t_ms[0] = c->t_ms[1] = 0.0;
+    c->t_n[0] = c->t_n[1] = 0;
+    return LBX_OK;
+}
+
+int lbx_kernel_time(lbx_ctx* c, int32_t which, double* total_ms, int64_t* launches) {
+    if (!c || which < 0 || which > 1) return LBX_ERR_ARG;
+    if (total_ms) *total_ms = c->t_ms[which];
+    if (launches) *launches = c->t_n[which];
+    return LBX_OK;
+}
+
+int64_t lbx_fluid_nodes(lbx_ctx* c) { return c ? c->nf : -1; }
+
+int lbx_sync(lbx_ctx* c) {
+    if (!c) return LBX_ERR_ARG;
+    return hip_status(hipStreamSynchronize(c->stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// Device-side building blocks of liblbx (gfx950): D3Q19 arithmetic in the
+// reference's evaluation order, neighbour addressing, wave/block reductions.
+//
+// Bit-for-bit parity with the reference relies on three things:
+//  * the translation unit is built with -ffp-contract=off (no FMA fusion),
+//    like flang's x86-64 code;
+//  * every sum and product is written in the reference's left-to-right
+//    order (Fortran a+b+c is (a+b)+c, a*b*c is (a*b)*c);
+//  * terms whose velocity factor is 0 are dropped and factors of +-1 become
+//    a sign.  x*(+-1) is exact, and adding x*0 = +-0 to a sum leaves its value
+//    unchanged (only the sign of an exact zero could differ, which no later
+//    operation here observes: divisions are by the density, comparisons are
+//    n < 0 and maxima of absolute values).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "lbx_tables.h"
+
+#pragma clang fp contract(off)
+
+namespace lbx {
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+template <int S>
+__device__ __forceinline__ double sgn(double x) {
+    static_assert(S == 1 || S == -1, "sign factor");
+    if constexpr (S > 0) return x; else return -x;
+}
+
+// (a*x + b*y) + c*z for a, b, c in {-1, 0, 1}, zero terms dropped.
+template <int A, int B, int C>
+__device__ __forceinline__ double dot3(double x, double y, double z) {
+    if constexpr (A != 0) {
+        double s = sgn<A>(x);
+        if constexpr (B != 0) s = s + sgn<B>(y);
+        if constexpr (C != 0) s = s + sgn<C>(z);
+        return s;
+    } else if constexpr (B != 0) {
+        double s = sgn<B>(y);
+        if constexpr (C != 0) s = s + sgn<C>(z);
+        return s;
+    } else if constexpr (C != 0) {
+        return sgn<C>(z);
+    } else {
+        return 0.0;
+    }
+}
+
+// One population of the BGK + Guo-force collision, module_collision.f90:90-104:
+//   neq = a0 rho + a1 (c.j) + a2 sum_ab j_a u_b (c_a c_b - cs^2 delta_ab)
+//   n'  = (1-1/tau) n + (1/tau) neq
+//         + (1-1/(2 tau)) [a1 ((c-u).F) + 2 a2 (c.u)(c.F)]
+// A = 1-1/tau, B = 1/tau, C = 1-1/(2 tau) are computed on the host with the
+// reference's expressions.
+template <int L>
+__device__ __forceinline__ double collide_pop(double f, double rho, double jx, double jy, double jz,
+                                              double ux, double uy, double uz, double fx, double fy,
+                                              double fz, double A, double B, double C) {
+    constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
+    constexpr double w0 = a0(L), w1 = a1(L), w2 = a2(L);
+    constexpr double Kx = (double)(cx * cx) - kCsq;
+    constexpr double Ky = (double)(cy * cy) - kCsq;
+    constexpr double Kz = (double)(cz * cz) - kCsq;
+    double q = (jx * ux) * Kx;
+    if constexpr (cx * cy != 0) q = q + sgn<cx * cy>(jx * uy);
+    if constexpr (cx * cz != 0) q = q + sgn<cx * cz>(jx * uz);
+    if constexpr (cy * cx != 0) q = q + sgn<cy * cx>(jy * ux);
+    q = q + (jy * uy) * Ky;
+    if constexpr (cy * cz != 0) q = q + sgn<cy * cz>(jy * uz);
+    if constexpr (cz * cx != 0) q = q + sgn<cz * cx>(jz * ux);
+    if constexpr (cz * cy != 0) q = q + sgn<cz * cy>(jz * uy);
+    q = q + (jz * uz) * Kz;
+    double neq = w0 * rho;
+    if constexpr (L != 0) neq = neq + w1 * dot3<cx, cy, cz>(jx, jy, jz);
+    neq = neq + w2 * q;
+    double g = w1 * ((((double)cx - ux) * fx + ((double)cy - uy) * fy) + ((double)cz - uz) * fz);
+    if constexpr (L != 0) g = g + ((2.0 * w2) * dot3<cx, cy, cz>(ux, uy, uz)) * dot3<cx, cy, cz>(fx, fy, fz);
+    return (A * f + B * neq) + C * g;
+}
+
+// Periodic neighbour offsets of a node: X[1+d] is the x index of x+d, and Y,
+// Z the row / plane offsets of y+d, z+d (module_system.f90:99-112: only a
+// +-1 step wraps; an axis of length 1 wraps onto itself).
+struct Nbr {
+    uint32_t X[3], Y[3], Z[3];
+    __device__ __forceinline__ void init(uint32_t r, uint32_t lx, uint32_t ly, uint32_t lz) {
+        const uint32_t q = r / lx;
+        const uint32_t x = r - q * lx;
+        const uint32_t z = q / ly;
+        const uint32_t y = q - z * ly;
+        const uint32_t lxy = lx * ly;
+        X[0] = x == 0 ? lx - 1 : x - 1;
+        X[1] = x;
+        X[2] = x + 1 == lx ? 0 : x + 1;
+        Y[0] = (y == 0 ? ly - 1 : y - 1) * lx;
+        Y[1] = y * lx;
+        Y[2] = (y + 1 == ly ? 0 : y + 1) * lx;
+        Z[0] = (z == 0 ? lz - 1 : z - 1) * lxy;
+        Z[1] = z * lxy;
+        Z[2] = (z + 1 == lz ? 0 : z + 1) * lxy;
+    }
+    // index of r + d
+    template <int dx, int dy, int dz>
+    __device__ __forceinline__ uint32_t at() const {
+        return X[1 + dx] + Y[1 + dy] + Z[1 + dz];
+    }
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give consecutive
+// logical tiles to the same XCD so that neighbouring tiles share its L2.
+__device__ __forceinline__ uint32_t xcd_tile() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t x = b & 7u, i = b >> 3, per = nb >> 3, rem = nb & 7u;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
+    return v;  // lane 0 holds the sum, fixed tree
+}
+
+// Deterministic block sum of 3 values; the result is valid in thread 0.
+template <int BLOCK>
+__device__ __forceinline__ void block_sum3(double v[3], double* lds /* [BLOCK/64*3] */) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int a = 0; a < 3; ++a) {
+        double s = wave_sum(v[a]);
+        if (lane == 0) lds[w * 3 + a] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; ++a) {
+            double s = lds[a];
+            for (int k = 1; k < BLOCK / 64; ++k) s = s + lds[k * 3 + a];
+            v[a] = s;
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace lbx

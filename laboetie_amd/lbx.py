@@ -1,0 +1,228 @@
+"""ctypes binding of liblbx.so, the HIP library behind include/lbx.h.
+
+Host-side mirror of the reference interface for the hot path: `Lattice`
+wraps one device-resident lattice (an lbx_ctx) with the operations the
+reference's Fortran routines perform (equilibration, moment propagation
+init/propagate, collide), returning numpy arrays in the reference's
+Fortran-order layouts.  The library must be built (make -C
+laboetie_amd/csrc); there is no fallback: if liblbx.so is missing or no HIP
+device is present, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbx.so")
+
+OK, ERR_NEG_POP, ERR_RESTPART, ERR_TAU, ERR_NOMEM, ERR_ARG, ERR_DEVICE, ERR_STATE = range(8)
+PREC_FP64, PREC_FP32_SHIFT = 0, 1
+EXIT_CONVERGED, EXIT_CAP = 1, 2
+KERNEL_LB, KERNEL_MP = 0, 1
+
+_p = ctypes.c_void_p
+_i32, _i64, _f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+_lib = None
+
+# every exported symbol and its (restype, argtypes)
+SIGNATURES = {
+    "lbx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "lbx_version": (ctypes.c_int, []),
+    "lbx_d3q19_tables": (ctypes.c_int, [_p] * 5),
+    "lbx_build_geometry": (ctypes.c_int, [_i32] * 4 + [_p, _i64, _p]),
+    "lbx_detect_interfacial": (ctypes.c_int, [_i32] * 3 + [_p, _p]),
+    "lbx_footprint_bytes": (_i64, [_i32] * 5),
+    "lbx_create": (ctypes.c_int, [_i32] * 3 + [_p, _f64, _i32, _p]),
+    "lbx_destroy": (ctypes.c_int, [_p]),
+    "lbx_init_rest": (ctypes.c_int, [_p, _f64]),
+    "lbx_set_state": (ctypes.c_int, [_p] * 6),
+    "lbx_equilibrate": (ctypes.c_int, [_p, _f64, _p, _i64, _p, _i64, _p, _p]),
+    "lbx_get_state": (ctypes.c_int, [_p] * 6),
+    "lbx_mp_init": (ctypes.c_int, [_p, _p, _f64, _f64, _f64, _p]),
+    "lbx_mp_init_from_flow": (ctypes.c_int, [_p] * 6 + [_f64] * 3 + [_p]),
+    "lbx_mp_run": (ctypes.c_int, [_p, _i64, _i32, _p, _p, _p]),
+    "lbx_mp_get": (ctypes.c_int, [_p, _p, _p]),
+    "lbx_mp_deff": (ctypes.c_int, [_p, _p]),
+    "lbx_mp_free": (ctypes.c_int, [_p]),
+    "lbx_collide": (ctypes.c_int, [_i32] * 3 + [_p, _f64] + [_p] * 8),
+    "lbx_timing_enable": (ctypes.c_int, [_p, _i32]),
+    "lbx_kernel_time": (ctypes.c_int, [_p, _i32, _p, _p]),
+    "lbx_fluid_nodes": (_i64, [_p]),
+    "lbx_sync": (ctypes.c_int, [_p]),
+}
+
+
+class LbxError(RuntimeError):
+    def __init__(self, status, where=""):
+        self.status = status
+        msg = lib().lbx_status_string(status).decode()
+        super().__init__(f"{where}: lbx status {status}: {msg}" if where else f"lbx status {status}: {msg}")
+
+
+def lib():
+    """Load liblbx.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C laboetie_amd/csrc` "
+                               "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _check(st, where):
+    if st != OK:
+        raise LbxError(st, where)
+
+
+def d3q19_tables():
+    c = np.zeros(57, np.int32)
+    a0, a1, a2 = np.zeros(19), np.zeros(19), np.zeros(19)
+    inv = np.zeros(19, np.int32)
+    _check(lib().lbx_d3q19_tables(_ptr(c), _ptr(a0), _ptr(a1), _ptr(a2), _ptr(inv)), "d3q19_tables")
+    return c.reshape(19, 3), a0, a1, a2, inv
+
+
+def build_geometry(label, lx, ly, lz, solid_ijk=None):
+    """node%nature of supercell_definition (labels -1, 0, 1, 2, 3)."""
+    nat = np.zeros(lx * ly * lz, np.int8)
+    pts = np.ascontiguousarray(np.zeros((0, 3), np.int32) if solid_ijk is None else solid_ijk, np.int32)
+    _check(lib().lbx_build_geometry(label, lx, ly, lz, _ptr(pts), pts.size // 3, _ptr(nat)), "build_geometry")
+    return nat
+
+
+def detect_interfacial(lx, ly, lz, nature):
+    itf = np.zeros(lx * ly * lz, np.int8)
+    _check(lib().lbx_detect_interfacial(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)), _ptr(itf)),
+           "detect_interfacial")
+    return itf
+
+
+def footprint_bytes(lx, ly, lz, precision=PREC_FP64, with_mp=True):
+    return int(lib().lbx_footprint_bytes(lx, ly, lz, precision, 1 if with_mp else 0))
+
+
+def collide(dims, nature, tau, n, density, j, f):
+    """module_collision::collide on host arrays; returns the new n."""
+    lx, ly, lz = dims
+    n = np.array(n, np.float64)
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (density, j[0], j[1], j[2], f[0], f[1], f[2])]
+    _check(lib().lbx_collide(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)), tau, _ptr(n),
+                             *[_ptr(a) for a in arrs]), "collide")
+    return n
+
+
+class Lattice:
+    """One lattice resident in HBM (an lbx_ctx)."""
+
+    def __init__(self, dims, nature, tau=1.0, precision=PREC_FP64):
+        self.dims = tuple(int(v) for v in dims)
+        lx, ly, lz = self.dims
+        self.N = lx * ly * lz
+        self.nature = np.ascontiguousarray(nature, np.int8)
+        assert self.nature.size == self.N
+        h = _p()
+        _check(lib().lbx_create(lx, ly, lz, _ptr(self.nature), float(tau), precision, ctypes.byref(h)), "create")
+        self._h = h
+        self.tau = tau
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().lbx_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def fluid_nodes(self):
+        return int(lib().lbx_fluid_nodes(self._h))
+
+    def init_rest(self, rho0=1.0):
+        _check(lib().lbx_init_rest(self._h, float(rho0)), "init_rest")
+
+    def set_state(self, n, rho, j):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (n, rho, j[0], j[1], j[2])]
+        _check(lib().lbx_set_state(self._h, *[_ptr(a) for a in arrs]), "set_state")
+
+    def equilibrate(self, fext, target_error=1e-10, max_steps=0, trace_cap=1 << 20, check=True):
+        fe = np.ascontiguousarray(fext, np.float64)
+        tr = np.zeros(trace_cap)
+        sd, er = _i64(), _i32()
+        st = lib().lbx_equilibrate(self._h, float(target_error), _ptr(fe), int(max_steps), _ptr(tr), trace_cap,
+                                   ctypes.byref(sd), ctypes.byref(er))
+        if check and st not in (OK,):
+            raise LbxError(st, "equilibrate")
+        steps = sd.value
+        nl2 = steps - 1 if st == ERR_NEG_POP else steps
+        return dict(status=st, steps=steps, exit_reason=er.value, l2err=tr[:min(nl2, trace_cap)].copy())
+
+    def get_state(self):
+        n = np.zeros(19 * self.N)
+        rho, jx, jy, jz = (np.zeros(self.N) for _ in range(4))
+        _check(lib().lbx_get_state(self._h, _ptr(n), _ptr(rho), _ptr(jx), _ptr(jy), _ptr(jz)), "get_state")
+        return n, rho, np.stack([jx, jy, jz])
+
+    def mp_init(self, fext, Db, ka=0.0, kd=0.0, flow=None):
+        fe = np.ascontiguousarray(fext, np.float64)
+        v0 = np.zeros(3)
+        if flow is None:
+            _check(lib().lbx_mp_init(self._h, _ptr(fe), float(Db), float(ka), float(kd), _ptr(v0)), "mp_init")
+        else:
+            rho, j = flow
+            arrs = [np.ascontiguousarray(a, np.float64) for a in (rho, j[0], j[1], j[2])]
+            _check(lib().lbx_mp_init_from_flow(self._h, *[_ptr(a) for a in arrs], _ptr(fe), float(Db), float(ka),
+                                               float(kd), _ptr(v0)), "mp_init_from_flow")
+        return v0
+
+    def mp_run(self, nsteps, stop_on_convergence=False, check=True):
+        tr = np.zeros(3 * max(nsteps, 1))
+        sd, cv = _i64(), _i32()
+        st = lib().lbx_mp_run(self._h, int(nsteps), 1 if stop_on_convergence else 0, _ptr(tr), ctypes.byref(sd),
+                              ctypes.byref(cv))
+        if check and st != OK:
+            raise LbxError(st, "mp_run")
+        return dict(status=st, steps=sd.value, converged=bool(cv.value), vacf=tr[:3 * sd.value].reshape(-1, 3))
+
+    def mp_get(self):
+        P, Pa = np.zeros(3 * self.N), np.zeros(3 * self.N)
+        _check(lib().lbx_mp_get(self._h, _ptr(P), _ptr(Pa)), "mp_get")
+        return P, Pa
+
+    def mp_deff(self):
+        d = np.zeros(3)
+        _check(lib().lbx_mp_deff(self._h, _ptr(d)), "mp_deff")
+        return d
+
+    def mp_free(self):
+        _check(lib().lbx_mp_free(self._h), "mp_free")
+
+    def timing(self, on=True):
+        _check(lib().lbx_timing_enable(self._h, 1 if on else 0), "timing_enable")
+
+    def kernel_time(self, which):
+        ms, n = _f64(), _i64()
+        _check(lib().lbx_kernel_time(self._h, which, ctypes.byref(ms), ctypes.byref(n)), "kernel_time")
+        return ms.value, n.value
+
+    def sync(self):
+        _check(lib().lbx_sync(self._h), "sync")

@@ -1,0 +1,181 @@
+"""Parity of the HIP path (liblbx.so through its C-ABI) with the reference.
+
+Bar: FP64 populations, density, flux, the per-step l2err trace and the exit
+step are bit-identical to the reference's (its golden fixtures, or the C
+restatement oracle/lbref.c pinned to them); per-node moment-propagation
+fields P are bit-identical to the restatement; VACF(t), a sum over all fluid
+nodes that the GPU reduces in a different order, agrees to 1e-12 relative
+to the scale sum_r |P(r,t) u*(r)| (bounded here by |VACF(0)|, which
+dominates every later term); D_eff to 1e-12 relative.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import lbref
+from golden_cases import FULL_CASES, Case, load
+from laboetie_amd import lbx
+
+pytestmark = pytest.mark.gpu
+
+VACF_RTOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def vacf_close(got, ref, scale):
+    return np.all(np.abs(got - ref) <= VACF_RTOL * scale)
+
+
+def test_collide_matches_reference_on_random_inputs():
+    g = load("collide_random")
+    say("collide_random", tuple(g["dims"]))
+    n = lbx.collide(tuple(g["dims"]), g["nature"], float(g["tau"]), g["n_in"], g["rho"], g["j"], g["f"])
+    assert np.array_equal(n, g["n_out"])
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_equilibration_and_mp_match_reference(name):
+    c = Case(name)
+    say("case", name, c.dims)
+    nat = lbx.build_geometry(c.label, *c.dims, c.solid_ijk)
+    assert np.array_equal(nat, c.nature)
+    with lbx.Lattice(c.dims, nat, c.tau) as L:
+        L.init_rest(c.rho0)
+        r = L.equilibrate(c.fext, c.target)
+        assert r["exit_reason"] == lbx.EXIT_CONVERGED
+        assert r["steps"] == int(c.g["l2err_step"][-1])
+        assert np.array_equal(r["l2err"], c.g["l2err"])
+        n, rho, j = L.get_state()
+        assert np.array_equal(n, c.g["n"])
+        assert np.array_equal(rho, c.g["rho"])
+        assert np.array_equal(j, c.j)
+        # moment propagation
+        v0 = L.mp_init(c.fext, c.Db, c.ka, c.kd)
+        ref = c.g["vacf"]
+        scale = np.abs(ref[0]).max()
+        assert vacf_close(v0, ref[0], scale)
+        m = L.mp_run(c.mp_steps)
+        assert m["steps"] == c.mp_steps
+        assert vacf_close(m["vacf"], ref[1:], scale), np.abs(m["vacf"] - ref[1:]).max()
+        # per-node P against the restatement after the same number of steps
+        nt = lbref.tracer_population(c.dims, nat, c.g["rho"], c.j, c.fext)
+        mp = lbref.MP(c.dims, nat, c.interfacial, nt, c.g["rho"], c.Db, c.ka, c.kd)
+        for _ in range(c.mp_steps):
+            mp.step()
+        P, Pa = L.mp_get()
+        assert np.array_equal(P, mp.P)
+        if mp.ads:
+            assert np.array_equal(Pa, mp.Pa)
+        d = L.mp_deff()
+        dref = lbref.deff(ref)
+        assert np.all(np.abs(d - dref) <= 1e-12 * np.abs(dref).max())
+
+
+def test_mp_per_step_calls_equal_one_run():
+    """propagate() called once per step (the reference's interface) gives the
+    same VACF as one multi-step run."""
+    c = Case("bcc12_ads")
+    with lbx.Lattice(c.dims, c.nature, c.tau) as L:
+        L.init_rest(c.rho0)
+        L.equilibrate(c.fext, c.target)
+        L.mp_init(c.fext, c.Db, c.ka, c.kd)
+        one = np.concatenate([L.mp_run(1)["vacf"] for _ in range(40)])
+        L.mp_init(c.fext, c.Db, c.ka, c.kd)
+        many = L.mp_run(40)["vacf"]
+        assert np.array_equal(one, many)
+
+
+def test_mp_from_host_flow_equals_resident_flow():
+    c = Case("cyl9_tau08")
+    with lbx.Lattice(c.dims, c.nature, c.tau) as L:
+        L.init_rest(c.rho0)
+        L.equilibrate(c.fext, c.target)
+        v_res = L.mp_init(c.fext, c.Db, c.ka, c.kd)
+        a = L.mp_run(50)["vacf"]
+        v_host = L.mp_init(c.fext, c.Db, c.ka, c.kd, flow=(c.g["rho"], c.j))
+        b = L.mp_run(50)["vacf"]
+        assert np.array_equal(v_res, v_host) and np.array_equal(a, b)
+
+
+def test_error_statuses():
+    nat = lbx.build_geometry(1, 3, 3, 6)
+    with pytest.raises(lbx.LbxError) as e:
+        lbx.Lattice((3, 3, 6), nat, tau=0.4)
+    assert e.value.status == lbx.ERR_TAU
+    g = load("failures")
+    rows = {str(n): i for i, n in enumerate(g["names"])}
+    nat = lbx.build_geometry(1, 2, 2, 8)
+    with lbx.Lattice((2, 2, 8), nat, 1.0) as L:
+        L.init_rest(1.0)
+        r = L.equilibrate([0.5, 0, 0], 1e-10, check=False)
+        assert r["status"] == lbx.ERR_NEG_POP
+        assert r["steps"] == int(g["l2err_lines"][rows["fail_negpop"]]) + 1
+    nat = lbx.build_geometry(1, 3, 3, 6)
+    with lbx.Lattice((3, 3, 6), nat, 1.0) as L:
+        L.init_rest(1.0)
+        L.equilibrate([0, 0, 0], 1e-10)
+        L.mp_init([0, 0, 0], 0.3)
+        r = L.mp_run(10, check=False)
+        assert r["status"] == lbx.ERR_RESTPART
+
+
+@pytest.mark.parametrize("label,dims,tau,fext", [
+    (3, (40, 40, 40), 1.0, [0.0, 0.0, 1e-6]),
+    (2, (33, 33, 20), 0.8, [0.0, 2e-6, 1e-5]),
+    (1, (24, 17, 30), 0.65, [1e-5, 0.0, 0.0]),
+])
+def test_capped_runs_match_restatement_at_larger_sizes(label, dims, tau, fext):
+    say("capped", label, dims)
+    nat = lbx.build_geometry(label, *dims)
+    steps = 150
+    ref = lbref.equilibrate(dims, nat, tau, 1.0, fext, 1e-10, max_steps=steps)
+    with lbx.Lattice(dims, nat, tau) as L:
+        L.init_rest(1.0)
+        r = L.equilibrate(fext, 1e-10, max_steps=steps)
+        assert r["steps"] == steps and r["exit_reason"] == lbx.EXIT_CAP
+        assert np.array_equal(r["l2err"], ref["l2err"])
+        n, rho, j = L.get_state()
+        assert np.array_equal(n, ref["n"])
+        assert np.array_equal(rho, ref["rho"])
+        assert np.array_equal(j, ref["j"])
+        # MP on the capped flow, 30 steps, against the restatement
+        itf = lbref.interfacial(*dims, nat)
+        L.mp_init(fext, 0.01, 0.02, 0.01)
+        m = L.mp_run(30)
+        nt = lbref.tracer_population(dims, nat, rho, j, fext)
+        mp = lbref.MP(dims, nat, itf, nt, rho, 0.01, 0.02, 0.01)
+        st, vref = mp.run(30)
+        scale = np.abs(vref[0]).max()
+        assert vacf_close(m["vacf"], vref[1:], scale)
+        P, Pa = L.mp_get()
+        assert np.array_equal(P, mp.P)
+
+
+def test_full_size_invariants_bcc128():
+    """128^3 BCC at full size: mass is conserved to rounding while the force
+    acts, stage 1 ends at t = 3, and the tracer probability (free + adsorbed)
+    is conserved by moment propagation."""
+    dims = (128, 128, 128)
+    say("bcc128 invariants")
+    nat = lbx.build_geometry(3, *dims)
+    with lbx.Lattice(dims, nat, 1.0) as L:
+        nf = L.fluid_nodes
+        assert nf == int((nat == 0).sum())
+        L.init_rest(1.0)
+        r = L.equilibrate([0.0, 0.0, 1e-6], 1e-10, max_steps=400)
+        assert list(r["l2err"][:3]) == [0.0, 0.0, 0.0] and r["l2err"][3] > 0
+        n, rho, j = L.get_state()
+        assert abs(rho.sum() - nf) <= 1e-12 * nf
+        assert np.all(n >= 0)
+        L.mp_init([0.0, 0.0, 1e-6], 0.01, 0.01 * np.e, 0.01)
+        P0, _ = L.mp_get()
+        tot0 = P0.reshape(-1, 3).sum(0)
+        L.mp_run(200)
+        P, Pa = L.mp_get()
+        tot = (P + Pa).reshape(-1, 3).sum(0)
+        assert np.allclose(tot, tot0, rtol=0, atol=1e-13 * np.abs(P0).sum())
