@@ -127,9 +127,11 @@ def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0):
         return None
     threads = max(1, min(16, os.cpu_count() or 1))
     N = int(np.prod(w["dims"]))
+    print("cpu_baseline: timing the reference LB phase", file=sys.stderr, flush=True)
     with tempfile.TemporaryDirectory() as d:
         open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, w["fext"], 0, 0.0))
         lb_rate = _sample_growth(exe, d, "l2err.dat", window, threads, max_wait=240)
+    print(f"cpu_baseline: LB {lb_rate} steps/s; timing the reference MP phase", file=sys.stderr, flush=True)
     with tempfile.TemporaryDirectory() as d:
         ka = w["kd"] * np.exp(1.0)
         open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, (0.0, 0.0, 0.0), 10 ** 7, ka))
@@ -167,6 +169,7 @@ def main():
         nf = L.fluid_nodes
         for _ in range(args.warmup):
             solve(L, w)
+            print("bench: warmup solve done", file=sys.stderr, flush=True)
         L.timing(True)
         L.sync()
         t0 = time.perf_counter()

@@ -1,15 +1,22 @@
 // liblbx: MI355X (gfx950) kernels and C-ABI of the laboetie hot path.
 //
-// Device layout of one lattice (N = lx*ly*lz nodes, x fastest, planes of Np
-// elements, Np = N rounded up to 64):
-//   f[2][19][Np]  post-collision populations, ping-pong (step t reads
-//                 f[(t-1)&1], writes f[t&1])
-//   j[3][Np]      flux of the last completed step (the next step's j_old)
-//   rho[Np]       density (written at set-up and when a run ends)
-//   mask[N]       per-node word: bit l (1..18) = pull source r - c_l is solid,
-//                 bit 30 = interfacial, bit 31 = solid node
-// Moment propagation adds win[18][Np] (incoming scattering probabilities),
-// rem[Np], ustar[3][Np], P[2][3][Np] and, with adsorption, Pa[2][3][Np].
+// Device layout of one lattice.  Only fluid nodes are stored (solid nodes
+// hold n = 0 forever in the reference: they start at 0, collide() skips them
+// and the bounce-back swap only moves fluid values into solid slots that the
+// next streaming sends straight back), packed in the reference's node order
+// (x fastest, then y, then z):
+//   nodes[Nf]     full-box index r of each fluid node; bit 31 = interfacial
+//   map[N]        full-box index -> packed index, kSolidSlot for solid nodes
+//                 (the neighbour lookup; it also replaces the nature array)
+//   f[2][19][Nfp] post-collision populations, ping-pong (step t reads
+//                 f[(t-1)&1], writes f[t&1]); plane stride Nfp = Nf rounded
+//                 up to 64
+//   j[3][Nfp]     flux of the last completed step (the next step's j_old)
+//   rho[Nfp]      density (written at set-up and when a run ends)
+// Moment propagation adds win[18][Nfp] (incoming scattering probabilities),
+// rem[Nfp], ustar[3][Nfp], P[2][3][Nfp] and, with adsorption, Pa[2][3][Nfp].
+// Every kernel runs one thread per packed fluid node, 256-node tiles, so all
+// lanes work and every fetched cache line holds fluid data.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,29 +38,47 @@ using namespace lbx;
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;          // one tile = 256 consecutive packed nodes
+constexpr int kRedBlock = 1024;      // the one-block reduction kernels
 constexpr int kRing = 1024;          // l2err ring slots (>= 2 chunks)
 constexpr int kChunk = 256;          // LB steps launched between host checks
 constexpr int kMpChunk = 512;        // MP steps per host check
-constexpr unsigned long long kNone = ~0ull;
+constexpr uint32_t kSolidSlot = 0xFFFFFFFFu;
+constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
+constexpr uint32_t kNodeInterfacial = 0x80000000u;
 
+// Device-side control block.  Every field has a single writer per launch
+// (block 0 of a step kernel, or the one-block reduction kernel), and is read
+// by later launches on the same stream, so plain stores suffice.
 struct Ctrl {
-    int32_t stop;                // LB: a kernel saw the previous step converge
+    int32_t stop;                // LB: a step kernel saw the previous step converge
     int32_t restpart;            // MP: a rest probability < eps (set by mp prep)
     int64_t exit_step;           // LB: that converged step
-    unsigned long long negpop;   // LB: first step with a negative population
+    int64_t negpop;              // LB: first step with a negative population (-1: none)
     int32_t mp_stop;             // MP: VACF convergence reached
-    uint32_t ticket;             // last-block reduction ticket
+    int32_t pad;
     int64_t mp_conv_step;
     double vacf0[3];
     double deff_s[3], deff_c[3]; // Neumaier running sums of the VACF integral
 };
 
 __device__ __forceinline__ int32_t vload(const int32_t* p) { return *(const volatile int32_t*)p; }
-__device__ __forceinline__ unsigned long long vload(const unsigned long long* p) {
-    return *(const volatile unsigned long long*)p;
-}
+__device__ __forceinline__ int64_t vload(const int64_t* p) { return *(const volatile int64_t*)p; }
 __device__ __forceinline__ double vload(const double* p) { return *(const volatile double*)p; }
+
+// packed indices of the 18 neighbours r + c_l (l = 1..18) of full-box node r
+struct Neighbours {
+    uint32_t q[kQ];
+    __device__ __forceinline__ void load(const uint32_t* __restrict__ map, uint32_t r, const FastDiv& dlx,
+                                         const FastDiv& dly, uint32_t lz) {
+        Nbr nb;
+        nb.init(r, dlx, dly, lz);
+        static_for<1, kQ>([&](auto I) {
+            constexpr int L = I;
+            q[L] = ld(map, nb.at<kCX[L], kCY[L], kCZ[L]>());
+        });
+    }
+};
 
 // ---------------------------------------------------------------- LB kernels
 
@@ -61,10 +86,13 @@ struct StepArgs {
     const double* __restrict__ fin;
     double* __restrict__ fout;
     double* __restrict__ j;
-    const uint32_t* __restrict__ mask;
-    unsigned long long* l2ring;
+    const uint32_t* __restrict__ nodes;
+    const uint32_t* __restrict__ map;
+    double* part;                         // per-block max|dj| (-1: a negative population)
+    const double* l2ring;
     Ctrl* ctrl;
-    uint32_t lx, ly, lz, N;
+    FastDiv dlx, dly;
+    uint32_t lz, Nf;
     uint64_t Np;
     double A, B, C;
     double fmx, fmy, fmz;   // F(t)/2, the force of step t in j(t) (equilibration.f90:293-295)
@@ -75,15 +103,33 @@ struct StepArgs {
     int32_t write_l2;       // 0 for a redo of a step whose l2err is already known
 };
 
-// One time step, fused: pull-streaming with half-way bounce-back of the
-// post-collision populations of step t (equilibration.f90:204-243), the
-// negativity check (:248), density (:254) and flux (:293-300), the l2err
-// contribution (:339-344), then the collision that opens step t+1
-// (module_collision.f90:87-108).
-__global__ __launch_bounds__(kBlock) void k_lb_step(StepArgs a) {
-    if (vload(&a.ctrl->stop) || vload(&a.ctrl->negpop) != kNone) return;
+// Pull-streaming with half-way bounce-back (equilibration.f90:204-243): the
+// population arriving at node i in direction L left r - c_L, or, when that
+// node is solid, left i itself in direction inv(L).
+// The two candidate sources differ only in the lane's element index, so one
+// load per direction serves both.
+template <int L, bool Wide>
+__device__ __forceinline__ double pull(const View<Wide>& fin, uint32_t Np, uint32_t i, const Neighbours& nb) {
+    if constexpr (L == 0) {
+        return fin.ld(i);
+    } else {
+        const uint32_t q = nb.q[kInv[L]];  // r - c_L = r + c_inv(L)
+        return fin.ld(q != kSolidSlot ? (uint32_t)L * Np + q : (uint32_t)kInv[L] * Np + i);
+    }
+}
+
+// One time step, fused: pull-streaming with bounce-back of the post-collision
+// populations of step t, the negativity check (equilibration.f90:248),
+// density (:254) and flux (:293-300), the l2err contribution (:339-344), then
+// the collision that opens step t+1 (module_collision.f90:87-108).
+#ifndef LBX_LB_WAVES
+#define LBX_LB_WAVES 1
+#endif
+template <bool Wide>
+__global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs a) {
+    if (vload(&a.ctrl->stop) || vload(&a.ctrl->negpop) >= 0) return;
     if (a.t - 1 > a.check_after) {
-        const double prev = __longlong_as_double((long long)vload(&a.l2ring[(a.t - 1) % kRing]));
+        const double prev = vload(&a.l2ring[(a.t - 1) % kRing]);
         if (prev <= a.target) {
             if (blockIdx.x == 0 && threadIdx.x == 0) {
                 a.ctrl->exit_step = a.t - 1;
@@ -92,183 +138,230 @@ __global__ __launch_bounds__(kBlock) void k_lb_step(StepArgs a) {
             return;
         }
     }
-    if (a.write_l2 && blockIdx.x == 0 && threadIdx.x == 0) a.l2ring[(a.t + 1) % kRing] = 0ull;
-
-    const uint32_t r = xcd_tile() * kBlock + threadIdx.x;
+    const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
     double l2 = 0.0;
     bool neg = false;
-    if (r < a.N) {
-        const uint32_t m = a.mask[r];
-        if (!(m & kSolidBit)) {
-            Nbr nb;
-            nb.init(r, a.lx, a.ly, a.lz);
-            double f[kQ];
-            static_for<0, kQ>([&](auto I) {
-                constexpr int L = I;
-                const uint32_t src = nb.at<-kCX[L], -kCY[L], -kCZ[L]>();
-                const bool bb = (m >> L) & 1u;
-                const uint64_t off = bb ? (uint64_t)kInv[L] * a.Np + r : (uint64_t)L * a.Np + src;
-                f[L] = a.fin[off];
-            });
-            static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < 0.0); });
-            double rho = f[0];
-            static_for<1, kQ>([&](auto I) { rho = rho + f[I]; });
-            double jx = a.fmx, jy = a.fmy, jz = a.fmz;
-            static_for<0, kQ>([&](auto I) {
-                constexpr int L = I;
-                if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(f[L]);
-                if constexpr (kCY[L] != 0) jy = jy + sgn<kCY[L]>(f[L]);
-                if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(f[L]);
-            });
-            double* jp = a.j + r;
-            if (a.write_l2) {
-                const double dx = fabs(jx - jp[0]), dy = fabs(jy - jp[a.Np]), dz = fabs(jz - jp[2 * a.Np]);
-                l2 = fmax(fmax(dx, dy), dz);
-            }
-            jp[0] = jx;
-            jp[a.Np] = jy;
-            jp[2 * a.Np] = jz;
-            const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
-            static_for<0, kQ>([&](auto I) {
-                constexpr int L = I;
-                a.fout[(uint64_t)L * a.Np + r] =
-                    collide_pop<L>(f[L], rho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy, a.fcz, a.A, a.B, a.C);
-            });
+    if (i < a.Nf) {
+        const uint32_t Np = (uint32_t)a.Np;
+        const View<Wide> fin(a.fin, kQ * a.Np), fout(a.fout, kQ * a.Np), jv(a.j, 3 * a.Np);
+        Neighbours nb;
+        nb.load(a.map, ld(a.nodes, i) & kIndexMask, a.dlx, a.dly, a.lz);
+        double jo[3] = {0.0, 0.0, 0.0};
+        if (a.write_l2) {  // j_old, issued before the populations so its latency overlaps theirs
+            jo[0] = jv.ld(i);
+            jo[1] = jv.ld(Np + i);
+            jo[2] = jv.ld(2 * Np + i);
         }
+        double f[kQ];
+        static_for<0, kQ>([&](auto I) { f[I] = pull<I, Wide>(fin, Np, i, nb); });
+        static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < 0.0); });
+        double rho = f[0];
+        static_for<1, kQ>([&](auto I) { rho = rho + f[I]; });
+        double jx = a.fmx, jy = a.fmy, jz = a.fmz;
+        static_for<0, kQ>([&](auto I) {
+            constexpr int L = I;
+            if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(f[L]);
+            if constexpr (kCY[L] != 0) jy = jy + sgn<kCY[L]>(f[L]);
+            if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(f[L]);
+        });
+        if (a.write_l2) {
+            const double dx = fabs(jx - jo[0]), dy = fabs(jy - jo[1]), dz = fabs(jz - jo[2]);
+            l2 = fmax(fmax(dx, dy), dz);
+        }
+        jv.st(i, jx);
+        jv.st(Np + i, jy);
+        jv.st(2 * Np + i, jz);
+        const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
+        static_for<0, kQ>([&](auto I) {
+            constexpr int L = I;
+            fout.st((uint32_t)L * Np + i,
+                    collide_pop<L>(f[L], rho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy, a.fcz, a.A, a.B, a.C));
+        });
     }
     if (a.write_l2) {
+        __shared__ double lds[kBlock / 64];
         l2 = wave_max(l2);
-        if ((threadIdx.x & 63) == 0 && l2 > 0.0)
-            atomicMax(&a.l2ring[a.t % kRing], (unsigned long long)__double_as_longlong(l2));
+        const bool wneg = __ballot(neg) != 0ull;
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = wneg ? -1.0 : l2;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = lds[0];
+            bool bneg = m < 0.0;
+            for (int w = 1; w < kBlock / 64; ++w) {
+                bneg = bneg || lds[w] < 0.0;
+                m = fmax(m, lds[w]);
+            }
+            a.part[blockIdx.x] = bneg ? -1.0 : m;
+        }
     }
-    if (__ballot(neg) != 0ull && (threadIdx.x & 63) == 0) atomicMin(&a.ctrl->negpop, (unsigned long long)a.t);
+}
+
+// l2err of step t (equilibration.f90:339-344) from the per-block maxima, and
+// the negativity verdict (:248); one block, fixed order.
+__global__ __launch_bounds__(kRedBlock) void k_lb_reduce(const double* __restrict__ part, uint32_t nparts,
+                                                         double* l2ring, Ctrl* ctrl, int64_t t) {
+    __shared__ double lds[kRedBlock / 64];
+    if (vload(&ctrl->stop) || vload(&ctrl->negpop) >= 0) return;
+    double m = 0.0;
+    bool neg = false;
+    for (uint32_t k = threadIdx.x; k < nparts; k += kRedBlock) {
+        const double v = part[k];
+        neg = neg || v < 0.0;
+        m = fmax(m, v);
+    }
+    m = wave_max(m);
+    const bool wneg = __ballot(neg) != 0ull;
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = wneg ? -1.0 : m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        bool bneg = false;
+        for (int w = 0; w < kRedBlock / 64; ++w) {
+            bneg = bneg || lds[w] < 0.0;
+            mm = fmax(mm, lds[w]);
+        }
+        l2ring[t % kRing] = mm;
+        if (bneg) ctrl->negpop = t;
+    }
 }
 
 // Streaming only: the post-streaming state f_s(t) of the step whose input is
-// fin, with its density; solids get n = 0, rho = 0, j = 0.  Writes the
-// reference-layout populations into nout (plane stride Np).
+// fin, and its density SUM(n,4) (equilibration.f90:254).
 __global__ __launch_bounds__(kBlock) void k_extract(const double* __restrict__ fin, double* __restrict__ nout,
-                                                    double* __restrict__ rho_out, double* __restrict__ j,
-                                                    const uint32_t* __restrict__ mask, uint32_t lx, uint32_t ly,
-                                                    uint32_t lz, uint32_t N, uint64_t Np) {
-    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r >= N) return;
-    const uint32_t m = mask[r];
-    if (m & kSolidBit) {
-        for (int l = 0; l < kQ; ++l) nout[(uint64_t)l * Np + r] = 0.0;
-        rho_out[r] = 0.0;
-        j[r] = 0.0;
-        j[Np + r] = 0.0;
-        j[2 * Np + r] = 0.0;
-        return;
-    }
-    Nbr nb;
-    nb.init(r, lx, ly, lz);
+                                                    double* __restrict__ rho_out, const uint32_t* __restrict__ nodes,
+                                                    const uint32_t* __restrict__ map, FastDiv dlx, FastDiv dly,
+                                                    uint32_t lz, uint32_t Nf, uint64_t Np) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= Nf) return;
+    Neighbours nb;
+    nb.load(map, nodes[i] & kIndexMask, dlx, dly, lz);
+    const View<true> fv(fin, kQ * Np);
     double rho = 0.0;
     static_for<0, kQ>([&](auto I) {
         constexpr int L = I;
-        const uint32_t src = nb.at<-kCX[L], -kCY[L], -kCZ[L]>();
-        const bool bb = (m >> L) & 1u;
-        const double v = fin[bb ? (uint64_t)kInv[L] * Np + r : (uint64_t)L * Np + src];
-        nout[(uint64_t)L * Np + r] = v;
+        const double v = pull<L, true>(fv, (uint32_t)Np, i, nb);
+        nout[(uint64_t)L * Np + i] = v;
         rho = (L == 0) ? v : rho + v;
     });
-    rho_out[r] = rho;
+    rho_out[i] = rho;
 }
 
-// collide() on whole fields (module_collision.f90:15-130): fluid nodes only.
-// F is per node (fx/fy/fz arrays) or uniform (arrays NULL).  Solid nodes are
-// copied (in place: untouched).
-__global__ __launch_bounds__(kBlock) void k_collide_field(const double* fin, double* fout,
-                                                          const double* __restrict__ rho,
-                                                          const double* __restrict__ j,
-                                                          const double* __restrict__ fxa,
-                                                          const double* __restrict__ fya,
-                                                          const double* __restrict__ fza,
-                                                          double fx0, double fy0, double fz0,
-                                                          const uint32_t* __restrict__ mask, uint32_t N,
-                                                          uint64_t Np, double A, double B, double C) {
-    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r >= N) return;
-    const bool solid = mask[r] & kSolidBit;
-    if (solid) {
-        if (fin != fout)
-            for (int l = 0; l < kQ; ++l) fout[(uint64_t)l * Np + r] = fin[(uint64_t)l * Np + r];
-        return;
-    }
-    const double d = rho[r], jx = j[r], jy = j[Np + r], jz = j[2 * Np + r];
-    const double fx = fxa ? fxa[r] : fx0, fy = fya ? fya[r] : fy0, fz = fza ? fza[r] : fz0;
+// The collision on packed fields with a uniform force (the opening collision
+// of a run), in place: module_collision.f90:87-108.
+__global__ __launch_bounds__(kBlock) void k_collide_packed(double* f, const double* __restrict__ rho,
+                                                           const double* __restrict__ j, double fx, double fy,
+                                                           double fz, uint32_t Nf, uint64_t Np, double A, double B,
+                                                           double C) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= Nf) return;
+    const double d = rho[i], jx = j[i], jy = j[Np + i], jz = j[2 * Np + i];
     const double ux = jx / d, uy = jy / d, uz = jz / d;
     static_for<0, kQ>([&](auto I) {
         constexpr int L = I;
-        const double v = fin[(uint64_t)L * Np + r];
-        fout[(uint64_t)L * Np + r] = collide_pop<L>(v, d, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
+        const double v = f[(uint64_t)L * Np + i];
+        f[(uint64_t)L * Np + i] = collide_pop<L>(v, d, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
     });
 }
 
-// Rest state, init_simu.f90:31-39: n = rho*a0 (rho = rho0 at fluid, 0 at solid).
-__global__ __launch_bounds__(kBlock) void k_init_rest(double* __restrict__ f, double* __restrict__ rho,
-                                                      double* __restrict__ j, const uint32_t* __restrict__ mask,
-                                                      uint32_t N, uint64_t Np, double rho0) {
+// collide() on whole full-box fields with per-node forces, in place
+// (module_collision.f90:15-130, fluid nodes only): the stateless entry point.
+__global__ __launch_bounds__(kBlock) void k_collide_box(double* f, const double* __restrict__ rho,
+                                                        const double* __restrict__ j,
+                                                        const double* __restrict__ F,
+                                                        const int8_t* __restrict__ nature, uint32_t N, double A,
+                                                        double B, double C) {
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r >= N) return;
-    const double d = (mask[r] & kSolidBit) ? 0.0 : rho0;
+    if (r >= N || nature[r] != 0) return;
+    const uint64_t n = N;
+    const double d = rho[r], jx = j[r], jy = j[n + r], jz = j[2 * n + r];
+    const double fx = F[r], fy = F[n + r], fz = F[2 * n + r];
+    const double ux = jx / d, uy = jy / d, uz = jz / d;
     static_for<0, kQ>([&](auto I) {
         constexpr int L = I;
-        f[(uint64_t)L * Np + r] = d * a0(L);
+        const double v = f[(uint64_t)L * n + r];
+        f[(uint64_t)L * n + r] = collide_pop<L>(v, d, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
     });
-    rho[r] = d;
-    j[r] = 0.0;
-    j[Np + r] = 0.0;
-    j[2 * Np + r] = 0.0;
+}
+
+// Rest state, init_simu.f90:31-39: n = rho0*a0 at fluid nodes, j = 0.
+__global__ __launch_bounds__(kBlock) void k_init_rest(double* __restrict__ f, double* __restrict__ rho,
+                                                      double* __restrict__ j, uint32_t Nf, uint64_t Np, double rho0) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= Nf) return;
+    static_for<0, kQ>([&](auto I) {
+        constexpr int L = I;
+        f[(uint64_t)L * Np + i] = rho0 * a0(L);
+    });
+    rho[i] = rho0;
+    j[i] = 0.0;
+    j[Np + i] = 0.0;
+    j[2 * Np + i] = 0.0;
 }
 
 // ---------------------------------------------------------------- MP kernels
 
-// Last-block reduction of per-block triples into one triple, in block order.
-// Returns true in thread 0 of the last block, with the total in out[].
-__device__ __forceinline__ bool last_block_sum3(double v[3], double* partials, uint32_t* ticket,
-                                                double out[3], double* lds) {
-    __shared__ bool is_last;
+// Per-block triple sums in a fixed order (the block partial of a step kernel).
+__device__ __forceinline__ void block_partial3(double v[3], double* part, double* lds) {
     block_sum3<kBlock>(v, lds);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x + 0] = v[0];
-        partials[3 * blockIdx.x + 1] = v[1];
-        partials[3 * blockIdx.x + 2] = v[2];
-        __threadfence();
-        is_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+        part[3 * blockIdx.x + 0] = v[0];
+        part[3 * blockIdx.x + 1] = v[1];
+        part[3 * blockIdx.x + 2] = v[2];
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
+}
+
+// VACF of one moment-propagation step from the per-block partial sums, in a
+// fixed order (so every run gives the same bits), then the running D_eff sum
+// and the convergence test of module_moment_propagation.f90:284-288.
+// mode 0: VACF(0) of mp init; mode 1: VACF(it) of a step.
+__global__ __launch_bounds__(kRedBlock) void k_mp_reduce(const double* __restrict__ part, uint32_t nparts,
+                                                         double* vtrace, Ctrl* ctrl, int32_t mode, int64_t it,
+                                                         int32_t check_conv, double lim1, double lim2) {
+    __shared__ double lds[kRedBlock / 64 * 3];
+    if (mode == 1 && vload(&ctrl->mp_stop)) return;
     double s[3] = {0.0, 0.0, 0.0};
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+    for (uint32_t k = threadIdx.x; k < nparts; k += kRedBlock)
+        for (int c = 0; c < 3; ++c) s[c] = (k == threadIdx.x) ? part[3 * k + c] : s[c] + part[3 * k + c];
+    block_sum3<kRedBlock>(s, lds);
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
         for (int c = 0; c < 3; ++c) {
-            const double x = __hip_atomic_load(&partials[3 * b + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s[c] = (b == threadIdx.x) ? x : s[c] + x;
+            ctrl->vacf0[c] = s[c];
+            ctrl->deff_s[c] = 0.5 * s[c];
+            ctrl->deff_c[c] = 0.0;
+        }
+        return;
+    }
+    for (int c = 0; c < 3; ++c) {
+        vtrace[c] = s[c];
+        // Neumaier update of D = VACF(0)/2 + sum VACF(t)
+        const double d = ctrl->deff_s[c], x = s[c], u = d + x;
+        ctrl->deff_c[c] += (fabs(d) >= fabs(x)) ? (d - u) + x : (x - u) + d;
+        ctrl->deff_s[c] = u;
+    }
+    if (check_conv && it > 2) {
+        bool conv = true;
+        for (int c = 0; c < 3; ++c) conv = conv && fabs(s[c]) < lim1 && fabs(s[c]) < lim2;
+        if (conv) {
+            ctrl->mp_conv_step = it;
+            ctrl->mp_stop = 1;
         }
     }
-    block_sum3<kBlock>(s, lds);
-    if (threadIdx.x == 0) {
-        out[0] = s[0];
-        out[1] = s[1];
-        out[2] = s[2];
-        *ticket = 0u;
-    }
-    return threadIdx.x == 0;
 }
 
 struct MpPrepArgs {
     const double* __restrict__ rho;
     const double* __restrict__ j;
-    const uint32_t* __restrict__ mask;
+    const uint32_t* __restrict__ nodes;
+    const uint32_t* __restrict__ map;
     double* __restrict__ win;
     double* __restrict__ rem;
     double* __restrict__ ustar;
     double* __restrict__ P0;
     double* partials;
     Ctrl* ctrl;
-    uint32_t lx, ly, lz, N;
+    FastDiv dlx, dly;
+    uint32_t lz, Nf;
     uint64_t Np;
     double fx, fy, fz;   // lb.in f_ext (drop_tracers.f90:92)
     double lambda, bw, ka, eps;
@@ -284,146 +377,121 @@ struct MpPrepArgs {
 // P(r,1) and the VACF(0) contribution.
 __global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs a) {
     __shared__ double lds[kBlock / 64 * 3];
-    const uint32_t r = xcd_tile() * kBlock + threadIdx.x;
+    const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
     double vp[3] = {0.0, 0.0, 0.0};
-    if (r < a.N) {
-        const uint32_t m = a.mask[r];
-        if (!(m & kSolidBit)) {
-            Nbr nb;
-            nb.init(r, a.lx, a.ly, a.lz);
-            const double rr = a.rho[r];
-            const double vx = a.j[r] + a.fx, vy = a.j[a.Np + r] + a.fy, vz = a.j[2 * a.Np + r] + a.fz;
-            double us[3] = {0.0, 0.0, 0.0};
-            double p1[3] = {0.0, 0.0, 0.0};
-            double remv = 1.0;
-            static_for<1, kQ>([&](auto I) {
-                constexpr int L = I;
-                constexpr int Li = kInv[L];
-                constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
-                // r + c_L is fluid iff the pull source of direction inv(L) is fluid
-                if (!((m >> Li) & 1u)) {
-                    const double nt = a0(L) * rr + a1(L) * dot3<cx, cy, cz>(vx, vy, vz);
-                    const double s = nt / rr - a0(L) + a.lambda * a0(L) * 0.5;
-                    remv = remv - s;
-                    if constexpr (cx != 0) us[0] = us[0] + sgn<cx>(s);
-                    if constexpr (cy != 0) us[1] = us[1] + sgn<cy>(s);
-                    if constexpr (cz != 0) us[2] = us[2] + sgn<cz>(s);
-                    if constexpr (cx != 0) vp[0] = vp[0] + a.bw * s;
-                    if constexpr (cy != 0) vp[1] = vp[1] + a.bw * s;
-                    if constexpr (cz != 0) vp[2] = vp[2] + a.bw * s;
-                    const uint32_t q = nb.at<cx, cy, cz>();
-                    const double rn = a.rho[q];
-                    const double wx = a.j[q] + a.fx, wy = a.j[a.Np + q] + a.fy, wz = a.j[2 * a.Np + q] + a.fz;
-                    const double ntn = a0(Li) * rn + a1(Li) * dot3<-cx, -cy, -cz>(wx, wy, wz);
-                    const double sp = ntn / rn - a0(Li) + a.lambda * a0(Li) * 0.5;
-                    a.win[(uint64_t)(L - 1) * a.Np + r] = sp;
-                    if constexpr (cx != 0) p1[0] = p1[0] + sgn<-cx>(sp) * a.bw;
-                    if constexpr (cy != 0) p1[1] = p1[1] + sgn<-cy>(sp) * a.bw;
-                    if constexpr (cz != 0) p1[2] = p1[2] + sgn<-cz>(sp) * a.bw;
-                } else {
-                    a.win[(uint64_t)(L - 1) * a.Np + r] = 0.0;
-                }
-            });
-            if (a.ads && (m & kInterfacialBit)) remv = remv - a.ka;
-            if (remv < a.eps) a.ctrl->restpart = 1;
-            a.rem[r] = remv;
-            for (int c = 0; c < 3; ++c) {
-                a.ustar[c * a.Np + r] = us[c];
-                a.P0[c * a.Np + r] = p1[c];
+    if (i < a.Nf) {
+        const uint32_t e = a.nodes[i];
+        Neighbours nb;
+        nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
+        const double rr = a.rho[i];
+        const double vx = a.j[i] + a.fx, vy = a.j[a.Np + i] + a.fy, vz = a.j[2 * a.Np + i] + a.fz;
+        double us[3] = {0.0, 0.0, 0.0};
+        double p1[3] = {0.0, 0.0, 0.0};
+        double remv = 1.0;
+        static_for<1, kQ>([&](auto I) {
+            constexpr int L = I;
+            constexpr int Li = kInv[L];
+            constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
+            const uint32_t q = nb.q[L];
+            if (q != kSolidSlot) {
+                const double nt = a0(L) * rr + a1(L) * dot3<cx, cy, cz>(vx, vy, vz);
+                const double s = nt / rr - a0(L) + a.lambda * a0(L) * 0.5;
+                remv = remv - s;
+                if constexpr (cx != 0) us[0] = us[0] + sgn<cx>(s);
+                if constexpr (cy != 0) us[1] = us[1] + sgn<cy>(s);
+                if constexpr (cz != 0) us[2] = us[2] + sgn<cz>(s);
+                if constexpr (cx != 0) vp[0] = vp[0] + a.bw * s;
+                if constexpr (cy != 0) vp[1] = vp[1] + a.bw * s;
+                if constexpr (cz != 0) vp[2] = vp[2] + a.bw * s;
+                const double rn = a.rho[q];
+                const double wx = a.j[q] + a.fx, wy = a.j[a.Np + q] + a.fy, wz = a.j[2 * a.Np + q] + a.fz;
+                const double ntn = a0(Li) * rn + a1(Li) * dot3<-cx, -cy, -cz>(wx, wy, wz);
+                const double sp = ntn / rn - a0(Li) + a.lambda * a0(Li) * 0.5;
+                a.win[(uint64_t)(L - 1) * a.Np + i] = sp;
+                if constexpr (cx != 0) p1[0] = p1[0] + sgn<-cx>(sp) * a.bw;
+                if constexpr (cy != 0) p1[1] = p1[1] + sgn<-cy>(sp) * a.bw;
+                if constexpr (cz != 0) p1[2] = p1[2] + sgn<-cz>(sp) * a.bw;
+            } else {
+                a.win[(uint64_t)(L - 1) * a.Np + i] = 0.0;
             }
-        }
-    }
-    double tot[3];
-    if (last_block_sum3(vp, a.partials, &a.ctrl->ticket, tot, lds)) {
+        });
+        if (a.ads && (e & kNodeInterfacial)) remv = remv - a.ka;
+        if (remv < a.eps) a.ctrl->restpart = 1;
+        a.rem[i] = remv;
         for (int c = 0; c < 3; ++c) {
-            a.ctrl->vacf0[c] = tot[c];
-            a.ctrl->deff_s[c] = 0.5 * tot[c];
-            a.ctrl->deff_c[c] = 0.0;
+            a.ustar[c * a.Np + i] = us[c];
+            a.P0[c * a.Np + i] = p1[c];
         }
     }
+    block_partial3(vp, a.partials, lds);
 }
 
 struct MpStepArgs {
     const double* __restrict__ win;
     const double* __restrict__ rem;
     const double* __restrict__ ustar;
-    const uint32_t* __restrict__ mask;
+    const uint32_t* __restrict__ nodes;
+    const uint32_t* __restrict__ map;
     const double* __restrict__ Pn;
     double* __restrict__ Px;
     const double* __restrict__ Pan;
     double* __restrict__ Pax;
     double* partials;
-    double* vtrace;   // 3 doubles for this step
     Ctrl* ctrl;
-    uint32_t lx, ly, lz, N;
+    FastDiv dlx, dly;
+    uint32_t lz, Nf;
     uint64_t Np;
     double ka, kd, one_minus_kd;
     int32_t ads;
-    int64_t it;
-    int32_t check_conv;
-    double lim1, lim2;
 };
 
 // One moment-propagation step, module_moment_propagation.f90:207-253:
 // P(r,t+1) = sum_l P(r+c_l,t) s(r+c_l, inv l) + rem(r) P(r,t) [+ kd Pa],
 // Pa(r,t+1) = (1-kd) Pa(r,t) + ka P(r,t) at adsorbing interfacial nodes,
 // VACF(t) = sum_r P(r,t) . u*(r).
+template <bool Wide>
 __global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs a) {
     __shared__ double lds[kBlock / 64 * 3];
     if (vload(&a.ctrl->mp_stop)) return;
-    const uint32_t r = xcd_tile() * kBlock + threadIdx.x;
+    const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
     double vp[3] = {0.0, 0.0, 0.0};
-    if (r < a.N) {
-        const uint32_t m = a.mask[r];
-        if (!(m & kSolidBit)) {
-            Nbr nb;
-            nb.init(r, a.lx, a.ly, a.lz);
-            double acc[3] = {0.0, 0.0, 0.0};
-            static_for<1, kQ>([&](auto I) {
-                constexpr int L = I;
-                if (!((m >> kInv[L]) & 1u)) {
-                    const double w = a.win[(uint64_t)(L - 1) * a.Np + r];
-                    const uint32_t q = nb.at<kCX[L], kCY[L], kCZ[L]>();
-                    acc[0] = acc[0] + a.Pn[q] * w;
-                    acc[1] = acc[1] + a.Pn[a.Np + q] * w;
-                    acc[2] = acc[2] + a.Pn[2 * a.Np + q] * w;
-                }
-            });
-            const double rm = a.rem[r];
-            double p[3];
-            for (int c = 0; c < 3; ++c) {
-                p[c] = a.Pn[c * a.Np + r];
-                vp[c] = p[c] * a.ustar[c * a.Np + r];
-            }
-            if (!a.ads || !(m & kInterfacialBit)) {
-                for (int c = 0; c < 3; ++c) a.Px[c * a.Np + r] = acc[c] + rm * p[c];
-            } else {
-                for (int c = 0; c < 3; ++c) {
-                    const double pa = a.Pan[c * a.Np + r];
-                    a.Px[c * a.Np + r] = (acc[c] + rm * p[c]) + pa * a.kd;
-                    a.Pax[c * a.Np + r] = pa * a.one_minus_kd + p[c] * a.ka;
-                }
-            }
-        }
-    }
-    double tot[3];
-    if (last_block_sum3(vp, a.partials, &a.ctrl->ticket, tot, lds)) {
+    if (i < a.Nf) {
+        const uint32_t Np = (uint32_t)a.Np;
+        const View<Wide> win(a.win, 18 * a.Np), Pn(a.Pn, 3 * a.Np), Px(a.Px, 3 * a.Np), us(a.ustar, 3 * a.Np),
+            rem(a.rem, a.Np);
+        const uint32_t e = ld(a.nodes, i);
+        Neighbours nb;
+        nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
+        double p[3];
         for (int c = 0; c < 3; ++c) {
-            a.vtrace[c] = tot[c];
-            // Neumaier update of D = VACF(0)/2 + sum VACF(t)
-            const double s = a.ctrl->deff_s[c], x = tot[c], u = s + x;
-            a.ctrl->deff_c[c] += (fabs(s) >= fabs(x)) ? (s - u) + x : (x - u) + s;
-            a.ctrl->deff_s[c] = u;
+            p[c] = Pn.ld(c * Np + i);
+            vp[c] = us.ld(c * Np + i);
         }
-        if (a.check_conv && a.it > 2) {
-            bool conv = true;
-            for (int c = 0; c < 3; ++c) conv = conv && fabs(tot[c]) < a.lim1 && fabs(tot[c]) < a.lim2;
-            if (conv) {
-                a.ctrl->mp_conv_step = a.it;
-                a.ctrl->mp_stop = 1;
+        const double rm = rem.ld(i);
+        double acc[3] = {0.0, 0.0, 0.0};
+        static_for<1, kQ>([&](auto I) {
+            constexpr int L = I;
+            const uint32_t q = nb.q[L];
+            if (q != kSolidSlot) {
+                const double w = win.ld((uint32_t)(L - 1) * Np + i);
+                acc[0] = acc[0] + Pn.ld(q) * w;
+                acc[1] = acc[1] + Pn.ld(Np + q) * w;
+                acc[2] = acc[2] + Pn.ld(2 * Np + q) * w;
+            }
+        });
+        for (int c = 0; c < 3; ++c) vp[c] = p[c] * vp[c];
+        if (!a.ads || !(e & kNodeInterfacial)) {
+            for (int c = 0; c < 3; ++c) Px.st(c * Np + i, acc[c] + rm * p[c]);
+        } else {
+            const View<Wide> Pan(a.Pan, 3 * a.Np), Pax(a.Pax, 3 * a.Np);
+            for (int c = 0; c < 3; ++c) {
+                const double pa = Pan.ld(c * Np + i);
+                Px.st(c * Np + i, (acc[c] + rm * p[c]) + pa * a.kd);
+                Pax.st(c * Np + i, pa * a.one_minus_kd + p[c] * a.ka);
             }
         }
     }
+    block_partial3(vp, a.partials, lds);
 }
 
 }  // namespace
@@ -432,20 +500,26 @@ __global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs a) {
 
 struct lbx_ctx {
     int32_t lx = 0, ly = 0, lz = 0;
-    uint32_t N = 0;
-    uint64_t Np = 0;
+    FastDiv dlx{}, dly{};
+    uint32_t N = 0, Nf = 0;
+    uint64_t Np = 0;  // packed plane stride
+    bool wide = false; // 19*Np*8 >= 2^32: 64-bit addressing of the population buffers
     int32_t prec = LBX_PREC_FP64;
     double tau = 1.0, A = 0, B = 0, C = 0;
-    int64_t nf = 0, nfi = 0;
+    int64_t nfi = 0;
     int device = 0;
     hipStream_t stream = nullptr;
     double* f[2] = {nullptr, nullptr};
     double* j = nullptr;
     double* rho = nullptr;
-    uint32_t* mask = nullptr;
+    uint32_t* nodes = nullptr;
+    uint32_t* map = nullptr;
+    uint32_t ntiles = 0;
+    double* part = nullptr;      // per-block partials (3 per tile)
     Ctrl* ctrl = nullptr;
-    unsigned long long* l2ring = nullptr;
-    int pre = 0;  // buffer holding the post-streaming state
+    double* l2ring = nullptr;
+    int pre = 0;                 // buffer holding the post-streaming state
+    std::vector<uint32_t> nodes_h;
     // moment propagation
     bool mp_ready = false;
     double* win = nullptr;
@@ -453,7 +527,6 @@ struct lbx_ctx {
     double* ustar = nullptr;
     double* P[2] = {nullptr, nullptr};
     double* Pa[2] = {nullptr, nullptr};
-    double* partials = nullptr;
     double* vtrace = nullptr;
     int mp_now = 0;
     int64_t mp_it = 0;
@@ -465,7 +538,7 @@ struct lbx_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};
     double t_ms[2] = {0, 0};
     int64_t t_n[2] = {0, 0};
-    std::vector<unsigned long long> l2host;
+    std::vector<double> l2host;
 };
 
 namespace {
@@ -511,7 +584,6 @@ void free_mp(lbx_ctx* c) {
     dfree(c->P[1]);
     dfree(c->Pa[0]);
     dfree(c->Pa[1]);
-    dfree(c->partials);
     dfree(c->vtrace);
     c->mp_ready = false;
 }
@@ -522,7 +594,9 @@ void free_all(lbx_ctx* c) {
     dfree(c->f[1]);
     dfree(c->j);
     dfree(c->rho);
-    dfree(c->mask);
+    dfree(c->nodes);
+    dfree(c->map);
+    dfree(c->part);
     dfree(c->ctrl);
     dfree(c->l2ring);
     for (auto& e : c->ev)
@@ -537,31 +611,9 @@ void relaxation_coeffs(double tau, double* A, double* B, double* C) {
     *C = 1.0 - 1.0 / (2.0 * tau);
 }
 
-std::vector<uint32_t> build_mask(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature,
-                                 const int8_t* interfacial) {
-    const uint64_t N = (uint64_t)lx * ly * lz;
-    std::vector<uint32_t> m(N, 0u);
-    auto wrap = [](int32_t i, int32_t n) { return i < 0 ? n - 1 : (i >= n ? 0 : i); };
-    for (int32_t k = 0; k < lz; ++k)
-        for (int32_t jj = 0; jj < ly; ++jj)
-            for (int32_t i = 0; i < lx; ++i) {
-                const uint64_t r = i + (uint64_t)lx * (jj + (uint64_t)ly * k);
-                uint32_t w = 0;
-                if (nature[r] != 0) w |= kSolidBit;
-                if (interfacial[r]) w |= kInterfacialBit;
-                for (int l = 1; l < kQ; ++l) {
-                    const uint64_t s = wrap(i - kCX[l], lx) + (uint64_t)lx * (wrap(jj - kCY[l], ly) +
-                                                                              (uint64_t)ly * wrap(k - kCZ[l], lz));
-                    if (nature[s] != 0) w |= 1u << l;
-                }
-                m[r] = w;
-            }
-    return m;
-}
-
 int reset_ctrl(lbx_ctx* c) {
     Ctrl h{};
-    h.negpop = kNone;
+    h.negpop = -1;
     return hip_status(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
 }
 
@@ -595,13 +647,15 @@ StepArgs step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3]
     a.fin = c->f[(t - 1) & 1];
     a.fout = c->f[t & 1];
     a.j = c->j;
-    a.mask = c->mask;
+    a.nodes = c->nodes;
+    a.map = c->map;
+    a.part = c->part;
     a.l2ring = c->l2ring;
     a.ctrl = c->ctrl;
-    a.lx = c->lx;
-    a.ly = c->ly;
+    a.dlx = c->dlx;
+    a.dly = c->dly;
     a.lz = c->lz;
-    a.N = c->N;
+    a.Nf = c->Nf;
     a.Np = c->Np;
     a.A = c->A;
     a.B = c->B;
@@ -617,6 +671,26 @@ StepArgs step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3]
     a.target = target;
     a.write_l2 = write_l2;
     return a;
+}
+
+// packed <-> full-box copies on the host (not on the hot path)
+int download_full(lbx_ctx* c, const double* dsrc, int planes, double* host_full) {
+    std::vector<double> tmp((size_t)planes * c->Np);
+    LBX_TRY(hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LBX_TRY(hipStreamSynchronize(c->stream));
+    std::memset(host_full, 0, (size_t)planes * c->N * sizeof(double));
+    for (int p = 0; p < planes; ++p)
+        for (uint32_t i = 0; i < c->Nf; ++i)
+            host_full[(size_t)p * c->N + (c->nodes_h[i] & kIndexMask)] = tmp[(size_t)p * c->Np + i];
+    return LBX_OK;
+}
+
+int upload_full(lbx_ctx* c, const double* const* host_planes, int planes, double* ddst) {
+    std::vector<double> tmp((size_t)planes * c->Np, 0.0);
+    for (int p = 0; p < planes; ++p)
+        for (uint32_t i = 0; i < c->Nf; ++i) tmp[(size_t)p * c->Np + i] = host_planes[p][c->nodes_h[i] & kIndexMask];
+    LBX_TRY(hipMemcpyAsync(ddst, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return hip_status(hipStreamSynchronize(c->stream));
 }
 
 }  // namespace
@@ -658,10 +732,11 @@ int lbx_d3q19_tables(int32_t c[57], double a0o[19], double a1o[19], double a2o[1
 
 int64_t lbx_footprint_bytes(int32_t lx, int32_t ly, int32_t lz, int32_t precision, int32_t with_mp) {
     if (lx <= 0 || ly <= 0 || lz <= 0) return -1;
+    // upper bound: every node fluid
     const uint64_t N = (uint64_t)lx * ly * lz, Np = (N + 63) / 64 * 64;
     const uint64_t pop = precision == LBX_PREC_FP32_SHIFT ? 4 : 8;
-    uint64_t b = 2 * kQ * Np * pop + 4 * Np * 8 + N * 4 + sizeof(Ctrl) + kRing * 8;
-    if (with_mp) b += (18 + 1 + 3 + 12) * Np * 8 + nblocks(N) * 3 * 8 + kMpChunk * 3 * 8;
+    uint64_t b = 2 * kQ * Np * pop + 4 * Np * 8 + N * 8 + nblocks(N) * 3 * 8 + sizeof(Ctrl) + kRing * 8;
+    if (with_mp) b += (18 + 1 + 3 + 12) * Np * 8 + kMpChunk * 3 * 8;
     return (int64_t)b;
 }
 
@@ -670,7 +745,8 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     if (!out) return LBX_ERR_ARG;
     *out = nullptr;
     if (lx <= 0 || ly <= 0 || lz <= 0 || !nature) return LBX_ERR_ARG;
-    if ((uint64_t)lx * ly * lz >= (1ull << 31)) return LBX_ERR_ARG;
+    // packed indices times 8 bytes must fit in 32 bits (the 32-bit offset loads)
+    if ((uint64_t)lx * ly * lz >= (1ull << 29)) return LBX_ERR_ARG;
     if (precision != LBX_PREC_FP64) return LBX_ERR_ARG;
     if (!(tau >= 0.5)) return LBX_ERR_TAU;
     int ndev = 0;
@@ -681,34 +757,45 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     c->ly = ly;
     c->lz = lz;
     c->N = (uint32_t)((uint64_t)lx * ly * lz);
-    c->Np = ((uint64_t)c->N + 63) / 64 * 64;
+    c->dlx.set((uint32_t)lx);
+    c->dly.set((uint32_t)ly);
     c->prec = precision;
     c->tau = tau;
     relaxation_coeffs(tau, &c->A, &c->B, &c->C);
     (void)hipGetDevice(&c->device);
     std::vector<int8_t> itf(c->N);
     lbx_detect_interfacial(lx, ly, lz, nature, itf.data());
-    for (uint32_t r = 0; r < c->N; ++r) {
-        c->nf += nature[r] == 0;
-        c->nfi += nature[r] == 0 && itf[r];
-    }
-    if (c->nf == 0) {
+    std::vector<uint32_t> map(c->N, kSolidSlot);
+    for (uint32_t r = 0; r < c->N; ++r)
+        if (nature[r] == 0) {
+            map[r] = (uint32_t)c->nodes_h.size();
+            c->nodes_h.push_back(r | (itf[r] ? kNodeInterfacial : 0u));
+            c->nfi += itf[r] != 0;
+        }
+    c->Nf = (uint32_t)c->nodes_h.size();
+    if (c->Nf == 0) {
         delete c;
         return LBX_ERR_ARG;
     }
-    std::vector<uint32_t> m = build_mask(lx, ly, lz, nature, itf.data());
+    c->Np = ((uint64_t)c->Nf + 63) / 64 * 64;
+    c->wide = kQ * c->Np * 8 >= (1ull << 32);
+    c->ntiles = nblocks(c->Nf);
     int st = LBX_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = LBX_ERR_DEVICE;
     if (!st) st = dalloc(&c->f[0], kQ * c->Np);
     if (!st) st = dalloc(&c->f[1], kQ * c->Np);
     if (!st) st = dalloc(&c->j, 3 * c->Np);
     if (!st) st = dalloc(&c->rho, c->Np);
-    if (!st) st = dalloc(&c->mask, (uint64_t)c->N);
+    if (!st) st = dalloc(&c->nodes, (uint64_t)c->Nf);
+    if (!st) st = dalloc(&c->map, (uint64_t)c->N);
+    if (!st) st = dalloc(&c->part, 3 * (uint64_t)c->ntiles);
     if (!st) st = dalloc(&c->ctrl, 1);
     if (!st) st = dalloc(&c->l2ring, kRing);
     if (!st && (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess))
         st = LBX_ERR_DEVICE;
-    if (!st) st = hip_status(hipMemcpyAsync(c->mask, m.data(), c->N * sizeof(uint32_t), hipMemcpyHostToDevice,
+    if (!st) st = hip_status(hipMemcpyAsync(c->nodes, c->nodes_h.data(), c->Nf * sizeof(uint32_t),
+                                            hipMemcpyHostToDevice, c->stream));
+    if (!st) st = hip_status(hipMemcpyAsync(c->map, map.data(), c->N * sizeof(uint32_t), hipMemcpyHostToDevice,
                                             c->stream));
     if (!st) st = hip_status(hipMemsetAsync(c->f[0], 0, kQ * c->Np * sizeof(double), c->stream));
     if (!st) st = hip_status(hipMemsetAsync(c->f[1], 0, kQ * c->Np * sizeof(double), c->stream));
@@ -736,8 +823,8 @@ int lbx_destroy(lbx_ctx* c) {
 
 int lbx_init_rest(lbx_ctx* c, double rho0) {
     if (!c) return LBX_ERR_ARG;
-    hipLaunchKernelGGL(k_init_rest, dim3(nblocks(c->N)), dim3(kBlock), 0, c->stream, c->f[0], c->rho, c->j,
-                       c->mask, c->N, c->Np, rho0);
+    hipLaunchKernelGGL(k_init_rest, dim3(c->ntiles), dim3(kBlock), 0, c->stream, c->f[0], c->rho, c->j, c->Nf,
+                       c->Np, rho0);
     c->pre = 0;
     LBX_TRY(hipGetLastError());
     return hip_status(hipStreamSynchronize(c->stream));
@@ -746,25 +833,24 @@ int lbx_init_rest(lbx_ctx* c, double rho0) {
 int lbx_set_state(lbx_ctx* c, const double* n, const double* rho, const double* jx, const double* jy,
                   const double* jz) {
     if (!c || !n || !rho || !jx || !jy || !jz) return LBX_ERR_ARG;
-    const size_t plane = (size_t)c->N * sizeof(double), pitch = (size_t)c->Np * sizeof(double);
-    LBX_TRY(hipMemcpy2DAsync(c->f[0], pitch, n, plane, plane, kQ, hipMemcpyHostToDevice, c->stream));
-    LBX_TRY(hipMemcpyAsync(c->rho, rho, plane, hipMemcpyHostToDevice, c->stream));
-    LBX_TRY(hipMemcpyAsync(c->j, jx, plane, hipMemcpyHostToDevice, c->stream));
-    LBX_TRY(hipMemcpyAsync(c->j + c->Np, jy, plane, hipMemcpyHostToDevice, c->stream));
-    LBX_TRY(hipMemcpyAsync(c->j + 2 * c->Np, jz, plane, hipMemcpyHostToDevice, c->stream));
+    const double* planes[kQ];
+    for (int l = 0; l < kQ; ++l) planes[l] = n + (size_t)l * c->N;
+    LBX_TRY(upload_full(c, planes, kQ, c->f[0]));
+    LBX_TRY(upload_full(c, &rho, 1, c->rho));
+    const double* jp[3] = {jx, jy, jz};
+    LBX_TRY(upload_full(c, jp, 3, c->j));
     c->pre = 0;
-    return hip_status(hipStreamSynchronize(c->stream));
+    return LBX_OK;
 }
 
 int lbx_get_state(lbx_ctx* c, double* n, double* rho, double* jx, double* jy, double* jz) {
     if (!c) return LBX_ERR_ARG;
-    const size_t plane = (size_t)c->N * sizeof(double), pitch = (size_t)c->Np * sizeof(double);
-    if (n) LBX_TRY(hipMemcpy2DAsync(n, plane, c->f[c->pre], pitch, plane, kQ, hipMemcpyDeviceToHost, c->stream));
-    if (rho) LBX_TRY(hipMemcpyAsync(rho, c->rho, plane, hipMemcpyDeviceToHost, c->stream));
-    if (jx) LBX_TRY(hipMemcpyAsync(jx, c->j, plane, hipMemcpyDeviceToHost, c->stream));
-    if (jy) LBX_TRY(hipMemcpyAsync(jy, c->j + c->Np, plane, hipMemcpyDeviceToHost, c->stream));
-    if (jz) LBX_TRY(hipMemcpyAsync(jz, c->j + 2 * c->Np, plane, hipMemcpyDeviceToHost, c->stream));
-    return hip_status(hipStreamSynchronize(c->stream));
+    if (n) LBX_TRY(download_full(c, c->f[c->pre], kQ, n));
+    if (rho) LBX_TRY(download_full(c, c->rho, 1, rho));
+    double* jp[3] = {jx, jy, jz};
+    for (int a = 0; a < 3; ++a)
+        if (jp[a]) LBX_TRY(download_full(c, c->j + a * c->Np, 1, jp[a]));
+    return LBX_OK;
 }
 
 int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64_t max_steps,
@@ -774,20 +860,22 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
     if (exit_reason) *exit_reason = 0;
     const double zero[3] = {0.0, 0.0, 0.0};
     const double F[3] = {fext[0], fext[1], fext[2]};
-    // opening collision of step 1 (stage 1: no force), in place into f[0]
-    hipLaunchKernelGGL(k_collide_field, dim3(nblocks(c->N)), dim3(kBlock), 0, c->stream, c->f[c->pre], c->f[0],
-                       c->rho, c->j, nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, c->mask, c->N, c->Np, c->A, c->B,
-                       c->C);
+    const uint32_t gt = c->ntiles;
+    // opening collision of step 1 (stage 1: no force), into f[0]
+    if (c->pre != 0)
+        LBX_TRY(hipMemcpyAsync(c->f[0], c->f[c->pre], kQ * c->Np * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+    hipLaunchKernelGGL(k_collide_packed, dim3(gt), dim3(kBlock), 0, c->stream, c->f[0], c->rho, c->j, 0.0, 0.0, 0.0,
+                       c->Nf, c->Np, c->A, c->B, c->C);
     LBX_TRY(hipGetLastError());
     LBX_TRY(reset_ctrl(c));
-    LBX_TRY(hipMemsetAsync(c->l2ring, 0, kRing * sizeof(unsigned long long), c->stream));
+    LBX_TRY(hipMemsetAsync(c->l2ring, 0, kRing * sizeof(double), c->stream));
 
     int stage = 1;
     int64_t t = 0;            // last completed step
     int64_t check_after = 2;  // stop tests apply to steps > check_after (t > 2, equilibration.f90:346)
     int status = LBX_OK;
     int32_t reason = 0;
-    const unsigned gb = nblocks(c->N);
     for (;;) {
         int64_t S = kChunk;
         if (max_steps > 0) S = std::min<int64_t>(S, max_steps - t);
@@ -799,38 +887,37 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
         timer_begin(c);
         for (int64_t k = 1; k <= S; ++k) {
             StepArgs a = step_args(c, t + k, Fs, Fs, check_after, target_error, 1);
-            hipLaunchKernelGGL(k_lb_step, dim3(gb), dim3(kBlock), 0, c->stream, a);
+            hipLaunchKernelGGL(c->wide ? k_lb_step<true> : k_lb_step<false>, dim3(gt), dim3(kBlock), 0, c->stream, a);
+            hipLaunchKernelGGL(k_lb_reduce, dim3(1), dim3(kRedBlock), 0, c->stream, c->part, gt, c->l2ring,
+                               c->ctrl, t + k);
         }
         timer_stop(c);
         LBX_TRY(hipGetLastError());
         Ctrl h;
-        LBX_TRY(hipMemcpyAsync(c->l2host.data(), c->l2ring, kRing * sizeof(unsigned long long),
-                               hipMemcpyDeviceToHost, c->stream));
+        LBX_TRY(hipMemcpyAsync(c->l2host.data(), c->l2ring, kRing * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
         LBX_TRY(read_ctrl(c, &h));
         int64_t last = t + S;  // steps actually executed in this chunk
-        if (h.negpop != kNone) last = std::min<int64_t>(last, (int64_t)h.negpop);
+        if (h.negpop >= 0) last = std::min<int64_t>(last, h.negpop);
         if (h.stop) last = std::min<int64_t>(last, h.exit_step);
         timer_end(c, LBX_KERNEL_LB, last - t);
         // l2err of the steps that completed
-        const int64_t last_l2 = (h.negpop != kNone && (int64_t)h.negpop <= last) ? last - 1 : last;
+        const int64_t last_l2 = (h.negpop >= 0 && h.negpop <= last) ? last - 1 : last;
         if (l2err_trace)
-            for (int64_t s = t + 1; s <= last_l2 && s <= trace_cap; ++s)
-                l2err_trace[s - 1] = __builtin_bit_cast(double, c->l2host[s % kRing]);
-        if (h.negpop != kNone) {
+            for (int64_t s = t + 1; s <= last_l2 && s <= trace_cap; ++s) l2err_trace[s - 1] = c->l2host[s % kRing];
+        if (h.negpop >= 0) {
             status = LBX_ERR_NEG_POP;
-            t = (int64_t)h.negpop;
+            t = h.negpop;
             break;
         }
         if (h.stop) {
             const int64_t e = h.exit_step;
             if (stage == 1) {
                 // the first convergence switches the force on at the end of step e
-                // (equilibration.f90:377-386): redo the collision that opened e+1
+                // (equilibration.f90:377-386): redo the collision that opens e+1
                 StepArgs a = step_args(c, e, zero, F, INT64_MAX, target_error, 0);
-                Ctrl clear{};
-                clear.negpop = kNone;
-                LBX_TRY(hipMemcpyAsync(c->ctrl, &clear, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
-                hipLaunchKernelGGL(k_lb_step, dim3(gb), dim3(kBlock), 0, c->stream, a);
+                LBX_TRY(reset_ctrl(c));
+                hipLaunchKernelGGL(c->wide ? k_lb_step<true> : k_lb_step<false>, dim3(gt), dim3(kBlock), 0, c->stream, a);
                 LBX_TRY(hipGetLastError());
                 stage = 2;
                 t = e;
@@ -844,7 +931,7 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
         t += S;
         if (max_steps > 0 && t >= max_steps) {
             // a convergence at the capped step is still reported as such
-            const double l2 = __builtin_bit_cast(double, c->l2host[t % kRing]);
+            const double l2 = c->l2host[t % kRing];
             reason = (stage == 2 && t > check_after && l2 <= target_error) ? LBX_EXIT_CONVERGED : LBX_EXIT_CAP;
             break;
         }
@@ -855,8 +942,8 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
         // post-streaming state of step t into the other buffer (f[t&1] holds
         // the unused opening collision of step t+1), density alongside
         const int src = (t - 1) & 1, dst = t & 1;
-        hipLaunchKernelGGL(k_extract, dim3(gb), dim3(kBlock), 0, c->stream, c->f[src], c->f[dst], c->rho, c->j,
-                           c->mask, c->lx, c->ly, c->lz, c->N, c->Np);
+        hipLaunchKernelGGL(k_extract, dim3(gt), dim3(kBlock), 0, c->stream, c->f[src], c->f[dst], c->rho, c->nodes,
+                           c->map, c->dlx, c->dly, c->lz, c->Nf, c->Np);
         LBX_TRY(hipGetLastError());
         c->pre = dst;
     }
@@ -870,13 +957,12 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, double ka, doub
     if (Db <= eps) return LBX_ERR_ARG;                // drop_tracers.f90:89
     free_mp(c);
     int st = LBX_OK;
-    const uint32_t gb = nblocks(c->N);
+    const uint32_t gt = c->ntiles;
     if (!st) st = dalloc(&c->win, 18 * c->Np);
     if (!st) st = dalloc(&c->rem, c->Np);
     if (!st) st = dalloc(&c->ustar, 3 * c->Np);
     if (!st) st = dalloc(&c->P[0], 3 * c->Np);
     if (!st) st = dalloc(&c->P[1], 3 * c->Np);
-    if (!st) st = dalloc(&c->partials, 3 * (uint64_t)gb);
     if (!st) st = dalloc(&c->vtrace, 3 * (uint64_t)kMpChunk);
     const double K = (std::fabs(kd) <= eps) ? 0.0 : ka / kd;  // :46-50
     c->ads = std::fabs(K) > eps;                                // :52-56
@@ -894,23 +980,24 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, double ka, doub
     c->Db = Db;
     c->ka = ka;
     c->kd = kd;
-    c->lambda = 4.0 * Db / kKbt;                               // calc_lambda, :331-337
-    const double Pstat = (double)c->nf + K * (double)c->nfi;   // :97
+    c->lambda = 4.0 * Db / kKbt;                                // calc_lambda, :331-337
+    const double Pstat = (double)c->Nf + K * (double)c->nfi;    // :97
     c->bw = 1.0 / Pstat;
     MpPrepArgs a;
     a.rho = c->rho;
     a.j = c->j;
-    a.mask = c->mask;
+    a.nodes = c->nodes;
+    a.map = c->map;
     a.win = c->win;
     a.rem = c->rem;
     a.ustar = c->ustar;
     a.P0 = c->P[0];
-    a.partials = c->partials;
+    a.partials = c->part;
     a.ctrl = c->ctrl;
-    a.lx = c->lx;
-    a.ly = c->ly;
+    a.dlx = c->dlx;
+    a.dly = c->dly;
     a.lz = c->lz;
-    a.N = c->N;
+    a.Nf = c->Nf;
     a.Np = c->Np;
     a.fx = fext[0];
     a.fy = fext[1];
@@ -920,7 +1007,9 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, double ka, doub
     a.ka = ka;
     a.eps = eps;
     a.ads = c->ads;
-    hipLaunchKernelGGL(k_mp_prep, dim3(gb), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_mp_prep, dim3(gt), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kRedBlock), 0, c->stream, c->part, gt, nullptr, c->ctrl, 0,
+                       (int64_t)0, 0, 0.0, 0.0);
     LBX_TRY(hipGetLastError());
     Ctrl h;
     LBX_TRY(read_ctrl(c, &h));
@@ -941,11 +1030,9 @@ int lbx_mp_init(lbx_ctx* c, const double fext[3], double Db, double ka, double k
 int lbx_mp_init_from_flow(lbx_ctx* c, const double* rho, const double* jx, const double* jy, const double* jz,
                           const double fext[3], double Db, double ka, double kd, double vacf0[3]) {
     if (!c || !rho || !jx || !jy || !jz || !fext) return LBX_ERR_ARG;
-    const size_t plane = (size_t)c->N * sizeof(double);
-    LBX_TRY(hipMemcpyAsync(c->rho, rho, plane, hipMemcpyHostToDevice, c->stream));
-    LBX_TRY(hipMemcpyAsync(c->j, jx, plane, hipMemcpyHostToDevice, c->stream));
-    LBX_TRY(hipMemcpyAsync(c->j + c->Np, jy, plane, hipMemcpyHostToDevice, c->stream));
-    LBX_TRY(hipMemcpyAsync(c->j + 2 * c->Np, jz, plane, hipMemcpyHostToDevice, c->stream));
+    LBX_TRY(upload_full(c, &rho, 1, c->rho));
+    const double* jp[3] = {jx, jy, jz};
+    LBX_TRY(upload_full(c, jp, 3, c->j));
     return mp_setup(c, fext, Db, ka, kd, vacf0);
 }
 
@@ -959,7 +1046,7 @@ int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* 
     LBX_TRY(read_ctrl(c, &h));
     if (h.restpart) return LBX_ERR_RESTPART;  // the reference stops in the first propagate call
     if (c->mp_stopped) return LBX_OK;
-    const uint32_t gb = nblocks(c->N);
+    const uint32_t gt = c->ntiles;
     const double lim1 = 1.0 / (2.0 * (double)c->lx * (double)c->ly * (double)c->lz / c->Db);  // :284
     const double lim2 = (double)1.e-12f;  // a default-real literal in the reference
     int64_t done = 0;
@@ -973,28 +1060,26 @@ int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* 
             a.win = c->win;
             a.rem = c->rem;
             a.ustar = c->ustar;
-            a.mask = c->mask;
+            a.nodes = c->nodes;
+            a.map = c->map;
             a.Pn = c->P[now];
             a.Px = c->P[now ^ 1];
             a.Pan = c->ads ? c->Pa[now] : nullptr;
             a.Pax = c->ads ? c->Pa[now ^ 1] : nullptr;
-            a.partials = c->partials;
-            a.vtrace = c->vtrace + 3 * k;
+            a.partials = c->part;
             a.ctrl = c->ctrl;
-            a.lx = c->lx;
-            a.ly = c->ly;
+            a.dlx = c->dlx;
+            a.dly = c->dly;
             a.lz = c->lz;
-            a.N = c->N;
+            a.Nf = c->Nf;
             a.Np = c->Np;
             a.ka = c->ka;
             a.kd = c->kd;
             a.one_minus_kd = 1.0 - c->kd;
             a.ads = c->ads;
-            a.it = c->mp_it + k + 1;
-            a.check_conv = stop_on_convergence;
-            a.lim1 = lim1;
-            a.lim2 = lim2;
-            hipLaunchKernelGGL(k_mp_step, dim3(gb), dim3(kBlock), 0, c->stream, a);
+            hipLaunchKernelGGL(c->wide ? k_mp_step<true> : k_mp_step<false>, dim3(gt), dim3(kBlock), 0, c->stream, a);
+            hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kRedBlock), 0, c->stream, c->part, gt, c->vtrace + 3 * k,
+                               c->ctrl, 1, c->mp_it + k + 1, stop_on_convergence, lim1, lim2);
         }
         timer_stop(c);
         LBX_TRY(hipGetLastError());
@@ -1020,25 +1105,19 @@ int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* 
 int lbx_mp_get(lbx_ctx* c, double* P, double* Pa) {
     if (!c) return LBX_ERR_ARG;
     if (!c->mp_ready) return LBX_ERR_STATE;
-    std::vector<double> tmp(3 * (size_t)c->Np);
+    std::vector<double> full(3 * (size_t)c->N);
     auto fetch = [&](const double* src, double* dst) -> int {
-        LBX_TRY(hipMemcpyAsync(tmp.data(), src, 3 * c->Np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        LBX_TRY(hipStreamSynchronize(c->stream));
+        LBX_TRY(download_full(c, src, 3, full.data()));
         for (uint32_t r = 0; r < c->N; ++r)
-            for (int a = 0; a < 3; ++a) dst[3 * (size_t)r + a] = tmp[a * c->Np + r];
+            for (int a = 0; a < 3; ++a) dst[3 * (size_t)r + a] = full[(size_t)a * c->N + r];
         return LBX_OK;
     };
-    if (P) {
-        int st = fetch(c->P[c->mp_now], P);
-        if (st) return st;
-    }
+    if (P) LBX_TRY(fetch(c->P[c->mp_now], P));
     if (Pa) {
-        if (c->ads) {
-            int st = fetch(c->Pa[c->mp_now], Pa);
-            if (st) return st;
-        } else {
+        if (c->ads)
+            LBX_TRY(fetch(c->Pa[c->mp_now], Pa));
+        else
             std::memset(Pa, 0, 3 * (size_t)c->N * sizeof(double));
-        }
     }
     return LBX_OK;
 }
@@ -1068,15 +1147,13 @@ int lbx_collide(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double
     if (!(tau >= 0.5)) return LBX_ERR_TAU;
     const uint64_t N = (uint64_t)lx * ly * lz;
     if (N >= (1ull << 31)) return LBX_ERR_ARG;
-    std::vector<uint32_t> m(N);
-    for (uint64_t r = 0; r < N; ++r) m[r] = nature[r] ? kSolidBit : 0u;
-    double *dn = nullptr, *dr = nullptr, *dj = nullptr, *dfx = nullptr;
-    uint32_t* dm = nullptr;
+    double *dn = nullptr, *dr = nullptr, *dj = nullptr, *dF = nullptr;
+    int8_t* dnat = nullptr;
     int st = dalloc(&dn, kQ * N);
     if (!st) st = dalloc(&dr, N);
     if (!st) st = dalloc(&dj, 3 * N);
-    if (!st) st = dalloc(&dfx, 3 * N);
-    if (!st) st = dalloc(&dm, N);
+    if (!st) st = dalloc(&dF, 3 * N);
+    if (!st) st = dalloc(&dnat, N);
     hipStream_t s = nullptr;
     if (!st) st = hip_status(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     const size_t b = N * sizeof(double);
@@ -1085,15 +1162,15 @@ int lbx_collide(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double
     if (!st) st = hip_status(hipMemcpyAsync(dj, jx, b, hipMemcpyHostToDevice, s));
     if (!st) st = hip_status(hipMemcpyAsync(dj + N, jy, b, hipMemcpyHostToDevice, s));
     if (!st) st = hip_status(hipMemcpyAsync(dj + 2 * N, jz, b, hipMemcpyHostToDevice, s));
-    if (!st) st = hip_status(hipMemcpyAsync(dfx, f_ext_x, b, hipMemcpyHostToDevice, s));
-    if (!st) st = hip_status(hipMemcpyAsync(dfx + N, f_ext_y, b, hipMemcpyHostToDevice, s));
-    if (!st) st = hip_status(hipMemcpyAsync(dfx + 2 * N, f_ext_z, b, hipMemcpyHostToDevice, s));
-    if (!st) st = hip_status(hipMemcpyAsync(dm, m.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dF, f_ext_x, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dF + N, f_ext_y, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dF + 2 * N, f_ext_z, b, hipMemcpyHostToDevice, s));
+    if (!st) st = hip_status(hipMemcpyAsync(dnat, nature, N, hipMemcpyHostToDevice, s));
     if (!st) {
         double A, B, C;
         relaxation_coeffs(tau, &A, &B, &C);
-        hipLaunchKernelGGL(k_collide_field, dim3(nblocks(N)), dim3(kBlock), 0, s, dn, dn, dr, dj, dfx, dfx + N,
-                           dfx + 2 * N, 0.0, 0.0, 0.0, dm, (uint32_t)N, N, A, B, C);
+        hipLaunchKernelGGL(k_collide_box, dim3(nblocks(N)), dim3(kBlock), 0, s, dn, dr, dj, dF, dnat, (uint32_t)N,
+                           A, B, C);
         st = hip_status(hipGetLastError());
     }
     if (!st) st = hip_status(hipMemcpyAsync(n, dn, kQ * b, hipMemcpyDeviceToHost, s));
@@ -1102,8 +1179,8 @@ int lbx_collide(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double
     dfree(dn);
     dfree(dr);
     dfree(dj);
-    dfree(dfx);
-    dfree(dm);
+    dfree(dF);
+    dfree(dnat);
     return st;
 }
 
@@ -1122,7 +1199,7 @@ int lbx_kernel_time(lbx_ctx* c, int32_t which, double* total_ms, int64_t* launch
     return LBX_OK;
 }
 
-int64_t lbx_fluid_nodes(lbx_ctx* c) { return c ? c->nf : -1; }
+int64_t lbx_fluid_nodes(lbx_ctx* c) { return c ? (int64_t)c->Nf : -1; }
 
 int lbx_sync(lbx_ctx* c) {
     if (!c) return LBX_ERR_ARG;

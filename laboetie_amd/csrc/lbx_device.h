@@ -88,15 +88,88 @@ __device__ __forceinline__ double collide_pop(double f, double rho, double jx, d
     return (A * f + B * neq) + C * g;
 }
 
+// Division by a runtime-invariant divisor 1 <= d < 2^31 as a multiply-high
+// and a shift: with s = ceil(log2 d) and m = ceil(2^(31+s) / d) (< 2^32),
+// floor(n/d) = (n*m) >> (31+s) for every n < 2^31, because the rounding
+// error of m times n stays below 2^(31+s).
+struct FastDiv {
+    uint32_t d, mul, shift;
+    void set(uint32_t div) {
+        d = div;
+        uint32_t s = 0;
+        while ((1ull << s) < div) ++s;
+        shift = s;
+        mul = (uint32_t)(((1ull << (31 + s)) + div - 1) / div);
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t n) const {
+        return d == 1 ? n : (__umulhi(n, mul) >> (shift - 1));
+    }
+};
+
+// Load / store through a uniform base pointer and a 32-bit element index:
+// the byte offset idx*8 fits in 32 bits (indices < 2^29), which lets the
+// compiler use the SGPR-base + 32-bit VGPR-offset addressing form instead of
+// a 64-bit address per lane.
+template <typename T>
+__device__ __forceinline__ T ld(const T* base, uint32_t idx) {
+    return *(const T*)((const char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st(T* base, uint32_t idx, T v) {
+    *(T*)((char*)base + (uint32_t)(idx * (uint32_t)sizeof(T))) = v;
+}
+
+// One array (a population plane, a flux component) as a raw buffer resource:
+// loads and stores take a 32-bit byte offset in a VGPR against a descriptor in
+// SGPRs, so a lane needs one address register instead of a 64-bit pointer, and
+// an access outside [0, bytes) reads 0 / writes nothing instead of faulting.
+// 0x00020000 is the gfx9-family descriptor word 3 for raw 32-bit-dword data.
+constexpr int kBufWord3 = 0x00020000;
+struct Plane {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Plane(const void* p, uint64_t bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0,
+                                              (int)(uint32_t)(bytes < 0xFFFFFFFFull ? bytes : 0xFFFFFFFFull),
+                                              kBufWord3)) {}
+    __device__ __forceinline__ double ld(uint32_t idx) const {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8u, 0, 0));
+    }
+    __device__ __forceinline__ void st(uint32_t idx, double v) const {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+                                              idx * 8u, 0, 0);
+    }
+};
+
+// A set of planes of doubles addressed by a 32-bit element index.  Narrow:
+// a buffer resource over the whole set (needs index*8 < 2^32); Wide: plain
+// 64-bit pointers (any size).
+template <bool Wide>
+struct View;
+template <>
+struct View<false> {
+    Plane p;
+    __device__ __forceinline__ View(const double* base, uint64_t n) : p(base, n * 8) {}
+    __device__ __forceinline__ double ld(uint32_t idx) const { return p.ld(idx); }
+    __device__ __forceinline__ void st(uint32_t idx, double v) const { p.st(idx, v); }
+};
+template <>
+struct View<true> {
+    double* b;
+    __device__ __forceinline__ View(const double* base, uint64_t) : b(const_cast<double*>(base)) {}
+    __device__ __forceinline__ double ld(uint32_t idx) const { return b[(uint64_t)idx]; }
+    __device__ __forceinline__ void st(uint32_t idx, double v) const { b[(uint64_t)idx] = v; }
+};
+
 // Periodic neighbour offsets of a node: X[1+d] is the x index of x+d, and Y,
 // Z the row / plane offsets of y+d, z+d (module_system.f90:99-112: only a
 // +-1 step wraps; an axis of length 1 wraps onto itself).
 struct Nbr {
     uint32_t X[3], Y[3], Z[3];
-    __device__ __forceinline__ void init(uint32_t r, uint32_t lx, uint32_t ly, uint32_t lz) {
-        const uint32_t q = r / lx;
+    __device__ __forceinline__ void init(uint32_t r, const FastDiv& dlx, const FastDiv& dly, uint32_t lz) {
+        const uint32_t lx = dlx.d, ly = dly.d;
+        const uint32_t q = dlx.div(r);
         const uint32_t x = r - q * lx;
-        const uint32_t z = q / ly;
+        const uint32_t z = dly.div(q);
         const uint32_t y = q - z * ly;
         const uint32_t lxy = lx * ly;
         X[0] = x == 0 ? lx - 1 : x - 1;
