@@ -80,48 +80,82 @@ def lb_in_text(w, fext, mp_steps, ka, print_frequency=1):
             f"tracer_Db = {w['Db']}\ntracer_ka = {ka}\ntracer_kd = {w['kd']}\n")
 
 
-def _sample_growth(exe, d, fname, window, threads, max_wait):
-    """Run the reference in d until `fname` has grown for `window` seconds;
-    return steps/second from its line count."""
+def _run_on_pty(exe, d, threads, stop, max_wait):
+    """Run the reference in directory d with its stdout on a pseudo-terminal
+    (so the Fortran runtime writes each line as it is printed) and return the
+    (arrival time, line) pairs; stop(lines) ends the run early."""
+    import pty
+    import select
+
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    path = os.path.join(d, "output", fname)
-    p = subprocess.Popen([exe], cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-    t_start = time.time()
-    first = None
-    last = None
+    master, slave = pty.openpty()
+    p = subprocess.Popen([exe], cwd=d, env=env, stdout=slave, stderr=slave, stdin=subprocess.DEVNULL)
+    os.close(slave)
+    lines, buf, t0 = [], b"", time.time()
     try:
-        while time.time() - t_start < max_wait:
-            time.sleep(0.25)
-            if p.poll() is not None:
-                break
-            if not os.path.exists(path):
-                continue
-            with open(path, "rb") as fh:
-                n = sum(1 for _ in fh)
-            now = time.time()
-            if first is None and n >= 2:
-                first = (now, n)
-            elif first is not None:
-                last = (now, n)
-                if now - first[0] >= window:
+        while time.time() - t0 < max_wait:
+            r, _, _ = select.select([master], [], [], 0.5)
+            if r:
+                try:
+                    chunk = os.read(master, 65536)
+                except OSError:
                     break
+                if not chunk:
+                    break
+                buf += chunk
+                now = time.time()
+                while b"\n" in buf:
+                    ln, buf = buf.split(b"\n", 1)
+                    lines.append((now, ln.decode(errors="replace").strip()))
+                if stop(lines):
+                    break
+            elif p.poll() is not None:
+                break
     finally:
         if p.poll() is None:
             p.kill()
-            p.wait()
-    if first is None or last is None or last[1] <= first[1]:
+        p.wait()
+        os.close(master)
+    return lines
+
+
+def _lb_steps_per_second(exe, d, threads, window):
+    """The equilibration prints `t l2err (target ...)` every step with
+    print_frequency = 1 (equilibration.f90:149); time those lines."""
+    def steps(lines):
+        return [(t, int(l.split()[0])) for t, l in lines if l.endswith(")") and "(target" in l]
+
+    def stop(lines):
+        s = steps(lines)
+        return len(s) >= 2 and s[-1][0] - s[0][0] >= window
+
+    s = steps(_run_on_pty(exe, d, threads, stop, max_wait=300))
+    if len(s) < 2 or s[-1][1] == s[0][1]:
         return None
-    return (last[1] - first[1]) / (last[0] - first[0])
+    return (s[-1][1] - s[0][1]) / (s[-1][0] - s[0][0])
 
 
-def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0):
+def _mp_steps_per_second(exe, d, threads, nsteps):
+    """Time nsteps moment-propagation steps: from the `0 VACF(0)` line printed
+    at the end of moment_propagation::init (module_moment_propagation.f90:139)
+    to the closing status line of print_tail (module_io.f90:64)."""
+    lines = _run_on_pty(exe, d, threads, lambda ls: any("status OK" in l for _, l in ls), max_wait=600)
+    t_init = next((t for t, l in lines if l.split()[:1] == ["0"] and len(l.split()) == 4), None)
+    t_end = next((t for t, l in lines if "status OK" in l), None)
+    if t_init is None or t_end is None or t_end <= t_init:
+        return None
+    return nsteps / (t_end - t_init)
+
+
+def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0, mp_sample=30):
     """The reference program (oracle/_ref/laboetie_fast: flang -O3 -fopenmp
     -funroll-loops, the reference Makefile's flags) on this machine's host
     cores, timed on a bounded sample of the same workload: the LB step rate
-    from the growth of output/l2err.dat over `window` seconds of the 128^3
-    BCC run, the MP step rate from output/vacf.dat of a run whose flow is at
-    rest (f_ext = 0, so equilibration ends at t = 4; MP cost does not depend
-    on the flow).  Whole-job MLUPS = N (LB + MP steps) / (LB/rate + MP/rate)."""
+    from its per-step stdout lines over `window` seconds of the 128^3 BCC
+    run, the MP step rate from a run of mp_sample steps of the dF = -1 tracer
+    on a flow at rest (f_ext = 0, so equilibration ends at t = 4; the MP cost
+    does not depend on the flow values).  Whole-job MLUPS =
+    N (LB + MP steps) / (LB/rate + MP/rate)."""
     exe = os.path.join(HERE, "oracle", "_ref", "laboetie_fast")
     if not os.path.exists(exe):
         return None
@@ -130,12 +164,13 @@ def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0):
     print("cpu_baseline: timing the reference LB phase", file=sys.stderr, flush=True)
     with tempfile.TemporaryDirectory() as d:
         open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, w["fext"], 0, 0.0))
-        lb_rate = _sample_growth(exe, d, "l2err.dat", window, threads, max_wait=240)
+        lb_rate = _lb_steps_per_second(exe, d, threads, window)
     print(f"cpu_baseline: LB {lb_rate} steps/s; timing the reference MP phase", file=sys.stderr, flush=True)
     with tempfile.TemporaryDirectory() as d:
         ka = w["kd"] * np.exp(1.0)
-        open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, (0.0, 0.0, 0.0), 10 ** 7, ka))
-        mp_rate = _sample_growth(exe, d, "vacf.dat", window, threads, max_wait=240)
+        open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, (0.0, 0.0, 0.0), mp_sample, ka))
+        mp_rate = _mp_steps_per_second(exe, d, threads, mp_sample)
+    print(f"cpu_baseline: MP {mp_rate} steps/s", file=sys.stderr, flush=True)
     if not lb_rate or not mp_rate:
         return None
     t_job = lb_steps / lb_rate + mp_steps_total / mp_rate
@@ -143,8 +178,8 @@ def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0):
             "cores": threads, "kind": "reference",
             "sample": (f"reference laboetie (flang -O3 -fopenmp -funroll-loops), OMP_NUM_THREADS={threads}: "
                        f"LB {lb_rate:.3f} steps/s over {window:.0f} s of the 128^3 BCC run, MP "
-                       f"{mp_rate:.3f} steps/s over {window:.0f} s (dF=-1 tracer), combined over this job's "
-                       f"{lb_steps} LB + {mp_steps_total} MP steps"),
+                       f"{mp_rate:.3f} steps/s over {mp_sample} steps (dF=-1 tracer), combined over this "
+                       f"job's {lb_steps} LB + {mp_steps_total} MP steps"),
             "lb_mlups": round(N * lb_rate / 1e6, 4), "mp_mlups": round(N * mp_rate / 1e6, 4)}
 
 

@@ -39,7 +39,7 @@ using namespace lbx;
 namespace {
 
 constexpr int kBlock = 256;          // one tile = 256 consecutive packed nodes
-constexpr int kRedBlock = 1024;      // the one-block reduction kernels
+constexpr int kSlots = 64;           // striped l2err slots per step
 constexpr int kRing = 1024;          // l2err ring slots (>= 2 chunks)
 constexpr int kChunk = 256;          // LB steps launched between host checks
 constexpr int kMpChunk = 512;        // MP steps per host check
@@ -88,8 +88,8 @@ struct StepArgs {
     double* __restrict__ j;
     const uint32_t* __restrict__ nodes;
     const uint32_t* __restrict__ map;
-    double* part;                         // per-block max|dj| (-1: a negative population)
-    const double* l2ring;
+    unsigned long long* slots;            // [3][kSlots] striped per-step max|dj| bit patterns
+    double* l2ring;
     Ctrl* ctrl;
     FastDiv dlx, dly;
     uint32_t lz, Nf;
@@ -101,7 +101,26 @@ struct StepArgs {
     int64_t check_after;    // stop check on l2err(t-1) only when t-1 > check_after
     double target;
     int32_t write_l2;       // 0 for a redo of a step whose l2err is already known
+    int32_t have_prev;      // step t-1 ran in this run: reduce its l2err first
 };
+
+// A negative population among a block's nodes is reported as +inf in place of
+// its max|dj| (max over bit patterns of non-negative doubles = max of values).
+constexpr unsigned long long kNegMark = 0x7FF0000000000000ull;
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// max over the kSlots slots of one step (called by the first wave of a block)
+__device__ __forceinline__ unsigned long long slots_max(const unsigned long long* set) {
+    return wave_max_u64(*(const volatile unsigned long long*)&set[threadIdx.x & (kSlots - 1)]);
+}
 
 // Pull-streaming with half-way bounce-back (equilibration.f90:204-243): the
 // population arriving at node i in direction L left r - c_L, or, when that
@@ -128,16 +147,32 @@ __device__ __forceinline__ double pull(const View<Wide>& fin, uint32_t Np, uint3
 template <bool Wide>
 __global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs a) {
     if (vload(&a.ctrl->stop) || vload(&a.ctrl->negpop) >= 0) return;
-    if (a.t - 1 > a.check_after) {
-        const double prev = vload(&a.l2ring[(a.t - 1) % kRing]);
-        if (prev <= a.target) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (a.have_prev) {
+        // l2err of step t-1 (equilibration.f90:339-344) from its striped
+        // slots, then the negativity (:248) and convergence (:346) verdicts
+        __shared__ unsigned long long prev_bits;
+        if (threadIdx.x < 64) {
+            const unsigned long long v = slots_max(a.slots + ((a.t - 1) % 3) * kSlots);
+            if (threadIdx.x == 0) prev_bits = v;
+        }
+        __syncthreads();
+        const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+        if (prev_bits == kNegMark) {
+            if (first) a.ctrl->negpop = a.t - 1;
+            return;
+        }
+        const double prev = __longlong_as_double((long long)prev_bits);
+        if (first) a.l2ring[(a.t - 1) % kRing] = prev;
+        if (a.t - 1 > a.check_after && prev <= a.target) {
+            if (first) {
                 a.ctrl->exit_step = a.t - 1;
                 a.ctrl->stop = 1;
             }
             return;
         }
     }
+    // clear the slots of step t+1 (last used by step t-2, read by step t-1)
+    if (a.write_l2 && blockIdx.x == 0 && threadIdx.x < kSlots) a.slots[((a.t + 1) % 3) * kSlots + threadIdx.x] = 0ull;
     const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
     double l2 = 0.0;
     bool neg = false;
@@ -191,37 +226,23 @@ __global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs a) {
                 bneg = bneg || lds[w] < 0.0;
                 m = fmax(m, lds[w]);
             }
-            a.part[blockIdx.x] = bneg ? -1.0 : m;
+            if (bneg || m > 0.0)
+                atomicMax(&a.slots[(a.t % 3) * kSlots + (blockIdx.x & (kSlots - 1))],
+                          bneg ? kNegMark : (unsigned long long)__double_as_longlong(m));
         }
     }
 }
 
-// l2err of step t (equilibration.f90:339-344) from the per-block maxima, and
-// the negativity verdict (:248); one block, fixed order.
-__global__ __launch_bounds__(kRedBlock) void k_lb_reduce(const double* __restrict__ part, uint32_t nparts,
-                                                         double* l2ring, Ctrl* ctrl, int64_t t) {
-    __shared__ double lds[kRedBlock / 64];
+// l2err of the last step of a launch run, which no later step kernel reads.
+__global__ __launch_bounds__(64) void k_lb_finalize(const unsigned long long* slots, double* l2ring, Ctrl* ctrl,
+                                                    int64_t t) {
     if (vload(&ctrl->stop) || vload(&ctrl->negpop) >= 0) return;
-    double m = 0.0;
-    bool neg = false;
-    for (uint32_t k = threadIdx.x; k < nparts; k += kRedBlock) {
-        const double v = part[k];
-        neg = neg || v < 0.0;
-        m = fmax(m, v);
-    }
-    m = wave_max(m);
-    const bool wneg = __ballot(neg) != 0ull;
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = wneg ? -1.0 : m;
-    __syncthreads();
+    const unsigned long long v = slots_max(slots + (t % 3) * kSlots);
     if (threadIdx.x == 0) {
-        double mm = 0.0;
-        bool bneg = false;
-        for (int w = 0; w < kRedBlock / 64; ++w) {
-            bneg = bneg || lds[w] < 0.0;
-            mm = fmax(mm, lds[w]);
-        }
-        l2ring[t % kRing] = mm;
-        if (bneg) ctrl->negpop = t;
+        if (v == kNegMark)
+            ctrl->negpop = t;
+        else
+            l2ring[t % kRing] = __longlong_as_double((long long)v);
     }
 }
 
@@ -311,18 +332,20 @@ __device__ __forceinline__ void block_partial3(double v[3], double* part, double
 }
 
 // VACF of one moment-propagation step from the per-block partial sums, in a
-// fixed order (so every run gives the same bits), then the running D_eff sum
-// and the convergence test of module_moment_propagation.f90:284-288.
-// mode 0: VACF(0) of mp init; mode 1: VACF(it) of a step.
-__global__ __launch_bounds__(kRedBlock) void k_mp_reduce(const double* __restrict__ part, uint32_t nparts,
-                                                         double* vtrace, Ctrl* ctrl, int32_t mode, int64_t it,
-                                                         int32_t check_conv, double lim1, double lim2) {
-    __shared__ double lds[kRedBlock / 64 * 3];
-    if (mode == 1 && vload(&ctrl->mp_stop)) return;
+// fixed order that does not depend on which kernel runs it (so every run gives
+// the same bits), then the running D_eff sum and, optionally, the convergence
+// test of module_moment_propagation.f90:284-288.  Called by all kBlock threads
+// of one block.  mode 0: VACF(0) of mp init; mode 1: VACF(it) of a step.
+__device__ __forceinline__ void mp_reduce_block(const double* part, uint32_t nparts, double* vtrace, Ctrl* ctrl,
+                                                int32_t mode, int64_t it, int32_t check_conv, double lim1,
+                                                double lim2, double* lds) {
     double s[3] = {0.0, 0.0, 0.0};
-    for (uint32_t k = threadIdx.x; k < nparts; k += kRedBlock)
-        for (int c = 0; c < 3; ++c) s[c] = (k == threadIdx.x) ? part[3 * k + c] : s[c] + part[3 * k + c];
-    block_sum3<kRedBlock>(s, lds);
+    for (uint32_t k = threadIdx.x; k < nparts; k += kBlock)
+        for (int c = 0; c < 3; ++c) {
+            const double x = *(const volatile double*)&part[3 * k + c];
+            s[c] = (k == threadIdx.x) ? x : s[c] + x;
+        }
+    block_sum3<kBlock>(s, lds);
     if (threadIdx.x != 0) return;
     if (mode == 0) {
         for (int c = 0; c < 3; ++c) {
@@ -347,6 +370,14 @@ __global__ __launch_bounds__(kRedBlock) void k_mp_reduce(const double* __restric
             ctrl->mp_stop = 1;
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mp_reduce(const double* __restrict__ part, uint32_t nparts,
+                                                      double* vtrace, Ctrl* ctrl, int32_t mode, int64_t it,
+                                                      int32_t check_conv, double lim1, double lim2) {
+    __shared__ double lds[kBlock / 64 * 3];
+    if (mode == 1 && vload(&ctrl->mp_stop)) return;
+    mp_reduce_block(part, nparts, vtrace, ctrl, mode, it, check_conv, lim1, lim2, lds);
 }
 
 struct MpPrepArgs {
@@ -436,7 +467,10 @@ struct MpStepArgs {
     double* __restrict__ Px;
     const double* __restrict__ Pan;
     double* __restrict__ Pax;
-    double* partials;
+    double* partials;           // this step's per-block VACF partials
+    const double* prev_partials; // the previous step's, reduced by block 0 (NULL: none)
+    double* prev_vtrace;        // where VACF(it-1) goes
+    int64_t it;
     Ctrl* ctrl;
     FastDiv dlx, dly;
     uint32_t lz, Nf;
@@ -453,6 +487,8 @@ template <bool Wide>
 __global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs a) {
     __shared__ double lds[kBlock / 64 * 3];
     if (vload(&a.ctrl->mp_stop)) return;
+    if (a.prev_partials && blockIdx.x == 0)
+        mp_reduce_block(a.prev_partials, gridDim.x, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
     const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
     double vp[3] = {0.0, 0.0, 0.0};
     if (i < a.Nf) {
@@ -515,7 +551,8 @@ struct lbx_ctx {
     uint32_t* nodes = nullptr;
     uint32_t* map = nullptr;
     uint32_t ntiles = 0;
-    double* part = nullptr;      // per-block partials (3 per tile)
+    double* part = nullptr;      // per-block partials, two buffers of 3 per tile
+    unsigned long long* slots = nullptr;  // [3][kSlots] l2err slots
     Ctrl* ctrl = nullptr;
     double* l2ring = nullptr;
     int pre = 0;                 // buffer holding the post-streaming state
@@ -597,6 +634,7 @@ void free_all(lbx_ctx* c) {
     dfree(c->nodes);
     dfree(c->map);
     dfree(c->part);
+    dfree(c->slots);
     dfree(c->ctrl);
     dfree(c->l2ring);
     for (auto& e : c->ev)
@@ -649,7 +687,7 @@ StepArgs step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3]
     a.j = c->j;
     a.nodes = c->nodes;
     a.map = c->map;
-    a.part = c->part;
+    a.slots = c->slots;
     a.l2ring = c->l2ring;
     a.ctrl = c->ctrl;
     a.dlx = c->dlx;
@@ -670,6 +708,7 @@ StepArgs step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3]
     a.check_after = check_after;
     a.target = target;
     a.write_l2 = write_l2;
+    a.have_prev = 0;
     return a;
 }
 
@@ -788,7 +827,8 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     if (!st) st = dalloc(&c->rho, c->Np);
     if (!st) st = dalloc(&c->nodes, (uint64_t)c->Nf);
     if (!st) st = dalloc(&c->map, (uint64_t)c->N);
-    if (!st) st = dalloc(&c->part, 3 * (uint64_t)c->ntiles);
+    if (!st) st = dalloc(&c->part, 6 * (uint64_t)c->ntiles);
+    if (!st) st = dalloc(&c->slots, 3 * kSlots);
     if (!st) st = dalloc(&c->ctrl, 1);
     if (!st) st = dalloc(&c->l2ring, kRing);
     if (!st && (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess))
@@ -870,6 +910,7 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
     LBX_TRY(hipGetLastError());
     LBX_TRY(reset_ctrl(c));
     LBX_TRY(hipMemsetAsync(c->l2ring, 0, kRing * sizeof(double), c->stream));
+    LBX_TRY(hipMemsetAsync(c->slots, 0, 3 * kSlots * sizeof(unsigned long long), c->stream));
 
     int stage = 1;
     int64_t t = 0;            // last completed step
@@ -887,11 +928,11 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
         timer_begin(c);
         for (int64_t k = 1; k <= S; ++k) {
             StepArgs a = step_args(c, t + k, Fs, Fs, check_after, target_error, 1);
+            a.have_prev = t + k > 1;
             hipLaunchKernelGGL(c->wide ? k_lb_step<true> : k_lb_step<false>, dim3(gt), dim3(kBlock), 0, c->stream, a);
-            hipLaunchKernelGGL(k_lb_reduce, dim3(1), dim3(kRedBlock), 0, c->stream, c->part, gt, c->l2ring,
-                               c->ctrl, t + k);
         }
         timer_stop(c);
+        hipLaunchKernelGGL(k_lb_finalize, dim3(1), dim3(kSlots), 0, c->stream, c->slots, c->l2ring, c->ctrl, t + S);
         LBX_TRY(hipGetLastError());
         Ctrl h;
         LBX_TRY(hipMemcpyAsync(c->l2host.data(), c->l2ring, kRing * sizeof(double), hipMemcpyDeviceToHost,
@@ -1008,7 +1049,7 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, double ka, doub
     a.eps = eps;
     a.ads = c->ads;
     hipLaunchKernelGGL(k_mp_prep, dim3(gt), dim3(kBlock), 0, c->stream, a);
-    hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kRedBlock), 0, c->stream, c->part, gt, nullptr, c->ctrl, 0,
+    hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kBlock), 0, c->stream, c->part, gt, nullptr, c->ctrl, 0,
                        (int64_t)0, 0, 0.0, 0.0);
     LBX_TRY(hipGetLastError());
     Ctrl h;
@@ -1066,7 +1107,13 @@ int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* 
             a.Px = c->P[now ^ 1];
             a.Pan = c->ads ? c->Pa[now] : nullptr;
             a.Pax = c->ads ? c->Pa[now ^ 1] : nullptr;
-            a.partials = c->part;
+            const int64_t it = c->mp_it + k + 1;
+            // partial buffers alternate by step; without a convergence test the
+            // previous step's VACF is reduced by block 0 of this step's kernel
+            a.partials = c->part + 3 * (uint64_t)gt * (it & 1);
+            a.prev_partials = (!stop_on_convergence && k > 0) ? c->part + 3 * (uint64_t)gt * ((it - 1) & 1) : nullptr;
+            a.prev_vtrace = c->vtrace + 3 * (k - 1);
+            a.it = it;
             a.ctrl = c->ctrl;
             a.dlx = c->dlx;
             a.dly = c->dly;
@@ -1078,10 +1125,17 @@ int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* 
             a.one_minus_kd = 1.0 - c->kd;
             a.ads = c->ads;
             hipLaunchKernelGGL(c->wide ? k_mp_step<true> : k_mp_step<false>, dim3(gt), dim3(kBlock), 0, c->stream, a);
-            hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kRedBlock), 0, c->stream, c->part, gt, c->vtrace + 3 * k,
-                               c->ctrl, 1, c->mp_it + k + 1, stop_on_convergence, lim1, lim2);
+            if (stop_on_convergence)
+                hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kBlock), 0, c->stream, a.partials, gt,
+                                   c->vtrace + 3 * k, c->ctrl, 1, it, 1, lim1, lim2);
         }
         timer_stop(c);
+        if (!stop_on_convergence) {  // the last step's VACF
+            const int64_t it = c->mp_it + S;
+            hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kBlock), 0, c->stream,
+                               c->part + 3 * (uint64_t)gt * (it & 1), gt, c->vtrace + 3 * (S - 1), c->ctrl, 1, it, 0,
+                               lim1, lim2);
+        }
         LBX_TRY(hipGetLastError());
         LBX_TRY(hipMemcpyAsync(vt.data(), c->vtrace, 3 * S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         LBX_TRY(read_ctrl(c, &h));
