@@ -189,7 +189,12 @@ def main():
     ap.add_argument("--steps", type=int, default=3, help="timed full solves")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the plain-copy bandwidth probe (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        print(json.dumps({"copy_gbs": round(lbx.copy_bandwidth(1 << 30, 20), 1), "bytes_each_way": 1 << 30}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
@@ -213,6 +218,7 @@ def main():
         dt = time.perf_counter() - t0
         lb_ms, lb_n = L.kernel_time(lbx.KERNEL_LB)
         mp_ms, mp_n = L.kernel_time(lbx.KERNEL_MP)
+    copy_gbs = lbx.copy_bandwidth(2 << 30, 20)  # measured ceiling, outside the timed region
     lb_steps, mp_steps, deffs = results[-1]
     updates = N * sum(r[0] + r[1] for r in results)
     value = updates / dt / 1e6
@@ -241,6 +247,7 @@ def main():
                    "precision": "fp64"},
         "roofline": roof_lb if dominant == "lb_step" else roof_mp,
         "roofline_lb": roof_lb, "roofline_mp": roof_mp,
+        "copy_ceiling_gbs": round(copy_gbs, 1),
         "lb_mlups": round(N * lb_steps * args.steps / (lb_ms / 1e3) / 1e6, 1),
         "mp_mlups": round(N * mp_steps * args.steps / (mp_ms / 1e3) / 1e6, 1),
         "deff": deffs,

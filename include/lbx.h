@@ -170,6 +170,13 @@ int lbx_kernel_time(lbx_ctx* ctx, int32_t which, double* total_ms, int64_t* laun
 /* Fluid-node count of the context (the MFLUPS denominator). */
 int64_t lbx_fluid_nodes(lbx_ctx* ctx);
 
+/* Measurement helper with no counterpart in the reference: read+write
+ * bandwidth (GB/s, 1e9 B/s) of a plain streaming copy of `bytes` bytes,
+ * iters back-to-back launches timed with HIP events.  It is the achievable
+ * ceiling the roofline fractions are compared with, and the known-size
+ * kernel that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters. */
+int lbx_copy_bandwidth(int64_t bytes, int32_t iters, double* gb_per_s);
+
 /* Block until the context's stream is idle. */
 int lbx_sync(lbx_ctx* ctx);
 

@@ -530,6 +530,14 @@ __global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs a) {
     block_partial3(vp, a.partials, lds);
 }
 
+// Plain streaming copy (16 B per lane per iteration, grid-stride): the
+// measured ceiling that the roofline fractions are read against, and the
+// known-size kernel that calibrates the HBM traffic counters.
+__global__ __launch_bounds__(kBlock) void k_copy(const double2* __restrict__ a, double2* __restrict__ b, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        b[i] = a[i];
+}
+
 }  // namespace
 
 // ====================================================================== host
@@ -917,8 +925,10 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
     int64_t check_after = 2;  // stop tests apply to steps > check_after (t > 2, equilibration.f90:346)
     int status = LBX_OK;
     int32_t reason = 0;
+    int64_t chunk = 4;  // grows to kChunk: the force switch normally comes at t = 3
     for (;;) {
-        int64_t S = kChunk;
+        int64_t S = chunk;
+        chunk = std::min<int64_t>(2 * chunk, kChunk);
         if (max_steps > 0) S = std::min<int64_t>(S, max_steps - t);
         if (S <= 0) {
             reason = LBX_EXIT_CAP;
@@ -1254,6 +1264,40 @@ int lbx_kernel_time(lbx_ctx* c, int32_t which, double* total_ms, int64_t* launch
 }
 
 int64_t lbx_fluid_nodes(lbx_ctx* c) { return c ? (int64_t)c->Nf : -1; }
+
+int lbx_copy_bandwidth(int64_t bytes, int32_t iters, double* gb_per_s) {
+    if (bytes < (1 << 20) || iters <= 0 || !gb_per_s) return LBX_ERR_ARG;
+    const uint64_t n = (uint64_t)bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    int st = dalloc(&a, n);
+    if (!st) st = dalloc(&b, n);
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st) st = hip_status(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (!st) st = hip_status(hipEventCreate(&e0));
+    if (!st) st = hip_status(hipEventCreate(&e1));
+    if (!st) st = hip_status(hipMemsetAsync(a, 0, n * sizeof(double2), s));
+    if (!st) st = hip_status(hipMemsetAsync(b, 0, n * sizeof(double2), s));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, 1u << 20);
+    if (!st) {
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, s, a, b, n);  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int32_t k = 0; k < iters; ++k) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, s, a, b, n);
+        (void)hipEventRecord(e1, s);
+        st = hip_status(hipEventSynchronize(e1));
+    }
+    if (!st) {
+        float ms = 0.f;
+        st = hip_status(hipEventElapsedTime(&ms, e0, e1));
+        *gb_per_s = 2.0 * (double)(n * sizeof(double2)) * iters / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    dfree(a);
+    dfree(b);
+    return st;
+}
 
 int lbx_sync(lbx_ctx* c) {
     if (!c) return LBX_ERR_ARG;

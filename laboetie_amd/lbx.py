@@ -50,6 +50,7 @@ SIGNATURES = {
     "lbx_kernel_time": (ctypes.c_int, [_p, _i32, _p, _p]),
     "lbx_fluid_nodes": (_i64, [_p]),
     "lbx_sync": (ctypes.c_int, [_p]),
+    "lbx_copy_bandwidth": (ctypes.c_int, [_i64, _i32, _p]),
 }
 
 
@@ -110,6 +111,13 @@ def detect_interfacial(lx, ly, lz, nature):
 
 def footprint_bytes(lx, ly, lz, precision=PREC_FP64, with_mp=True):
     return int(lib().lbx_footprint_bytes(lx, ly, lz, precision, 1 if with_mp else 0))
+
+
+def copy_bandwidth(bytes_=2 << 30, iters=20):
+    """Read+write GB/s of a plain streaming copy on the device."""
+    g = _f64()
+    _check(lib().lbx_copy_bandwidth(int(bytes_), int(iters), ctypes.byref(g)), "copy_bandwidth")
+    return g.value
 
 
 def collide(dims, nature, tau, n, density, j, f):
