@@ -109,6 +109,12 @@ int lbx_set_state(lbx_ctx* ctx, const double* n, const double* rho,
 int lbx_equilibrate(lbx_ctx* ctx, double target_error, const double fext[3], int64_t max_steps,
                     double* l2err_trace, int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason);
 
+/* The per-step l2err of the last lbx_equilibrate call, steps first_step ..
+ * first_step+count-1 (1-based, as written to output/l2err.dat by
+ * equilibration.f90:339-344); *available (may be NULL) receives how many
+ * steps are held.  count = 0 only queries *available. */
+int lbx_l2err_trace(lbx_ctx* ctx, int64_t first_step, int64_t count, double* out, int64_t* available);
+
 /* State after the last step (equilibration.f90:551-555): n is the
  * post-streaming population, rho = SUM(n,4), j = F/2 + sum n c.  Any pointer
  * may be NULL. */

@@ -584,6 +584,7 @@ struct lbx_ctx {
     double t_ms[2] = {0, 0};
     int64_t t_n[2] = {0, 0};
     std::vector<double> l2host;
+    std::vector<double> l2trace;  // every l2err of the last lbx_equilibrate call
 };
 
 namespace {
@@ -891,6 +892,18 @@ int lbx_set_state(lbx_ctx* c, const double* n, const double* rho, const double* 
     return LBX_OK;
 }
 
+int lbx_l2err_trace(lbx_ctx* c, int64_t first_step, int64_t count, double* out, int64_t* available) {
+    if (!c || first_step < 1 || count < 0 || (count > 0 && !out)) return LBX_ERR_ARG;
+    const int64_t n = (int64_t)c->l2trace.size();
+    if (available) *available = n;
+    for (int64_t k = 0; k < count; ++k) {
+        const int64_t s = first_step + k;
+        if (s > n) return LBX_ERR_ARG;
+        out[k] = c->l2trace[s - 1];
+    }
+    return LBX_OK;
+}
+
 int lbx_get_state(lbx_ctx* c, double* n, double* rho, double* jx, double* jy, double* jz) {
     if (!c) return LBX_ERR_ARG;
     if (n) LBX_TRY(download_full(c, c->f[c->pre], kQ, n));
@@ -917,6 +930,7 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
                        c->Nf, c->Np, c->A, c->B, c->C);
     LBX_TRY(hipGetLastError());
     LBX_TRY(reset_ctrl(c));
+    c->l2trace.clear();
     LBX_TRY(hipMemsetAsync(c->l2ring, 0, kRing * sizeof(double), c->stream));
     LBX_TRY(hipMemsetAsync(c->slots, 0, 3 * kSlots * sizeof(unsigned long long), c->stream));
 
@@ -954,8 +968,11 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
         timer_end(c, LBX_KERNEL_LB, last - t);
         // l2err of the steps that completed
         const int64_t last_l2 = (h.negpop >= 0 && h.negpop <= last) ? last - 1 : last;
-        if (l2err_trace)
-            for (int64_t s = t + 1; s <= last_l2 && s <= trace_cap; ++s) l2err_trace[s - 1] = c->l2host[s % kRing];
+        for (int64_t s = t + 1; s <= last_l2; ++s) {
+            if ((int64_t)c->l2trace.size() < s) c->l2trace.resize(s);
+            c->l2trace[s - 1] = c->l2host[s % kRing];
+            if (l2err_trace && s <= trace_cap) l2err_trace[s - 1] = c->l2host[s % kRing];
+        }
         if (h.negpop >= 0) {
             status = LBX_ERR_NEG_POP;
             t = h.negpop;

@@ -39,6 +39,7 @@ SIGNATURES = {
     "lbx_set_state": (ctypes.c_int, [_p] * 6),
     "lbx_equilibrate": (ctypes.c_int, [_p, _f64, _p, _i64, _p, _i64, _p, _p]),
     "lbx_get_state": (ctypes.c_int, [_p] * 6),
+    "lbx_l2err_trace": (ctypes.c_int, [_p, _i64, _i64, _p, _p]),
     "lbx_mp_init": (ctypes.c_int, [_p, _p, _f64, _f64, _f64, _p]),
     "lbx_mp_init_from_flow": (ctypes.c_int, [_p] * 6 + [_f64] * 3 + [_p]),
     "lbx_mp_run": (ctypes.c_int, [_p, _i64, _i32, _p, _p, _p]),
@@ -183,6 +184,15 @@ class Lattice:
         steps = sd.value
         nl2 = steps - 1 if st == ERR_NEG_POP else steps
         return dict(status=st, steps=steps, exit_reason=er.value, l2err=tr[:min(nl2, trace_cap)].copy())
+
+    def l2err_trace(self):
+        """Every l2err of the last equilibrate call."""
+        n = _i64()
+        _check(lib().lbx_l2err_trace(self._h, 1, 0, None, ctypes.byref(n)), "l2err_trace")
+        out = np.zeros(n.value)
+        if n.value:
+            _check(lib().lbx_l2err_trace(self._h, 1, n.value, _ptr(out), None), "l2err_trace")
+        return out
 
     def get_state(self):
         n = np.zeros(19 * self.N)

@@ -1,0 +1,61 @@
+"""The Fortran drop-in (laboetie_amd/fortran: the reference program with its
+module_collision, equilibration and moment_propagation replaced by the lbx
+shims) against the reference program itself, on the same lb.in: every output
+file the two write must agree -- the LB outputs (l2err.dat, the profiles, the
+2-D flux field, supercell.xsf) character for character, vacf.dat to 1e-12
+relative to VACF(0).  Both binaries are built in the container that holds
+the reference sources (oracle/Makefile, laboetie_amd/fortran/Makefile)."""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+from golden_cases import GOLDEN, load
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_BIN = os.path.join(REPO, "oracle", "_ref", "laboetie")
+GPU_BIN = os.path.join(REPO, "laboetie_amd", "fortran", "_build", "laboetie_gpu")
+
+TEXT_OUTPUTS = ["l2err.dat", "mass-flux_profile_along_x.dat", "mass-flux_profile_along_y.dat",
+                "mass-flux_profile_along_z.dat", "mean-density_profile_along_x.dat",
+                "mean-density_profile_along_y.dat", "mean-density_profile_along_z.dat",
+                "mass-flux_field_2d_at_x.eq.1.dat", "supercell.xsf"]
+
+
+def run_case(exe, lb_in, geom_in, d):
+    open(os.path.join(d, "lb.in"), "w").write(lb_in)
+    if geom_in:
+        open(os.path.join(d, "geom.in"), "w").write(geom_in)
+    p = subprocess.run([exe], cwd=d, env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True,
+                       text=True, timeout=600)
+    return p.returncode, p.stdout + p.stderr
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_BIN) and os.path.exists(GPU_BIN)),
+                    reason="drop-in or reference binary not built (needs the reference sources at build time)")
+@pytest.mark.parametrize("name,mp_steps", [("cfg1_slit", 600), ("bcc12_ads", 400), ("chromat_2disks", 200)])
+def test_dropin_matches_reference_program(name, mp_steps):
+    g = load(name)
+    lb_in = str(g["lb_in"]).replace(
+        [ln for ln in str(g["lb_in"]).splitlines() if ln.startswith("maximum_moment_propagation_steps")][0],
+        f"maximum_moment_propagation_steps = {mp_steps}")
+    print("drop-in case", name, flush=True)
+    with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg:
+        rc_r, out_r = run_case(REF_BIN, lb_in, str(g["geom_in"]), dr)
+        rc_g, out_g = run_case(GPU_BIN, lb_in, str(g["geom_in"]), dg)
+        assert rc_r == 0, out_r
+        assert rc_g == 0, out_g
+        for f in TEXT_OUTPUTS:
+            a = open(os.path.join(dr, "output", f)).read()
+            b = open(os.path.join(dg, "output", f)).read()
+            assert a == b, f
+        vr = np.loadtxt(os.path.join(dr, "output", "vacf.dat"), comments="#")
+        vg = np.loadtxt(os.path.join(dg, "output", "vacf.dat"), comments="#")
+        assert vr.shape == vg.shape == (mp_steps + 1, 4)
+        assert np.array_equal(vr[:, 0], vg[:, 0])
+        assert np.all(np.abs(vr[:, 1:] - vg[:, 1:]) <= 1e-12 * np.abs(vr[0, 1:]).max())
