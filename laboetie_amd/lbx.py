@@ -114,6 +114,18 @@ def footprint_bytes(lx, ly, lz, precision=PREC_FP64, with_mp=True):
     return int(lib().lbx_footprint_bytes(lx, ly, lz, precision, 1 if with_mp else 0))
 
 
+KD_DEFAULT = 0.01
+
+
+def ka_kd_from_dF(dF, kd=KD_DEFAULT):
+    """Adsorption/desorption probabilities per step for an adsorption free
+    energy dF (in kT).  The reference takes ka and kd directly
+    (module_moment_propagation.f90:41-56, K = ka/kd); this build fixes
+    kd = 0.01 and sets K = exp(-dF), so ka = kd exp(-dF) (dF in [-3, 0]
+    gives ka in [0.01, 0.201])."""
+    return kd * float(np.exp(-dF)), kd
+
+
 def copy_bandwidth(bytes_=2 << 30, iters=20):
     """Read+write GB/s of a plain streaming copy on the device."""
     g = _f64()
