@@ -6,7 +6,9 @@ periodic, density 1, tau 1, f_ext = 1e-6 along z; D3Q19 equilibration until
 max|j - j_old| <= 1e-10 (the reference's two-stage loop), then for each
 adsorption free energy dF in {0, -1, -2} kT (kd = 0.01, ka = kd exp(-dF))
 10000 moment-propagation steps of a tracer with Db = 0.01, giving VACF and
-D_eff.  FP64 throughout.
+D_eff.  FP64 throughout.  The three tracers share the flow, so by default one
+kernel propagates them together (--mp-one-at-a-time: one after the other);
+either way every tracer's results are bit-identical to its own run.
 
 metric: lattice-node updates per second over LB and MP steps together
 (lx*ly*lz per step, the usual MLUPS convention), whole job.  The JSON line
@@ -33,26 +35,42 @@ from laboetie_amd import lbx  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
 LB_BYTES_PER_FLUID = 19 * 8 * 2                 # read + write of 19 FP64 populations
-MP_BYTES_PER_FLUID = 19 * 8 + 8 + 2 * 3 * 8     # coefficients (n, rho) + P read/write
-MP_BYTES_PER_ADS_INTERFACIAL = 2 * 3 * 8        # Pa read + write
+MP_COEF_BYTES_PER_FLUID = 19 * 8 + 8            # coefficient source (n, rho), shared by the tracers of a batch
+MP_P_BYTES_PER_FLUID = 2 * 3 * 8                # P read + write, per tracer
+MP_BYTES_PER_ADS_INTERFACIAL = 2 * 3 * 8        # Pa read + write, per adsorbing tracer
 
 WORKLOAD = dict(label=3, dims=(128, 128, 128), tau=1.0, rho0=1.0, fext=(0.0, 0.0, 1e-6), target=1e-10,
                 Db=0.01, kd=0.01, dF=(0.0, -1.0, -2.0), mp_steps=10000)
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_bcc128.txt")
 
 
-def solve(L, w):
-    """One full solve; returns (lb_steps, mp_steps, D_eff per dF)."""
+def tracer_params(w):
+    """ka, kd per dF (ka = kd exp(-dF))."""
+    ka = [lbx.ka_kd_from_dF(dF, w["kd"])[0] for dF in w["dF"]]
+    return ka, [w["kd"]] * len(ka)
+
+
+def solve(L, w, batch=True):
+    """One full solve; returns (lb_steps, MP tracer-steps, D_eff per dF).
+
+    With batch (the default) the tracers of the dF sweep, which share the
+    flow and Db, are propagated together by one kernel (each tracer's results
+    are bit-identical to its own run); otherwise one after the other."""
     L.init_rest(w["rho0"])
     r = L.equilibrate(w["fext"], w["target"], trace_cap=1)
-    deffs = []
-    mp_steps = 0
-    for dF in w["dF"]:
-        ka = w["kd"] * np.exp(-dF)
-        L.mp_init(w["fext"], w["Db"], ka, w["kd"])
+    ka, kd = tracer_params(w)
+    if batch:
+        L.mp_init_batch(w["fext"], w["Db"], ka, kd)
         m = L.mp_run(w["mp_steps"])
-        mp_steps += m["steps"]
-        deffs.append(L.mp_deff().tolist())
+        mp_steps = m["steps"] * len(ka)
+        deffs = L.mp_deff().tolist()
+    else:
+        deffs, mp_steps = [], 0
+        for a, d in zip(ka, kd):
+            L.mp_init(w["fext"], w["Db"], a, d)
+            m = L.mp_run(w["mp_steps"])
+            mp_steps += m["steps"]
+            deffs.append(L.mp_deff().tolist())
     L.sync()
     return r["steps"], mp_steps, deffs
 
@@ -189,6 +207,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3, help="timed full solves")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mp-one-at-a-time", action="store_true",
+                    help="propagate the dF tracers one after the other instead of together")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain-copy bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -208,12 +228,12 @@ def main():
     with lbx.Lattice(w["dims"], nat, w["tau"]) as L:
         nf = L.fluid_nodes
         for _ in range(args.warmup):
-            solve(L, w)
+            solve(L, w, not args.mp_one_at_a_time)
             print("bench: warmup solve done", file=sys.stderr, flush=True)
         L.timing(True)
         L.sync()
         t0 = time.perf_counter()
-        results = [solve(L, w) for _ in range(args.steps)]
+        results = [solve(L, w, not args.mp_one_at_a_time) for _ in range(args.steps)]
         L.sync()
         dt = time.perf_counter() - t0
         lb_ms, lb_n = L.kernel_time(lbx.KERNEL_LB)
@@ -225,7 +245,13 @@ def main():
     lb_avg_s = lb_ms / lb_n / 1e3
     mp_avg_s = mp_ms / mp_n / 1e3
     lb_bytes = LB_BYTES_PER_FLUID * nf
-    mp_bytes = MP_BYTES_PER_FLUID * nf + MP_BYTES_PER_ADS_INTERFACIAL * nfi * 2 / 3  # 2 of 3 dF adsorb
+    # adsorption is on for a tracer when K = ka/kd > eps (module_moment_propagation.f90:46-56)
+    ka, kd = tracer_params(w)
+    ads = [kd_ > np.finfo(float).eps and ka_ / kd_ > np.finfo(float).eps for ka_, kd_ in zip(ka, kd)]
+    per_launch = 1 if args.mp_one_at_a_time else len(ka)  # tracers advanced by one launch
+    n_ads = sum(ads) / len(ads) * per_launch
+    mp_bytes = (MP_COEF_BYTES_PER_FLUID * nf + MP_P_BYTES_PER_FLUID * nf * per_launch
+                + MP_BYTES_PER_ADS_INTERFACIAL * nfi * n_ads)
     lb_gbs = lb_bytes / lb_avg_s / 1e9
     mp_gbs = mp_bytes / mp_avg_s / 1e9
     traffic = read_traffic()
@@ -235,14 +261,16 @@ def main():
                "kernel": "k_lb_step", "bytes_per_launch": lb_bytes, "avg_launch_us": round(lb_avg_s * 1e6, 2)}
     roof_mp = {"bound": "hbm", "achieved": round(mp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(mp_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("k_mp_step"),
-               "kernel": "k_mp_step", "bytes_per_launch": round(mp_bytes), "avg_launch_us": round(mp_avg_s * 1e6, 2)}
+               "kernel": "k_mp_step", "tracers_per_launch": per_launch, "bytes_per_launch": round(mp_bytes),
+               "avg_launch_us": round(mp_avg_s * 1e6, 2)}
     line = {
         "metric": "D3Q19+MP MLUPS & HBM roofline %, FP64 vs FP32-shift; p50/p99 solve latency",
         "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "BCC touching spheres 128^3 (geometryLabel 3), FP64: equilibrate to "
-                               "max|dj|<=1e-10 then 3 x 10000 adsorbing MP steps (dF 0,-1,-2 kT)",
+                               "max|dj|<=1e-10 then 10000 MP steps of 3 adsorbing tracers (dF 0,-1,-2 kT) "
+                               + ("one after the other" if args.mp_one_at_a_time else "propagated together"),
                    "lattice": [lx, ly, lz], "fluid_nodes": nf, "lb_steps": lb_steps, "mp_steps": mp_steps,
                    "precision": "fp64"},
         "roofline": roof_lb if dominant == "lb_step" else roof_mp,
@@ -250,6 +278,7 @@ def main():
         "copy_ceiling_gbs": round(copy_gbs, 1),
         "lb_mlups": round(N * lb_steps * args.steps / (lb_ms / 1e3) / 1e6, 1),
         "mp_mlups": round(N * mp_steps * args.steps / (mp_ms / 1e3) / 1e6, 1),
+        "mp_tracers_together": per_launch,
         "deff": deffs,
     }
     if not args.no_cpu_baseline:

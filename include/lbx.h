@@ -49,6 +49,8 @@ extern "C" {
 /* lbx_equilibrate exit reasons */
 #define LBX_EXIT_CONVERGED 1    /* second convergence, equilibration.f90:373-374 */
 #define LBX_EXIT_CAP       2    /* max_steps reached (extension: the reference has no cap) */
+#define LBX_EXIT_FLOOR     3    /* FP32-shift only: l2err stopped improving above target_error
+                                   (the form's rounding floor, lbx_set_floor_window) */
 
 typedef struct lbx_ctx lbx_ctx;
 
@@ -78,7 +80,8 @@ int lbx_detect_interfacial(int32_t lx, int32_t ly, int32_t lz, const int8_t* nat
 /* --------------------------------------------------------------- context */
 
 /* HBM bytes a context of this shape needs (admission control for batching
- * and serving); with_mp adds the moment-propagation arrays. */
+ * and serving); with_mp > 0 adds the moment-propagation arrays of that many
+ * tracers (lbx_mp_init_batch). */
 int64_t lbx_footprint_bytes(int32_t lx, int32_t ly, int32_t lz, int32_t precision, int32_t with_mp);
 
 /* Create a context on the current HIP device holding one lattice: nature
@@ -135,22 +138,42 @@ int lbx_mp_init_from_flow(lbx_ctx* ctx, const double* rho, const double* jx, con
                           const double* jz, const double fext[3], double Db, double ka, double kd,
                           double vacf0[3]);
 
+/* Several tracers on the same flow with the same Db and different ka, kd
+ * (1 <= ntracers <= 4; ka, kd: ntracers values), propagated together: the
+ * shared scattering probabilities are read once per node per step for all of
+ * them.  Each tracer's P, Pa, VACF and D_eff are bit-identical to those of a
+ * single-tracer lbx_mp_init with its ka, kd (module_moment_propagation.f90
+ * :30-160 per tracer).  vacf0 (may be NULL) receives 3*ntracers values. */
+int lbx_mp_init_batch(lbx_ctx* ctx, const double fext[3], double Db, int32_t ntracers, const double* ka,
+                      const double* kd, double* vacf0);
+
+/* Number of tracers of the current MP state (0 before lbx_mp_init*). */
+int lbx_mp_tracers(lbx_ctx* ctx, int32_t* ntracers);
+
 /* nsteps calls of moment_propagation::propagate (module_moment_propagation.f90
  * :164-289).  vacf_trace (may be NULL) receives VACF(it) for each step taken,
- * 3 per step.  With stop_on_convergence, stops after the first step whose
- * VACF passes the test of :284-288 and sets *converged.  LBX_ERR_RESTPART
- * when a rest probability is below epsilon (the reference stops in the
- * first propagate call). */
+ * 3 per tracer per step (tracer-major within a step).  With
+ * stop_on_convergence (single tracer only, else LBX_ERR_ARG), stops after the
+ * first step whose VACF passes the test of :284-288 and sets *converged.
+ * LBX_ERR_RESTPART when a rest probability is below epsilon (the reference
+ * stops in the first propagate call). */
 int lbx_mp_run(lbx_ctx* ctx, int64_t nsteps, int32_t stop_on_convergence, double* vacf_trace,
                int64_t* steps_done, int32_t* converged);
 
-/* Current P(now) and, with adsorption, Pa(now), (3,lx,ly,lz); either may be NULL. */
+/* Current P(now) and, with adsorption, Pa(now), (3,lx,ly,lz), of tracer 0;
+ * either may be NULL. */
 int lbx_mp_get(lbx_ctx* ctx, double* P, double* Pa);
+
+/* The same for tracer `tracer` (0-based) of a batch. */
+int lbx_mp_get_tracer(lbx_ctx* ctx, int32_t tracer, double* P, double* Pa);
 
 /* Effective diffusion coefficient of the steps run so far (not in the
  * reference: drop_tracers.f90:53-54 is commented out):
  * D_a = VACF_a(0)/2 + sum_{t>=1} VACF_a(t), FP64 with compensated summation. */
 int lbx_mp_deff(lbx_ctx* ctx, double deff[3]);
+
+/* D_eff of every tracer of a batch, 3*ntracers values. */
+int lbx_mp_deff_batch(lbx_ctx* ctx, double* deff);
 
 /* Release the MP arrays (deallocate_propagated_quantity,
  * module_moment_propagation.f90:350-353). */
@@ -172,6 +195,14 @@ int lbx_collide(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double
 #define LBX_KERNEL_MP 1
 int lbx_timing_enable(lbx_ctx* ctx, int32_t on);
 int lbx_kernel_time(lbx_ctx* ctx, int32_t which, double* total_ms, int64_t* launches);
+
+/* FP32-shift form: a stage of lbx_equilibrate also ends when l2err has not
+ * set a new minimum for `steps` consecutive steps (its FP32 rounding floor;
+ * a stage-1 floor switches the force on like a convergence, a stage-2 floor
+ * exits with LBX_EXIT_FLOOR).  0 disables the test.  Default 1000 for
+ * FP32-shift contexts; FP64 contexts keep the reference's semantics and
+ * reject a non-zero window (LBX_ERR_ARG).  No counterpart in the reference. */
+int lbx_set_floor_window(lbx_ctx* ctx, int64_t steps);
 
 /* Fluid-node count of the context (the MFLUPS denominator). */
 int64_t lbx_fluid_nodes(lbx_ctx* ctx);

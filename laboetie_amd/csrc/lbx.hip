@@ -26,6 +26,7 @@
 #include <cstdint>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/lbx.h"
@@ -58,8 +59,12 @@ struct Ctrl {
     int32_t mp_stop;             // MP: VACF convergence reached
     int32_t pad;
     int64_t mp_conv_step;
-    double vacf0[3];
-    double deff_s[3], deff_c[3]; // Neumaier running sums of the VACF integral
+    int32_t floor_hit;           // LB (FP32-shift): the stop at exit_step was the l2err floor test
+    int32_t pad2;
+    double l2min[2];             // LB floor test: running minimum of l2err of the stage up to step s,
+    int64_t l2min_step[2];       // and the step that set it, in slot s & 1
+    double vacf0[12];            // 3 per tracer (at most 4)
+    double deff_s[12], deff_c[12]; // Neumaier running sums of the VACF integrals
 };
 
 __device__ __forceinline__ int32_t vload(const int32_t* p) { return *(const volatile int32_t*)p; }
@@ -82,10 +87,14 @@ struct Neighbours {
 
 // ---------------------------------------------------------------- LB kernels
 
+// Populations, flux and step constants are of type E: double for the FP64
+// form, float for the FP32-shift form (populations stored as deviations
+// g = n - a0 rho0 from the rest state).
+template <typename E>
 struct StepArgs {
-    const double* __restrict__ fin;
-    double* __restrict__ fout;
-    double* __restrict__ j;
+    const E* __restrict__ fin;
+    E* __restrict__ fout;
+    E* __restrict__ j;
     const uint32_t* __restrict__ nodes;
     const uint32_t* __restrict__ map;
     unsigned long long* slots;            // [3][kSlots] striped per-step max|dj| bit patterns
@@ -94,11 +103,13 @@ struct StepArgs {
     FastDiv dlx, dly;
     uint32_t lz, Nf;
     uint64_t Np;
-    double A, B, C;
-    double fmx, fmy, fmz;   // F(t)/2, the force of step t in j(t) (equilibration.f90:293-295)
-    double fcx, fcy, fcz;   // F(t+1), the force of the collision that opens step t+1
+    E A, B, C;
+    E fmx, fmy, fmz;   // F(t)/2, the force of step t in j(t) (equilibration.f90:293-295)
+    E fcx, fcy, fcz;   // F(t+1), the force of the collision that opens step t+1
+    E rho0;            // FP32-shift: the rest density of the shift
     int64_t t;
     int64_t check_after;    // stop check on l2err(t-1) only when t-1 > check_after
+    int64_t floor_window;   // > 0: also stop when l2err set no new minimum for this many steps
     double target;
     int32_t write_l2;       // 0 for a redo of a step whose l2err is already known
     int32_t have_prev;      // step t-1 ran in this run: reduce its l2err first
@@ -127,8 +138,8 @@ __device__ __forceinline__ unsigned long long slots_max(const unsigned long long
 // node is solid, left i itself in direction inv(L).
 // The two candidate sources differ only in the lane's element index, so one
 // load per direction serves both.
-template <int L, bool Wide>
-__device__ __forceinline__ double pull(const View<Wide>& fin, uint32_t Np, uint32_t i, const Neighbours& nb) {
+template <int L, bool Wide, typename E>
+__device__ __forceinline__ E pull(const View<Wide, E>& fin, uint32_t Np, uint32_t i, const Neighbours& nb) {
     if constexpr (L == 0) {
         return fin.ld(i);
     } else {
@@ -141,11 +152,13 @@ __device__ __forceinline__ double pull(const View<Wide>& fin, uint32_t Np, uint3
 // populations of step t, the negativity check (equilibration.f90:248),
 // density (:254) and flux (:293-300), the l2err contribution (:339-344), then
 // the collision that opens step t+1 (module_collision.f90:87-108).
+// E = float: the FP32-shift form; the l2err is still a max of FP64 values.
 #ifndef LBX_LB_WAVES
 #define LBX_LB_WAVES 1
 #endif
-template <bool Wide>
-__global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs a) {
+template <bool Wide, typename E>
+__global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs<E> a) {
+    constexpr bool kShift = std::is_same<E, float>::value;
     if (vload(&a.ctrl->stop) || vload(&a.ctrl->negpop) >= 0) return;
     if (a.have_prev) {
         // l2err of step t-1 (equilibration.f90:339-344) from its striped
@@ -163,9 +176,27 @@ __global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs a) {
         }
         const double prev = __longlong_as_double((long long)prev_bits);
         if (first) a.l2ring[(a.t - 1) % kRing] = prev;
-        if (a.t - 1 > a.check_after && prev <= a.target) {
+        bool floor = false;
+        if (a.floor_window > 0 && a.t - 1 > a.check_after) {
+            // record of the stage up to step t-1 from the one up to t-2 (every
+            // block derives the same record; block 0 stores it for step t+1)
+            const int s0 = (int)((a.t - 2) & 1), s1 = (int)((a.t - 1) & 1);
+            double m = vload(&a.ctrl->l2min[s0]);
+            int64_t ms = vload(&a.ctrl->l2min_step[s0]);
+            if (prev < m) {
+                m = prev;
+                ms = a.t - 1;
+            }
+            if (first) {
+                a.ctrl->l2min[s1] = m;
+                a.ctrl->l2min_step[s1] = ms;
+            }
+            floor = (a.t - 1) - ms >= a.floor_window;
+        }
+        if (a.t - 1 > a.check_after && (prev <= a.target || floor)) {
             if (first) {
                 a.ctrl->exit_step = a.t - 1;
+                a.ctrl->floor_hit = prev <= a.target ? 0 : 1;
                 a.ctrl->stop = 1;
             }
             return;
@@ -178,21 +209,24 @@ __global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs a) {
     bool neg = false;
     if (i < a.Nf) {
         const uint32_t Np = (uint32_t)a.Np;
-        const View<Wide> fin(a.fin, kQ * a.Np), fout(a.fout, kQ * a.Np), jv(a.j, 3 * a.Np);
+        const View<Wide, E> fin(a.fin, kQ * a.Np), fout(a.fout, kQ * a.Np), jv(a.j, 3 * a.Np);
         Neighbours nb;
         nb.load(a.map, ld(a.nodes, i) & kIndexMask, a.dlx, a.dly, a.lz);
-        double jo[3] = {0.0, 0.0, 0.0};
+        E jo[3] = {E(0), E(0), E(0)};
         if (a.write_l2) {  // j_old, issued before the populations so its latency overlaps theirs
             jo[0] = jv.ld(i);
             jo[1] = jv.ld(Np + i);
             jo[2] = jv.ld(2 * Np + i);
         }
-        double f[kQ];
-        static_for<0, kQ>([&](auto I) { f[I] = pull<I, Wide>(fin, Np, i, nb); });
-        static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < 0.0); });
-        double rho = f[0];
+        E f[kQ];
+        static_for<0, kQ>([&](auto I) { f[I] = pull<I, Wide, E>(fin, Np, i, nb); });
+        if constexpr (kShift)  // n < 0  <=>  g < -a0 rho0
+            static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < -((float)a0(I) * a.rho0)); });
+        else
+            static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < 0.0); });
+        E rho = f[0];  // FP32-shift: drho = rho - rho0
         static_for<1, kQ>([&](auto I) { rho = rho + f[I]; });
-        double jx = a.fmx, jy = a.fmy, jz = a.fmz;
+        E jx = a.fmx, jy = a.fmy, jz = a.fmz;
         static_for<0, kQ>([&](auto I) {
             constexpr int L = I;
             if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(f[L]);
@@ -200,18 +234,28 @@ __global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs a) {
             if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(f[L]);
         });
         if (a.write_l2) {
-            const double dx = fabs(jx - jo[0]), dy = fabs(jy - jo[1]), dz = fabs(jz - jo[2]);
-            l2 = fmax(fmax(dx, dy), dz);
+            const E dx = fabs(jx - jo[0]), dy = fabs(jy - jo[1]), dz = fabs(jz - jo[2]);
+            l2 = (double)fmax(fmax(dx, dy), dz);
         }
         jv.st(i, jx);
         jv.st(Np + i, jy);
         jv.st(2 * Np + i, jz);
-        const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
-        static_for<0, kQ>([&](auto I) {
-            constexpr int L = I;
-            fout.st((uint32_t)L * Np + i,
-                    collide_pop<L>(f[L], rho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy, a.fcz, a.A, a.B, a.C));
-        });
+        if constexpr (kShift) {
+            const float drho = rho, full = a.rho0 + drho;
+            const float ux = jx / full, uy = jy / full, uz = jz / full;
+            static_for<0, kQ>([&](auto I) {
+                constexpr int L = I;
+                fout.st((uint32_t)L * Np + i, collide_shift<L>(f[L], drho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy,
+                                                               a.fcz, a.A, a.B, a.C));
+            });
+        } else {
+            const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
+            static_for<0, kQ>([&](auto I) {
+                constexpr int L = I;
+                fout.st((uint32_t)L * Np + i,
+                        collide_pop<L>(f[L], rho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy, a.fcz, a.A, a.B, a.C));
+            });
+        }
     }
     if (a.write_l2) {
         __shared__ double lds[kBlock / 64];
@@ -267,6 +311,37 @@ __global__ __launch_bounds__(kBlock) void k_extract(const double* __restrict__ f
     rho_out[i] = rho;
 }
 
+// The same for the FP32-shift form: the post-streaming deviations g into
+// gout, and the FP64 density rho0 + sum g and flux F/2 + sum g c, summed in
+// FP64 from the FP32 deviations (the flow that moment propagation reads).
+__global__ __launch_bounds__(kBlock) void k_extract_shift(const float* __restrict__ gin, float* __restrict__ gout,
+                                                          double* __restrict__ rho_out, double* __restrict__ j_out,
+                                                          const uint32_t* __restrict__ nodes,
+                                                          const uint32_t* __restrict__ map, FastDiv dlx, FastDiv dly,
+                                                          uint32_t lz, uint32_t Nf, uint64_t Np, double rho0,
+                                                          double fmx, double fmy, double fmz) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= Nf) return;
+    Neighbours nb;
+    nb.load(map, nodes[i] & kIndexMask, dlx, dly, lz);
+    const View<true, float> gv(gin, kQ * Np);
+    double drho = 0.0, jx = fmx, jy = fmy, jz = fmz;
+    static_for<0, kQ>([&](auto I) {
+        constexpr int L = I;
+        const float v = pull<L, true>(gv, (uint32_t)Np, i, nb);
+        gout[(uint64_t)L * Np + i] = v;
+        const double d = v;
+        drho = (L == 0) ? d : drho + d;
+        if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(d);
+        if constexpr (kCY[L] != 0) jy = jy + sgn<kCY[L]>(d);
+        if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(d);
+    });
+    rho_out[i] = rho0 + drho;
+    j_out[i] = jx;
+    j_out[Np + i] = jy;
+    j_out[2 * Np + i] = jz;
+}
+
 // The collision on packed fields with a uniform force (the opening collision
 // of a run), in place: module_collision.f90:87-108.
 __global__ __launch_bounds__(kBlock) void k_collide_packed(double* f, const double* __restrict__ rho,
@@ -281,6 +356,29 @@ __global__ __launch_bounds__(kBlock) void k_collide_packed(double* f, const doub
         constexpr int L = I;
         const double v = f[(uint64_t)L * Np + i];
         f[(uint64_t)L * Np + i] = collide_pop<L>(v, d, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
+    });
+}
+
+// The same for the FP32-shift form: the FP64 density and flux of the state
+// become the FP32 drho = rho - rho0 and j of the collision (and j_old of the
+// first step).
+__global__ __launch_bounds__(kBlock) void k_collide_packed_shift(float* g, const double* __restrict__ rho,
+                                                                 const double* __restrict__ j,
+                                                                 float* __restrict__ j32, float fx, float fy,
+                                                                 float fz, uint32_t Nf, uint64_t Np, float A,
+                                                                 float B, float C, double rho0) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= Nf) return;
+    const float drho = (float)(rho[i] - rho0), full = (float)rho0 + drho;
+    const float jx = (float)j[i], jy = (float)j[Np + i], jz = (float)j[2 * Np + i];
+    j32[i] = jx;  // j_old of the first step
+    j32[Np + i] = jy;
+    j32[2 * Np + i] = jz;
+    const float ux = jx / full, uy = jy / full, uz = jz / full;
+    static_for<0, kQ>([&](auto I) {
+        constexpr int L = I;
+        const float v = g[(uint64_t)L * Np + i];
+        g[(uint64_t)L * Np + i] = collide_shift<L>(v, drho, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
     });
 }
 
@@ -304,112 +402,177 @@ __global__ __launch_bounds__(kBlock) void k_collide_box(double* f, const double*
     });
 }
 
-// Rest state, init_simu.f90:31-39: n = rho0*a0 at fluid nodes, j = 0.
-__global__ __launch_bounds__(kBlock) void k_init_rest(double* __restrict__ f, double* __restrict__ rho,
+// Rest state, init_simu.f90:31-39: n = rho0*a0 at fluid nodes, j = 0; in the
+// FP32-shift form (g != NULL) the deviations g = 0 and the FP32 flux j32 = 0.
+__global__ __launch_bounds__(kBlock) void k_init_rest(double* __restrict__ f, float* __restrict__ g,
+                                                      float* __restrict__ j32, double* __restrict__ rho,
                                                       double* __restrict__ j, uint32_t Nf, uint64_t Np, double rho0) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= Nf) return;
     static_for<0, kQ>([&](auto I) {
         constexpr int L = I;
-        f[(uint64_t)L * Np + i] = rho0 * a0(L);
+        if (f) f[(uint64_t)L * Np + i] = rho0 * a0(L);
+        if (g) g[(uint64_t)L * Np + i] = 0.0f;
     });
     rho[i] = rho0;
-    j[i] = 0.0;
-    j[Np + i] = 0.0;
-    j[2 * Np + i] = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        j[c * Np + i] = 0.0;
+        if (j32) j32[c * Np + i] = 0.0f;
+    }
 }
 
 // ---------------------------------------------------------------- MP kernels
+//
+// Several tracers that share the flow and Db (so the scattering
+// probabilities win, rem and u*) but differ in ka, kd can be propagated in one
+// pass: the shared coefficients are read once per node for all of them.  The
+// arithmetic of each tracer is exactly that of a single-tracer run.
 
-// Per-block triple sums in a fixed order (the block partial of a step kernel).
-__device__ __forceinline__ void block_partial3(double v[3], double* part, double* lds) {
-    block_sum3<kBlock>(v, lds);
-    if (threadIdx.x == 0) {
-        part[3 * blockIdx.x + 0] = v[0];
-        part[3 * blockIdx.x + 1] = v[1];
-        part[3 * blockIdx.x + 2] = v[2];
+constexpr int kMaxTracers = 4;
+
+// Per-block sums of NV values in a fixed order, written by thread 0 to
+// part[NV*blockIdx.x ...].
+template <int NV>
+__device__ __forceinline__ void block_partial(const double (&v)[NV], double* part, double* lds /* [4*NV] */) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+        const double s = wave_sum(v[a]);
+        if (lane == 0) lds[w * NV + a] = s;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < NV; ++a) {
+            double s = lds[a];
+            for (int k = 1; k < kBlock / 64; ++k) s = s + lds[k * NV + a];
+            part[NV * blockIdx.x + a] = s;
+        }
+    }
+    __syncthreads();
 }
 
-// VACF of one moment-propagation step from the per-block partial sums, in a
-// fixed order that does not depend on which kernel runs it (so every run gives
-// the same bits), then the running D_eff sum and, optionally, the convergence
-// test of module_moment_propagation.f90:284-288.  Called by all kBlock threads
-// of one block.  mode 0: VACF(0) of mp init; mode 1: VACF(it) of a step.
+// VACF of one moment-propagation step (NV = 3 per tracer) from the per-block
+// partial sums, in a fixed order that does not depend on which kernel runs it
+// (so every run gives the same bits), then the running D_eff sums and,
+// optionally, the convergence test of module_moment_propagation.f90:284-288
+// (all NV components below both limits).  Called by all kBlock threads of one
+// block.  mode 0: VACF(0) of mp init; mode 1: VACF(it) of a step.
+template <int NV>
 __device__ __forceinline__ void mp_reduce_block(const double* part, uint32_t nparts, double* vtrace, Ctrl* ctrl,
                                                 int32_t mode, int64_t it, int32_t check_conv, double lim1,
-                                                double lim2, double* lds) {
-    double s[3] = {0.0, 0.0, 0.0};
+                                                double lim2, double* lds /* [4*NV] */) {
+    double s[NV];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) s[a] = 0.0;
     for (uint32_t k = threadIdx.x; k < nparts; k += kBlock)
-        for (int c = 0; c < 3; ++c) {
-            const double x = *(const volatile double*)&part[3 * k + c];
-            s[c] = (k == threadIdx.x) ? x : s[c] + x;
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+            const double x = *(const volatile double*)&part[(uint64_t)NV * k + a];
+            s[a] = (k == threadIdx.x) ? x : s[a] + x;
         }
-    block_sum3<kBlock>(s, lds);
-    if (threadIdx.x != 0) return;
-    if (mode == 0) {
-        for (int c = 0; c < 3; ++c) {
-            ctrl->vacf0[c] = s[c];
-            ctrl->deff_s[c] = 0.5 * s[c];
-            ctrl->deff_c[c] = 0.0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+        const double t = wave_sum(s[a]);
+        if (lane == 0) lds[w * NV + a] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+            double t = lds[a];
+            for (int k = 1; k < kBlock / 64; ++k) t = t + lds[k * NV + a];
+            s[a] = t;
         }
-        return;
-    }
-    for (int c = 0; c < 3; ++c) {
-        vtrace[c] = s[c];
-        // Neumaier update of D = VACF(0)/2 + sum VACF(t)
-        const double d = ctrl->deff_s[c], x = s[c], u = d + x;
-        ctrl->deff_c[c] += (fabs(d) >= fabs(x)) ? (d - u) + x : (x - u) + d;
-        ctrl->deff_s[c] = u;
-    }
-    if (check_conv && it > 2) {
-        bool conv = true;
-        for (int c = 0; c < 3; ++c) conv = conv && fabs(s[c]) < lim1 && fabs(s[c]) < lim2;
-        if (conv) {
-            ctrl->mp_conv_step = it;
-            ctrl->mp_stop = 1;
+        if (mode == 0) {
+#pragma unroll
+            for (int a = 0; a < NV; ++a) {
+                ctrl->vacf0[a] = s[a];
+                ctrl->deff_s[a] = 0.5 * s[a];
+                ctrl->deff_c[a] = 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < NV; ++a) {
+                vtrace[a] = s[a];
+                // Neumaier update of D = VACF(0)/2 + sum VACF(t)
+                const double d = ctrl->deff_s[a], x = s[a], u = d + x;
+                ctrl->deff_c[a] += (fabs(d) >= fabs(x)) ? (d - u) + x : (x - u) + d;
+                ctrl->deff_s[a] = u;
+            }
+            if (check_conv && it > 2) {
+                bool conv = true;
+#pragma unroll
+                for (int a = 0; a < NV; ++a) conv = conv && fabs(s[a]) < lim1 && fabs(s[a]) < lim2;
+                if (conv) {
+                    ctrl->mp_conv_step = it;
+                    ctrl->mp_stop = 1;
+                }
+            }
         }
     }
+    __syncthreads();
 }
 
+template <int NV>
 __global__ __launch_bounds__(kBlock) void k_mp_reduce(const double* __restrict__ part, uint32_t nparts,
                                                       double* vtrace, Ctrl* ctrl, int32_t mode, int64_t it,
                                                       int32_t check_conv, double lim1, double lim2) {
-    __shared__ double lds[kBlock / 64 * 3];
+    __shared__ double lds[kBlock / 64 * NV];
     if (mode == 1 && vload(&ctrl->mp_stop)) return;
-    mp_reduce_block(part, nparts, vtrace, ctrl, mode, it, check_conv, lim1, lim2, lds);
+    mp_reduce_block<NV>(part, nparts, vtrace, ctrl, mode, it, check_conv, lim1, lim2, lds);
 }
 
+struct TracerParams {
+    double bw[kMaxTracers];    // 1/Pstat (module_moment_propagation.f90:97,108)
+    double ka[kMaxTracers], kd[kMaxTracers], one_minus_kd[kMaxTracers];
+    int32_t ads[kMaxTracers];  // adsorption on (K = ka/kd > eps, :46-56)
+};
+
+// W is the type the scattering probabilities win and u* are stored in:
+// double, or float in the FP32-shift form.  There each probability is rounded
+// to FP32 once, and every quantity built from it (the rest probability
+// rem = 1 - sum s, which stays FP64, u*, P(r,1), VACF(0)) uses the rounded
+// value, so the propagator still conserves sum_r P exactly up to FP64 rounding.
+template <typename W>
+__device__ __forceinline__ double as_coef(double v) {
+    return (double)(W)v;
+}
+
+template <typename W>
 struct MpPrepArgs {
     const double* __restrict__ rho;
     const double* __restrict__ j;
     const uint32_t* __restrict__ nodes;
     const uint32_t* __restrict__ map;
-    double* __restrict__ win;
+    W* __restrict__ win;
     double* __restrict__ rem;
-    double* __restrict__ ustar;
-    double* __restrict__ P0;
+    W* __restrict__ ustar;
+    double* __restrict__ P0;   // [T][3][Np]
     double* partials;
     Ctrl* ctrl;
     FastDiv dlx, dly;
     uint32_t lz, Nf;
     uint64_t Np;
     double fx, fy, fz;   // lb.in f_ext (drop_tracers.f90:92)
-    double lambda, bw, ka, eps;
-    int32_t ads;
+    double lambda, eps;
+    TracerParams tp;
 };
 
 // update_tracer_population (drop_tracers.f90:97-100) fused with
 // moment_propagation::init (module_moment_propagation.f90:105-137) and the
 // static part of propagate (:218-249): per fluid node, the scattering
 // probabilities s(r,l) = n_l/rho - w_l + lambda w_l / 2 towards fluid
-// neighbours, the rest probability rem (minus ka at adsorbing interfacial
-// nodes), u* = sum s c, the incoming probabilities win(r,l) = s(r+c_l, inv l),
-// P(r,1) and the VACF(0) contribution.
-__global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs a) {
-    __shared__ double lds[kBlock / 64 * 3];
+// neighbours, the rest probability rem, u* = sum s c, the incoming
+// probabilities win(r,l) = s(r+c_l, inv l), and per tracer P(r,1) and the
+// VACF(0) contribution.
+template <int T, typename W>
+__global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs<W> a) {
+    __shared__ double lds[kBlock / 64 * 3 * T];
     const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
-    double vp[3] = {0.0, 0.0, 0.0};
+    double vp[3 * T];
+#pragma unroll
+    for (int k = 0; k < 3 * T; ++k) vp[k] = 0.0;
     if (i < a.Nf) {
         const uint32_t e = a.nodes[i];
         Neighbours nb;
@@ -417,7 +580,9 @@ __global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs a) {
         const double rr = a.rho[i];
         const double vx = a.j[i] + a.fx, vy = a.j[a.Np + i] + a.fy, vz = a.j[2 * a.Np + i] + a.fz;
         double us[3] = {0.0, 0.0, 0.0};
-        double p1[3] = {0.0, 0.0, 0.0};
+        double p1[3 * T];
+#pragma unroll
+        for (int k = 0; k < 3 * T; ++k) p1[k] = 0.0;
         double remv = 1.0;
         static_for<1, kQ>([&](auto I) {
             constexpr int L = I;
@@ -426,108 +591,129 @@ __global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs a) {
             const uint32_t q = nb.q[L];
             if (q != kSolidSlot) {
                 const double nt = a0(L) * rr + a1(L) * dot3<cx, cy, cz>(vx, vy, vz);
-                const double s = nt / rr - a0(L) + a.lambda * a0(L) * 0.5;
+                const double s = as_coef<W>(nt / rr - a0(L) + a.lambda * a0(L) * 0.5);
                 remv = remv - s;
                 if constexpr (cx != 0) us[0] = us[0] + sgn<cx>(s);
                 if constexpr (cy != 0) us[1] = us[1] + sgn<cy>(s);
                 if constexpr (cz != 0) us[2] = us[2] + sgn<cz>(s);
-                if constexpr (cx != 0) vp[0] = vp[0] + a.bw * s;
-                if constexpr (cy != 0) vp[1] = vp[1] + a.bw * s;
-                if constexpr (cz != 0) vp[2] = vp[2] + a.bw * s;
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    if constexpr (cx != 0) vp[3 * t + 0] = vp[3 * t + 0] + a.tp.bw[t] * s;
+                    if constexpr (cy != 0) vp[3 * t + 1] = vp[3 * t + 1] + a.tp.bw[t] * s;
+                    if constexpr (cz != 0) vp[3 * t + 2] = vp[3 * t + 2] + a.tp.bw[t] * s;
+                }
                 const double rn = a.rho[q];
                 const double wx = a.j[q] + a.fx, wy = a.j[a.Np + q] + a.fy, wz = a.j[2 * a.Np + q] + a.fz;
                 const double ntn = a0(Li) * rn + a1(Li) * dot3<-cx, -cy, -cz>(wx, wy, wz);
-                const double sp = ntn / rn - a0(Li) + a.lambda * a0(Li) * 0.5;
-                a.win[(uint64_t)(L - 1) * a.Np + i] = sp;
-                if constexpr (cx != 0) p1[0] = p1[0] + sgn<-cx>(sp) * a.bw;
-                if constexpr (cy != 0) p1[1] = p1[1] + sgn<-cy>(sp) * a.bw;
-                if constexpr (cz != 0) p1[2] = p1[2] + sgn<-cz>(sp) * a.bw;
+                const double sp = as_coef<W>(ntn / rn - a0(Li) + a.lambda * a0(Li) * 0.5);
+                a.win[(uint64_t)(L - 1) * a.Np + i] = (W)sp;
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    if constexpr (cx != 0) p1[3 * t + 0] = p1[3 * t + 0] + sgn<-cx>(sp) * a.tp.bw[t];
+                    if constexpr (cy != 0) p1[3 * t + 1] = p1[3 * t + 1] + sgn<-cy>(sp) * a.tp.bw[t];
+                    if constexpr (cz != 0) p1[3 * t + 2] = p1[3 * t + 2] + sgn<-cz>(sp) * a.tp.bw[t];
+                }
             } else {
-                a.win[(uint64_t)(L - 1) * a.Np + i] = 0.0;
+                a.win[(uint64_t)(L - 1) * a.Np + i] = W(0);
             }
         });
-        if (a.ads && (e & kNodeInterfacial)) remv = remv - a.ka;
-        if (remv < a.eps) a.ctrl->restpart = 1;
         a.rem[i] = remv;
-        for (int c = 0; c < 3; ++c) {
-            a.ustar[c * a.Np + i] = us[c];
-            a.P0[c * a.Np + i] = p1[c];
+        bool low = false;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const double rt = (a.tp.ads[t] && (e & kNodeInterfacial)) ? remv - a.tp.ka[t] : remv;
+            low = low || rt < a.eps;
         }
+        if (low) a.ctrl->restpart = 1;
+        for (int c = 0; c < 3; ++c) a.ustar[c * a.Np + i] = (W)us[c];
+#pragma unroll
+        for (int k = 0; k < 3 * T; ++k) a.P0[(uint64_t)k * a.Np + i] = p1[k];
     }
-    block_partial3(vp, a.partials, lds);
+    block_partial<3 * T>(vp, a.partials, lds);
 }
 
+template <typename W>
 struct MpStepArgs {
-    const double* __restrict__ win;
+    const W* __restrict__ win;
     const double* __restrict__ rem;
-    const double* __restrict__ ustar;
+    const W* __restrict__ ustar;
     const uint32_t* __restrict__ nodes;
     const uint32_t* __restrict__ map;
-    const double* __restrict__ Pn;
+    const double* __restrict__ Pn;   // [T][3][Np]
     double* __restrict__ Px;
     const double* __restrict__ Pan;
     double* __restrict__ Pax;
-    double* partials;           // this step's per-block VACF partials
-    const double* prev_partials; // the previous step's, reduced by block 0 (NULL: none)
-    double* prev_vtrace;        // where VACF(it-1) goes
+    double* partials;                // this step's per-block VACF partials
+    const double* prev_partials;     // the previous step's, reduced by block 0 (NULL: none)
+    double* prev_vtrace;             // where VACF(it-1) goes
     int64_t it;
     Ctrl* ctrl;
     FastDiv dlx, dly;
     uint32_t lz, Nf;
     uint64_t Np;
-    double ka, kd, one_minus_kd;
-    int32_t ads;
+    TracerParams tp;
 };
 
-// One moment-propagation step, module_moment_propagation.f90:207-253:
+// One moment-propagation step of T tracers, module_moment_propagation.f90:207-253:
 // P(r,t+1) = sum_l P(r+c_l,t) s(r+c_l, inv l) + rem(r) P(r,t) [+ kd Pa],
 // Pa(r,t+1) = (1-kd) Pa(r,t) + ka P(r,t) at adsorbing interfacial nodes,
 // VACF(t) = sum_r P(r,t) . u*(r).
-template <bool Wide>
-__global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs a) {
-    __shared__ double lds[kBlock / 64 * 3];
+template <bool Wide, int T, typename W>
+__global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs<W> a) {
+    __shared__ double lds[kBlock / 64 * 3 * T];
     if (vload(&a.ctrl->mp_stop)) return;
     if (a.prev_partials && blockIdx.x == 0)
-        mp_reduce_block(a.prev_partials, gridDim.x, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
+        mp_reduce_block<3 * T>(a.prev_partials, gridDim.x, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
     const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
-    double vp[3] = {0.0, 0.0, 0.0};
+    double vp[3 * T];
+#pragma unroll
+    for (int k = 0; k < 3 * T; ++k) vp[k] = 0.0;
     if (i < a.Nf) {
         const uint32_t Np = (uint32_t)a.Np;
-        const View<Wide> win(a.win, 18 * a.Np), Pn(a.Pn, 3 * a.Np), Px(a.Px, 3 * a.Np), us(a.ustar, 3 * a.Np),
-            rem(a.rem, a.Np);
+        const View<Wide, W> win(a.win, 18 * a.Np), us(a.ustar, 3 * a.Np);
+        const View<Wide> Pn(a.Pn, 3 * T * a.Np), Px(a.Px, 3 * T * a.Np), rem(a.rem, a.Np);
         const uint32_t e = ld(a.nodes, i);
         Neighbours nb;
         nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
-        double p[3];
-        for (int c = 0; c < 3; ++c) {
-            p[c] = Pn.ld(c * Np + i);
-            vp[c] = us.ld(c * Np + i);
+        double p[3 * T], acc[3 * T];
+#pragma unroll
+        for (int k = 0; k < 3 * T; ++k) {
+            p[k] = Pn.ld((uint32_t)k * Np + i);
+            acc[k] = 0.0;
         }
+        double u[3];
+        for (int c = 0; c < 3; ++c) u[c] = (double)us.ld(c * Np + i);
         const double rm = rem.ld(i);
-        double acc[3] = {0.0, 0.0, 0.0};
         static_for<1, kQ>([&](auto I) {
             constexpr int L = I;
             const uint32_t q = nb.q[L];
             if (q != kSolidSlot) {
-                const double w = win.ld((uint32_t)(L - 1) * Np + i);
-                acc[0] = acc[0] + Pn.ld(q) * w;
-                acc[1] = acc[1] + Pn.ld(Np + q) * w;
-                acc[2] = acc[2] + Pn.ld(2 * Np + q) * w;
+                const double w = (double)win.ld((uint32_t)(L - 1) * Np + i);
+#pragma unroll
+                for (int k = 0; k < 3 * T; ++k) acc[k] = acc[k] + Pn.ld((uint32_t)k * Np + q) * w;
             }
         });
-        for (int c = 0; c < 3; ++c) vp[c] = p[c] * vp[c];
-        if (!a.ads || !(e & kNodeInterfacial)) {
-            for (int c = 0; c < 3; ++c) Px.st(c * Np + i, acc[c] + rm * p[c]);
-        } else {
-            const View<Wide> Pan(a.Pan, 3 * a.Np), Pax(a.Pax, 3 * a.Np);
-            for (int c = 0; c < 3; ++c) {
-                const double pa = Pan.ld(c * Np + i);
-                Px.st(c * Np + i, (acc[c] + rm * p[c]) + pa * a.kd);
-                Pax.st(c * Np + i, pa * a.one_minus_kd + p[c] * a.ka);
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+            for (int c = 0; c < 3; ++c) vp[3 * t + c] = p[3 * t + c] * u[c];
+        const bool itf = e & kNodeInterfacial;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            if (!a.tp.ads[t] || !itf) {
+                for (int c = 0; c < 3; ++c) Px.st((uint32_t)(3 * t + c) * Np + i, acc[3 * t + c] + rm * p[3 * t + c]);
+            } else {
+                const View<Wide> Pan(a.Pan, 3 * T * a.Np), Pax(a.Pax, 3 * T * a.Np);
+                const double rt = rm - a.tp.ka[t];
+                for (int c = 0; c < 3; ++c) {
+                    const uint32_t k = (uint32_t)(3 * t + c) * Np + i;
+                    const double pa = Pan.ld(k);
+                    Px.st(k, (acc[3 * t + c] + rt * p[3 * t + c]) + pa * a.tp.kd[t]);
+                    Pax.st(k, pa * a.tp.one_minus_kd[t] + p[3 * t + c] * a.tp.ka[t]);
+                }
             }
         }
     }
-    block_partial3(vp, a.partials, lds);
+    block_partial<3 * T>(vp, a.partials, lds);
 }
 
 // Plain streaming copy (16 B per lane per iteration, grid-stride): the
@@ -551,15 +737,19 @@ struct lbx_ctx {
     int32_t prec = LBX_PREC_FP64;
     double tau = 1.0, A = 0, B = 0, C = 0;
     int64_t nfi = 0;
+    int64_t floor_window = 0;    // FP32-shift l2err floor test (0: off)
     int device = 0;
     hipStream_t stream = nullptr;
-    double* f[2] = {nullptr, nullptr};
-    double* j = nullptr;
+    double* f[2] = {nullptr, nullptr};  // FP64 populations (ping-pong)
+    float* g[2] = {nullptr, nullptr};   // FP32-shift: deviations g = n - a0 rho0 (ping-pong)
+    float* j32 = nullptr;               // FP32-shift: the flux of the last step (j_old)
+    double rho0 = 1.0;                  // FP32-shift: the rest density of the shift
+    double* j = nullptr;                // FP64 flux (FP32-shift: of the final state only)
     double* rho = nullptr;
     uint32_t* nodes = nullptr;
     uint32_t* map = nullptr;
     uint32_t ntiles = 0;
-    double* part = nullptr;      // per-block partials, two buffers of 3 per tile
+    double* part = nullptr;      // per-block VACF partials, two buffers of 3*kMaxTracers per tile
     unsigned long long* slots = nullptr;  // [3][kSlots] l2err slots
     Ctrl* ctrl = nullptr;
     double* l2ring = nullptr;
@@ -567,16 +757,20 @@ struct lbx_ctx {
     std::vector<uint32_t> nodes_h;
     // moment propagation
     bool mp_ready = false;
-    double* win = nullptr;
-    double* rem = nullptr;
+    double* win = nullptr;       // FP64 form
     double* ustar = nullptr;
+    float* win32 = nullptr;      // FP32-shift form
+    float* ustar32 = nullptr;
+    double* rem = nullptr;
     double* P[2] = {nullptr, nullptr};
     double* Pa[2] = {nullptr, nullptr};
     double* vtrace = nullptr;
     int mp_now = 0;
     int64_t mp_it = 0;
-    double Db = 0, ka = 0, kd = 0, lambda = 0, bw = 0;
-    int32_t ads = 0;
+    int32_t ntr = 0;             // tracers propagated together (1..kMaxTracers)
+    double Db = 0, lambda = 0;
+    TracerParams tp{};
+    bool any_ads = false;
     bool mp_stopped = false;
     // timing
     bool timing = false;
@@ -603,6 +797,18 @@ int hip_status(int st) { return st; }  // an lbx status passes through
 
 uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
+// f(std::integral_constant<int, T>) for the run-time tracer count T
+template <typename F>
+void with_tracers(int ntr, F&& f) {
+    switch (ntr) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        default: f(std::integral_constant<int, 4>{}); break;
+    }
+}
+static_assert(kMaxTracers == 4, "with_tracers dispatches 1..4");
+
 template <typename T>
 int dalloc(T** p, uint64_t count) {
     *p = nullptr;
@@ -626,6 +832,8 @@ void free_mp(lbx_ctx* c) {
     dfree(c->win);
     dfree(c->rem);
     dfree(c->ustar);
+    dfree(c->win32);
+    dfree(c->ustar32);
     dfree(c->P[0]);
     dfree(c->P[1]);
     dfree(c->Pa[0]);
@@ -638,6 +846,9 @@ void free_all(lbx_ctx* c) {
     free_mp(c);
     dfree(c->f[0]);
     dfree(c->f[1]);
+    dfree(c->g[0]);
+    dfree(c->g[1]);
+    dfree(c->j32);
     dfree(c->j);
     dfree(c->rho);
     dfree(c->nodes);
@@ -658,9 +869,15 @@ void relaxation_coeffs(double tau, double* A, double* B, double* C) {
     *C = 1.0 - 1.0 / (2.0 * tau);
 }
 
-int reset_ctrl(lbx_ctx* c) {
+// check_after: the step after which an LB stage's stop tests apply; the
+// floor-test record starts there (no minimum yet)
+int reset_ctrl(lbx_ctx* c, int64_t check_after = 2) {
     Ctrl h{};
     h.negpop = -1;
+    for (int k = 0; k < 2; ++k) {
+        h.l2min[k] = HUGE_VAL;
+        h.l2min_step[k] = check_after;
+    }
     return hip_status(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
 }
 
@@ -688,12 +905,18 @@ void timer_end(lbx_ctx* c, int which, int64_t launches) {
     }
 }
 
-StepArgs step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3], int64_t check_after,
-                   double target, int write_l2) {
-    StepArgs a;
-    a.fin = c->f[(t - 1) & 1];
-    a.fout = c->f[t & 1];
-    a.j = c->j;
+template <typename E>
+E* pop_buf(lbx_ctx* c, int b) {
+    if constexpr (std::is_same<E, float>::value) return c->g[b]; else return c->f[b];
+}
+
+template <typename E>
+StepArgs<E> step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3], int64_t check_after,
+                      double target, int write_l2) {
+    StepArgs<E> a;
+    a.fin = pop_buf<E>(c, (t - 1) & 1);
+    a.fout = pop_buf<E>(c, t & 1);
+    if constexpr (std::is_same<E, float>::value) a.j = c->j32; else a.j = c->j;
     a.nodes = c->nodes;
     a.map = c->map;
     a.slots = c->slots;
@@ -704,21 +927,40 @@ StepArgs step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3]
     a.lz = c->lz;
     a.Nf = c->Nf;
     a.Np = c->Np;
-    a.A = c->A;
-    a.B = c->B;
-    a.C = c->C;
-    a.fmx = fm[0] / 2.0;
-    a.fmy = fm[1] / 2.0;
-    a.fmz = fm[2] / 2.0;
-    a.fcx = fc[0];
-    a.fcy = fc[1];
-    a.fcz = fc[2];
+    a.A = (E)c->A;
+    a.B = (E)c->B;
+    a.C = (E)c->C;
+    a.fmx = (E)(fm[0] / 2.0);
+    a.fmy = (E)(fm[1] / 2.0);
+    a.fmz = (E)(fm[2] / 2.0);
+    a.fcx = (E)fc[0];
+    a.fcy = (E)fc[1];
+    a.fcz = (E)fc[2];
+    a.rho0 = (E)c->rho0;
     a.t = t;
     a.check_after = check_after;
+    a.floor_window = c->floor_window;
     a.target = target;
     a.write_l2 = write_l2;
     a.have_prev = 0;
     return a;
+}
+
+// one LB step kernel of the context's precision and addressing
+void launch_lb_step(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3], int64_t check_after,
+                    double target, int write_l2, int have_prev) {
+    const uint32_t gt = c->ntiles;
+    if (c->prec == LBX_PREC_FP32_SHIFT) {
+        StepArgs<float> a = step_args<float>(c, t, fm, fc, check_after, target, write_l2);
+        a.have_prev = have_prev;
+        auto kern = c->wide ? k_lb_step<true, float> : k_lb_step<false, float>;
+        hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
+    } else {
+        StepArgs<double> a = step_args<double>(c, t, fm, fc, check_after, target, write_l2);
+        a.have_prev = have_prev;
+        auto kern = c->wide ? k_lb_step<true, double> : k_lb_step<false, double>;
+        hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
+    }
 }
 
 // packed <-> full-box copies on the host (not on the hot path)
@@ -739,6 +981,113 @@ int upload_full(lbx_ctx* c, const double* const* host_planes, int planes, double
         for (uint32_t i = 0; i < c->Nf; ++i) tmp[(size_t)p * c->Np + i] = host_planes[p][c->nodes_h[i] & kIndexMask];
     LBX_TRY(hipMemcpyAsync(ddst, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     return hip_status(hipStreamSynchronize(c->stream));
+}
+
+// FP32-shift populations <-> full-box FP64 n = a0 rho0 + g (0 at solid nodes)
+int download_shift(lbx_ctx* c, const float* gsrc, double* n_full) {
+    std::vector<float> tmp((size_t)kQ * c->Np);
+    LBX_TRY(hipMemcpyAsync(tmp.data(), gsrc, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    LBX_TRY(hipStreamSynchronize(c->stream));
+    std::memset(n_full, 0, (size_t)kQ * c->N * sizeof(double));
+    for (int l = 0; l < kQ; ++l) {
+        const double rest = a0(l) * c->rho0;
+        for (uint32_t i = 0; i < c->Nf; ++i)
+            n_full[(size_t)l * c->N + (c->nodes_h[i] & kIndexMask)] = rest + (double)tmp[(size_t)l * c->Np + i];
+    }
+    return LBX_OK;
+}
+
+int upload_shift(lbx_ctx* c, const double* n_full, float* gdst) {
+    std::vector<float> tmp((size_t)kQ * c->Np, 0.0f);
+    for (int l = 0; l < kQ; ++l) {
+        const double rest = a0(l) * c->rho0;
+        for (uint32_t i = 0; i < c->Nf; ++i)
+            tmp[(size_t)l * c->Np + i] = (float)(n_full[(size_t)l * c->N + (c->nodes_h[i] & kIndexMask)] - rest);
+    }
+    LBX_TRY(hipMemcpyAsync(gdst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    return hip_status(hipStreamSynchronize(c->stream));
+}
+
+template <typename W>
+W* win_buf(lbx_ctx* c) {
+    if constexpr (std::is_same<W, float>::value) return c->win32; else return c->win;
+}
+template <typename W>
+W* ustar_buf(lbx_ctx* c) {
+    if constexpr (std::is_same<W, float>::value) return c->ustar32; else return c->ustar;
+}
+
+// k_mp_prep + the VACF(0) reduction for the context's tracers
+template <typename W>
+void launch_mp_prep(lbx_ctx* c, const double fext[3], double eps) {
+    MpPrepArgs<W> a;
+    a.rho = c->rho;
+    a.j = c->j;
+    a.nodes = c->nodes;
+    a.map = c->map;
+    a.win = win_buf<W>(c);
+    a.rem = c->rem;
+    a.ustar = ustar_buf<W>(c);
+    a.P0 = c->P[0];
+    a.partials = c->part;
+    a.ctrl = c->ctrl;
+    a.dlx = c->dlx;
+    a.dly = c->dly;
+    a.lz = c->lz;
+    a.Nf = c->Nf;
+    a.Np = c->Np;
+    a.fx = fext[0];
+    a.fy = fext[1];
+    a.fz = fext[2];
+    a.lambda = c->lambda;
+    a.eps = eps;
+    a.tp = c->tp;
+    const uint32_t gt = c->ntiles;
+    with_tracers(c->ntr, [&](auto TT) {
+        constexpr int T = TT;
+        hipLaunchKernelGGL((k_mp_prep<T, W>), dim3(gt), dim3(kBlock), 0, c->stream, a);
+        hipLaunchKernelGGL(k_mp_reduce<3 * T>, dim3(1), dim3(kBlock), 0, c->stream, c->part, gt, nullptr, c->ctrl, 0,
+                           (int64_t)0, 0, 0.0, 0.0);
+    });
+}
+
+// one k_mp_step launch (and, with the convergence test, its VACF reduction)
+template <typename W>
+void launch_mp_step(lbx_ctx* c, int now, int64_t it, int64_t k, int32_t stop_on_convergence, double lim1,
+                           double lim2) {
+    const uint32_t gt = c->ntiles;
+    const int nv = 3 * c->ntr;
+    MpStepArgs<W> a;
+    a.win = win_buf<W>(c);
+    a.rem = c->rem;
+    a.ustar = ustar_buf<W>(c);
+    a.nodes = c->nodes;
+    a.map = c->map;
+    a.Pn = c->P[now];
+    a.Px = c->P[now ^ 1];
+    a.Pan = c->any_ads ? c->Pa[now] : nullptr;
+    a.Pax = c->any_ads ? c->Pa[now ^ 1] : nullptr;
+    // partial buffers alternate by step; without a convergence test the
+    // previous step's VACF is reduced by block 0 of this step's kernel
+    a.partials = c->part + (uint64_t)nv * gt * (it & 1);
+    a.prev_partials = (!stop_on_convergence && k > 0) ? c->part + (uint64_t)nv * gt * ((it - 1) & 1) : nullptr;
+    a.prev_vtrace = c->vtrace + nv * (k - 1);
+    a.it = it;
+    a.ctrl = c->ctrl;
+    a.dlx = c->dlx;
+    a.dly = c->dly;
+    a.lz = c->lz;
+    a.Nf = c->Nf;
+    a.Np = c->Np;
+    a.tp = c->tp;
+    with_tracers(c->ntr, [&](auto TT) {
+        constexpr int T = TT;
+        auto kern = c->wide ? k_mp_step<true, T, W> : k_mp_step<false, T, W>;
+        hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
+        if (stop_on_convergence)
+            hipLaunchKernelGGL(k_mp_reduce<3 * T>, dim3(1), dim3(kBlock), 0, c->stream, a.partials, gt,
+                               c->vtrace + nv * k, c->ctrl, 1, it, 1, lim1, lim2);
+    });
 }
 
 }  // namespace
@@ -782,9 +1131,14 @@ int64_t lbx_footprint_bytes(int32_t lx, int32_t ly, int32_t lz, int32_t precisio
     if (lx <= 0 || ly <= 0 || lz <= 0) return -1;
     // upper bound: every node fluid
     const uint64_t N = (uint64_t)lx * ly * lz, Np = (N + 63) / 64 * 64;
-    const uint64_t pop = precision == LBX_PREC_FP32_SHIFT ? 4 : 8;
-    uint64_t b = 2 * kQ * Np * pop + 4 * Np * 8 + N * 8 + nblocks(N) * 3 * 8 + sizeof(Ctrl) + kRing * 8;
-    if (with_mp) b += (18 + 1 + 3 + 12) * Np * 8 + kMpChunk * 3 * 8;
+    const bool shift = precision == LBX_PREC_FP32_SHIFT;
+    const uint64_t pop = shift ? 4 : 8;  // populations; also win and u* of moment propagation
+    uint64_t b = 2 * kQ * Np * pop + (shift ? 3 * Np * 4 : 0) + 4 * Np * 8 + N * 8 +
+                 nblocks(N) * 6 * kMaxTracers * 8 + sizeof(Ctrl) + kRing * 8;
+    if (with_mp > 0) {  // with_mp tracers, each with adsorption (the upper bound)
+        const uint64_t T = (uint64_t)std::min(with_mp, kMaxTracers);
+        b += (18 + 3) * Np * pop + Np * 8 + 12 * T * Np * 8 + kMpChunk * 3 * T * 8;
+    }
     return (int64_t)b;
 }
 
@@ -795,7 +1149,7 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     if (lx <= 0 || ly <= 0 || lz <= 0 || !nature) return LBX_ERR_ARG;
     // packed indices times 8 bytes must fit in 32 bits (the 32-bit offset loads)
     if ((uint64_t)lx * ly * lz >= (1ull << 29)) return LBX_ERR_ARG;
-    if (precision != LBX_PREC_FP64) return LBX_ERR_ARG;
+    if (precision != LBX_PREC_FP64 && precision != LBX_PREC_FP32_SHIFT) return LBX_ERR_ARG;
     if (!(tau >= 0.5)) return LBX_ERR_TAU;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LBX_ERR_DEVICE;
@@ -808,6 +1162,7 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     c->dlx.set((uint32_t)lx);
     c->dly.set((uint32_t)ly);
     c->prec = precision;
+    c->floor_window = precision == LBX_PREC_FP32_SHIFT ? 1000 : 0;
     c->tau = tau;
     relaxation_coeffs(tau, &c->A, &c->B, &c->C);
     (void)hipGetDevice(&c->device);
@@ -830,13 +1185,17 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     c->ntiles = nblocks(c->Nf);
     int st = LBX_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = LBX_ERR_DEVICE;
-    if (!st) st = dalloc(&c->f[0], kQ * c->Np);
-    if (!st) st = dalloc(&c->f[1], kQ * c->Np);
+    const bool shift = precision == LBX_PREC_FP32_SHIFT;
+    if (!st && !shift) st = dalloc(&c->f[0], kQ * c->Np);
+    if (!st && !shift) st = dalloc(&c->f[1], kQ * c->Np);
+    if (!st && shift) st = dalloc(&c->g[0], kQ * c->Np);
+    if (!st && shift) st = dalloc(&c->g[1], kQ * c->Np);
+    if (!st && shift) st = dalloc(&c->j32, 3 * c->Np);
     if (!st) st = dalloc(&c->j, 3 * c->Np);
     if (!st) st = dalloc(&c->rho, c->Np);
     if (!st) st = dalloc(&c->nodes, (uint64_t)c->Nf);
     if (!st) st = dalloc(&c->map, (uint64_t)c->N);
-    if (!st) st = dalloc(&c->part, 6 * (uint64_t)c->ntiles);
+    if (!st) st = dalloc(&c->part, 6 * (uint64_t)kMaxTracers * c->ntiles);
     if (!st) st = dalloc(&c->slots, 3 * kSlots);
     if (!st) st = dalloc(&c->ctrl, 1);
     if (!st) st = dalloc(&c->l2ring, kRing);
@@ -846,8 +1205,11 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
                                             hipMemcpyHostToDevice, c->stream));
     if (!st) st = hip_status(hipMemcpyAsync(c->map, map.data(), c->N * sizeof(uint32_t), hipMemcpyHostToDevice,
                                             c->stream));
-    if (!st) st = hip_status(hipMemsetAsync(c->f[0], 0, kQ * c->Np * sizeof(double), c->stream));
-    if (!st) st = hip_status(hipMemsetAsync(c->f[1], 0, kQ * c->Np * sizeof(double), c->stream));
+    for (int b = 0; b < 2; ++b) {
+        if (!st && !shift) st = hip_status(hipMemsetAsync(c->f[b], 0, kQ * c->Np * sizeof(double), c->stream));
+        if (!st && shift) st = hip_status(hipMemsetAsync(c->g[b], 0, kQ * c->Np * sizeof(float), c->stream));
+    }
+    if (!st && shift) st = hip_status(hipMemsetAsync(c->j32, 0, 3 * c->Np * sizeof(float), c->stream));
     if (!st) st = hip_status(hipMemsetAsync(c->j, 0, 3 * c->Np * sizeof(double), c->stream));
     if (!st) st = hip_status(hipMemsetAsync(c->rho, 0, c->Np * sizeof(double), c->stream));
     if (!st) st = reset_ctrl(c);
@@ -872,8 +1234,9 @@ int lbx_destroy(lbx_ctx* c) {
 
 int lbx_init_rest(lbx_ctx* c, double rho0) {
     if (!c) return LBX_ERR_ARG;
-    hipLaunchKernelGGL(k_init_rest, dim3(c->ntiles), dim3(kBlock), 0, c->stream, c->f[0], c->rho, c->j, c->Nf,
-                       c->Np, rho0);
+    if (c->prec == LBX_PREC_FP32_SHIFT) c->rho0 = rho0;  // the shift follows the rest state
+    hipLaunchKernelGGL(k_init_rest, dim3(c->ntiles), dim3(kBlock), 0, c->stream, c->f[0], c->g[0], c->j32, c->rho,
+                       c->j, c->Nf, c->Np, rho0);
     c->pre = 0;
     LBX_TRY(hipGetLastError());
     return hip_status(hipStreamSynchronize(c->stream));
@@ -882,9 +1245,13 @@ int lbx_init_rest(lbx_ctx* c, double rho0) {
 int lbx_set_state(lbx_ctx* c, const double* n, const double* rho, const double* jx, const double* jy,
                   const double* jz) {
     if (!c || !n || !rho || !jx || !jy || !jz) return LBX_ERR_ARG;
-    const double* planes[kQ];
-    for (int l = 0; l < kQ; ++l) planes[l] = n + (size_t)l * c->N;
-    LBX_TRY(upload_full(c, planes, kQ, c->f[0]));
+    if (c->prec == LBX_PREC_FP32_SHIFT) {
+        LBX_TRY(upload_shift(c, n, c->g[0]));
+    } else {
+        const double* planes[kQ];
+        for (int l = 0; l < kQ; ++l) planes[l] = n + (size_t)l * c->N;
+        LBX_TRY(upload_full(c, planes, kQ, c->f[0]));
+    }
     LBX_TRY(upload_full(c, &rho, 1, c->rho));
     const double* jp[3] = {jx, jy, jz};
     LBX_TRY(upload_full(c, jp, 3, c->j));
@@ -906,7 +1273,8 @@ int lbx_l2err_trace(lbx_ctx* c, int64_t first_step, int64_t count, double* out, 
 
 int lbx_get_state(lbx_ctx* c, double* n, double* rho, double* jx, double* jy, double* jz) {
     if (!c) return LBX_ERR_ARG;
-    if (n) LBX_TRY(download_full(c, c->f[c->pre], kQ, n));
+    if (n && c->prec == LBX_PREC_FP32_SHIFT) LBX_TRY(download_shift(c, c->g[c->pre], n));
+    if (n && c->prec != LBX_PREC_FP32_SHIFT) LBX_TRY(download_full(c, c->f[c->pre], kQ, n));
     if (rho) LBX_TRY(download_full(c, c->rho, 1, rho));
     double* jp[3] = {jx, jy, jz};
     for (int a = 0; a < 3; ++a)
@@ -922,12 +1290,21 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
     const double zero[3] = {0.0, 0.0, 0.0};
     const double F[3] = {fext[0], fext[1], fext[2]};
     const uint32_t gt = c->ntiles;
-    // opening collision of step 1 (stage 1: no force), into f[0]
-    if (c->pre != 0)
+    const bool shift = c->prec == LBX_PREC_FP32_SHIFT;
+    // opening collision of step 1 (stage 1: no force), into buffer 0
+    if (c->pre != 0 && !shift)
         LBX_TRY(hipMemcpyAsync(c->f[0], c->f[c->pre], kQ * c->Np * sizeof(double), hipMemcpyDeviceToDevice,
                                c->stream));
-    hipLaunchKernelGGL(k_collide_packed, dim3(gt), dim3(kBlock), 0, c->stream, c->f[0], c->rho, c->j, 0.0, 0.0, 0.0,
-                       c->Nf, c->Np, c->A, c->B, c->C);
+    if (c->pre != 0 && shift)
+        LBX_TRY(hipMemcpyAsync(c->g[0], c->g[c->pre], kQ * c->Np * sizeof(float), hipMemcpyDeviceToDevice,
+                               c->stream));
+    if (shift) {
+        hipLaunchKernelGGL(k_collide_packed_shift, dim3(gt), dim3(kBlock), 0, c->stream, c->g[0], c->rho, c->j,
+                           c->j32, 0.0f, 0.0f, 0.0f, c->Nf, c->Np, (float)c->A, (float)c->B, (float)c->C, c->rho0);
+    } else {
+        hipLaunchKernelGGL(k_collide_packed, dim3(gt), dim3(kBlock), 0, c->stream, c->f[0], c->rho, c->j, 0.0, 0.0,
+                           0.0, c->Nf, c->Np, c->A, c->B, c->C);
+    }
     LBX_TRY(hipGetLastError());
     LBX_TRY(reset_ctrl(c));
     c->l2trace.clear();
@@ -935,6 +1312,7 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
     LBX_TRY(hipMemsetAsync(c->slots, 0, 3 * kSlots * sizeof(unsigned long long), c->stream));
 
     int stage = 1;
+    int64_t switch_step = INT64_MAX;  // the stage-1 convergence step: F acts from the step after it
     int64_t t = 0;            // last completed step
     int64_t check_after = 2;  // stop tests apply to steps > check_after (t > 2, equilibration.f90:346)
     int status = LBX_OK;
@@ -950,11 +1328,7 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
         }
         const double* Fs = stage == 1 ? zero : F;
         timer_begin(c);
-        for (int64_t k = 1; k <= S; ++k) {
-            StepArgs a = step_args(c, t + k, Fs, Fs, check_after, target_error, 1);
-            a.have_prev = t + k > 1;
-            hipLaunchKernelGGL(c->wide ? k_lb_step<true> : k_lb_step<false>, dim3(gt), dim3(kBlock), 0, c->stream, a);
-        }
+        for (int64_t k = 1; k <= S; ++k) launch_lb_step(c, t + k, Fs, Fs, check_after, target_error, 1, t + k > 1);
         timer_stop(c);
         hipLaunchKernelGGL(k_lb_finalize, dim3(1), dim3(kSlots), 0, c->stream, c->slots, c->l2ring, c->ctrl, t + S);
         LBX_TRY(hipGetLastError());
@@ -983,17 +1357,17 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
             if (stage == 1) {
                 // the first convergence switches the force on at the end of step e
                 // (equilibration.f90:377-386): redo the collision that opens e+1
-                StepArgs a = step_args(c, e, zero, F, INT64_MAX, target_error, 0);
-                LBX_TRY(reset_ctrl(c));
-                hipLaunchKernelGGL(c->wide ? k_lb_step<true> : k_lb_step<false>, dim3(gt), dim3(kBlock), 0, c->stream, a);
+                LBX_TRY(reset_ctrl(c, e));
+                launch_lb_step(c, e, zero, F, INT64_MAX, target_error, 0, 0);
                 LBX_TRY(hipGetLastError());
                 stage = 2;
+                switch_step = e;
                 t = e;
                 check_after = e;  // l2err(e) belongs to stage 1
                 continue;
             }
             t = e;
-            reason = LBX_EXIT_CONVERGED;
+            reason = h.floor_hit ? LBX_EXIT_FLOOR : LBX_EXIT_CONVERGED;
             break;
         }
         t += S;
@@ -1010,8 +1384,17 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
         // post-streaming state of step t into the other buffer (f[t&1] holds
         // the unused opening collision of step t+1), density alongside
         const int src = (t - 1) & 1, dst = t & 1;
-        hipLaunchKernelGGL(k_extract, dim3(gt), dim3(kBlock), 0, c->stream, c->f[src], c->f[dst], c->rho, c->nodes,
-                           c->map, c->dlx, c->dly, c->lz, c->Nf, c->Np);
+        if (shift) {
+            // FP64 density and flux of step t from its FP32 deviations; j(t)
+            // carries F(t)/2 as in the step kernel
+            const double* Ft = t > switch_step ? F : zero;
+            hipLaunchKernelGGL(k_extract_shift, dim3(gt), dim3(kBlock), 0, c->stream, c->g[src], c->g[dst], c->rho,
+                               c->j, c->nodes, c->map, c->dlx, c->dly, c->lz, c->Nf, c->Np, c->rho0, Ft[0] / 2.0,
+                               Ft[1] / 2.0, Ft[2] / 2.0);
+        } else {
+            hipLaunchKernelGGL(k_extract, dim3(gt), dim3(kBlock), 0, c->stream, c->f[src], c->f[dst], c->rho,
+                               c->nodes, c->map, c->dlx, c->dly, c->lz, c->Nf, c->Np);
+        }
         LBX_TRY(hipGetLastError());
         c->pre = dst;
     }
@@ -1019,70 +1402,63 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
     return status;
 }
 
-static int mp_setup(lbx_ctx* c, const double fext[3], double Db, double ka, double kd, double vacf0[3]) {
+static int mp_setup(lbx_ctx* c, const double fext[3], double Db, int32_t ntr, const double* ka, const double* kd,
+                    double* vacf0) {
     const double eps = DBL_EPSILON;
-    if (ka < -eps || kd < -eps) return LBX_ERR_ARG;  // module_moment_propagation.f90:43-44
-    if (Db <= eps) return LBX_ERR_ARG;                // drop_tracers.f90:89
+    if (ntr < 1 || ntr > kMaxTracers) return LBX_ERR_ARG;
+    for (int t = 0; t < ntr; ++t)
+        if (!(ka[t] >= -eps) || !(kd[t] >= -eps)) return LBX_ERR_ARG;  // module_moment_propagation.f90:43-44
+    if (!(Db > eps)) return LBX_ERR_ARG;                                  // drop_tracers.f90:89
     free_mp(c);
+    TracerParams tp{};
+    bool any_ads = false;
+    c->Db = Db;
+    c->lambda = 4.0 * Db / kKbt;  // calc_lambda, :331-337
+    for (int t = 0; t < ntr; ++t) {
+        const double K = (std::fabs(kd[t]) <= eps) ? 0.0 : ka[t] / kd[t];  // :46-50
+        tp.ads[t] = std::fabs(K) > eps;                                     // :52-56
+        const double Pstat = (double)c->Nf + K * (double)c->nfi;           // :97
+        tp.bw[t] = 1.0 / Pstat;
+        tp.ka[t] = ka[t];
+        tp.kd[t] = kd[t];
+        tp.one_minus_kd[t] = 1.0 - kd[t];
+        any_ads = any_ads || tp.ads[t];
+    }
+    const uint64_t np3 = 3 * (uint64_t)ntr * c->Np;
     int st = LBX_OK;
-    const uint32_t gt = c->ntiles;
-    if (!st) st = dalloc(&c->win, 18 * c->Np);
+    const bool shift = c->prec == LBX_PREC_FP32_SHIFT;
+    if (!st && !shift) st = dalloc(&c->win, 18 * c->Np);
+    if (!st && !shift) st = dalloc(&c->ustar, 3 * c->Np);
+    if (!st && shift) st = dalloc(&c->win32, 18 * c->Np);
+    if (!st && shift) st = dalloc(&c->ustar32, 3 * c->Np);
     if (!st) st = dalloc(&c->rem, c->Np);
-    if (!st) st = dalloc(&c->ustar, 3 * c->Np);
-    if (!st) st = dalloc(&c->P[0], 3 * c->Np);
-    if (!st) st = dalloc(&c->P[1], 3 * c->Np);
-    if (!st) st = dalloc(&c->vtrace, 3 * (uint64_t)kMpChunk);
-    const double K = (std::fabs(kd) <= eps) ? 0.0 : ka / kd;  // :46-50
-    c->ads = std::fabs(K) > eps;                                // :52-56
-    if (!st && c->ads) st = dalloc(&c->Pa[0], 3 * c->Np);
-    if (!st && c->ads) st = dalloc(&c->Pa[1], 3 * c->Np);
+    if (!st) st = dalloc(&c->P[0], np3);
+    if (!st) st = dalloc(&c->P[1], np3);
+    if (!st) st = dalloc(&c->vtrace, 3 * (uint64_t)ntr * kMpChunk);
+    // Pa of a tracer without adsorption is never read or written
+    if (!st && any_ads) st = dalloc(&c->Pa[0], np3);
+    if (!st && any_ads) st = dalloc(&c->Pa[1], np3);
     if (st) {
         free_mp(c);
         return st;
     }
     for (int b = 0; b < 2; ++b) {
-        LBX_TRY(hipMemsetAsync(c->P[b], 0, 3 * c->Np * sizeof(double), c->stream));
-        if (c->ads) LBX_TRY(hipMemsetAsync(c->Pa[b], 0, 3 * c->Np * sizeof(double), c->stream));
+        LBX_TRY(hipMemsetAsync(c->P[b], 0, np3 * sizeof(double), c->stream));
+        if (any_ads) LBX_TRY(hipMemsetAsync(c->Pa[b], 0, np3 * sizeof(double), c->stream));
     }
     LBX_TRY(reset_ctrl(c));
-    c->Db = Db;
-    c->ka = ka;
-    c->kd = kd;
-    c->lambda = 4.0 * Db / kKbt;                                // calc_lambda, :331-337
-    const double Pstat = (double)c->Nf + K * (double)c->nfi;    // :97
-    c->bw = 1.0 / Pstat;
-    MpPrepArgs a;
-    a.rho = c->rho;
-    a.j = c->j;
-    a.nodes = c->nodes;
-    a.map = c->map;
-    a.win = c->win;
-    a.rem = c->rem;
-    a.ustar = c->ustar;
-    a.P0 = c->P[0];
-    a.partials = c->part;
-    a.ctrl = c->ctrl;
-    a.dlx = c->dlx;
-    a.dly = c->dly;
-    a.lz = c->lz;
-    a.Nf = c->Nf;
-    a.Np = c->Np;
-    a.fx = fext[0];
-    a.fy = fext[1];
-    a.fz = fext[2];
-    a.lambda = c->lambda;
-    a.bw = c->bw;
-    a.ka = ka;
-    a.eps = eps;
-    a.ads = c->ads;
-    hipLaunchKernelGGL(k_mp_prep, dim3(gt), dim3(kBlock), 0, c->stream, a);
-    hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kBlock), 0, c->stream, c->part, gt, nullptr, c->ctrl, 0,
-                       (int64_t)0, 0, 0.0, 0.0);
+    c->ntr = ntr;
+    c->tp = tp;
+    c->any_ads = any_ads;
+    if (shift)
+        launch_mp_prep<float>(c, fext, eps);
+    else
+        launch_mp_prep<double>(c, fext, eps);
     LBX_TRY(hipGetLastError());
     Ctrl h;
     LBX_TRY(read_ctrl(c, &h));
     if (vacf0)
-        for (int k = 0; k < 3; ++k) vacf0[k] = h.vacf0[k];
+        for (int k = 0; k < 3 * ntr; ++k) vacf0[k] = h.vacf0[k];
     c->mp_ready = true;
     c->mp_now = 0;
     c->mp_it = 0;
@@ -1092,7 +1468,13 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, double ka, doub
 
 int lbx_mp_init(lbx_ctx* c, const double fext[3], double Db, double ka, double kd, double vacf0[3]) {
     if (!c || !fext) return LBX_ERR_ARG;
-    return mp_setup(c, fext, Db, ka, kd, vacf0);
+    return mp_setup(c, fext, Db, 1, &ka, &kd, vacf0);
+}
+
+int lbx_mp_init_batch(lbx_ctx* c, const double fext[3], double Db, int32_t ntracers, const double* ka,
+                      const double* kd, double* vacf0) {
+    if (!c || !fext || !ka || !kd) return LBX_ERR_ARG;
+    return mp_setup(c, fext, Db, ntracers, ka, kd, vacf0);
 }
 
 int lbx_mp_init_from_flow(lbx_ctx* c, const double* rho, const double* jx, const double* jy, const double* jz,
@@ -1101,7 +1483,7 @@ int lbx_mp_init_from_flow(lbx_ctx* c, const double* rho, const double* jx, const
     LBX_TRY(upload_full(c, &rho, 1, c->rho));
     const double* jp[3] = {jx, jy, jz};
     LBX_TRY(upload_full(c, jp, 3, c->j));
-    return mp_setup(c, fext, Db, ka, kd, vacf0);
+    return mp_setup(c, fext, Db, 1, &ka, &kd, vacf0);
 }
 
 int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* vacf_trace, int64_t* steps_done,
@@ -1110,66 +1492,45 @@ int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* 
     if (steps_done) *steps_done = 0;
     if (converged) *converged = 0;
     if (!c->mp_ready) return LBX_ERR_STATE;
+    if (stop_on_convergence && c->ntr != 1) return LBX_ERR_ARG;  // the test is per tracer
     Ctrl h;
     LBX_TRY(read_ctrl(c, &h));
     if (h.restpart) return LBX_ERR_RESTPART;  // the reference stops in the first propagate call
     if (c->mp_stopped) return LBX_OK;
     const uint32_t gt = c->ntiles;
+    const int nv = 3 * c->ntr;
     const double lim1 = 1.0 / (2.0 * (double)c->lx * (double)c->ly * (double)c->lz / c->Db);  // :284
     const double lim2 = (double)1.e-12f;  // a default-real literal in the reference
     int64_t done = 0;
-    std::vector<double> vt(3 * kMpChunk);
+    std::vector<double> vt((size_t)nv * kMpChunk);
     while (done < nsteps) {
         const int64_t S = std::min<int64_t>(kMpChunk, nsteps - done);
         timer_begin(c);
         for (int64_t k = 0; k < S; ++k) {
             const int now = c->mp_now ^ (int)(k & 1);
-            MpStepArgs a;
-            a.win = c->win;
-            a.rem = c->rem;
-            a.ustar = c->ustar;
-            a.nodes = c->nodes;
-            a.map = c->map;
-            a.Pn = c->P[now];
-            a.Px = c->P[now ^ 1];
-            a.Pan = c->ads ? c->Pa[now] : nullptr;
-            a.Pax = c->ads ? c->Pa[now ^ 1] : nullptr;
             const int64_t it = c->mp_it + k + 1;
-            // partial buffers alternate by step; without a convergence test the
-            // previous step's VACF is reduced by block 0 of this step's kernel
-            a.partials = c->part + 3 * (uint64_t)gt * (it & 1);
-            a.prev_partials = (!stop_on_convergence && k > 0) ? c->part + 3 * (uint64_t)gt * ((it - 1) & 1) : nullptr;
-            a.prev_vtrace = c->vtrace + 3 * (k - 1);
-            a.it = it;
-            a.ctrl = c->ctrl;
-            a.dlx = c->dlx;
-            a.dly = c->dly;
-            a.lz = c->lz;
-            a.Nf = c->Nf;
-            a.Np = c->Np;
-            a.ka = c->ka;
-            a.kd = c->kd;
-            a.one_minus_kd = 1.0 - c->kd;
-            a.ads = c->ads;
-            hipLaunchKernelGGL(c->wide ? k_mp_step<true> : k_mp_step<false>, dim3(gt), dim3(kBlock), 0, c->stream, a);
-            if (stop_on_convergence)
-                hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kBlock), 0, c->stream, a.partials, gt,
-                                   c->vtrace + 3 * k, c->ctrl, 1, it, 1, lim1, lim2);
+            if (c->prec == LBX_PREC_FP32_SHIFT)
+                launch_mp_step<float>(c, now, it, k, stop_on_convergence, lim1, lim2);
+            else
+                launch_mp_step<double>(c, now, it, k, stop_on_convergence, lim1, lim2);
         }
         timer_stop(c);
         if (!stop_on_convergence) {  // the last step's VACF
             const int64_t it = c->mp_it + S;
-            hipLaunchKernelGGL(k_mp_reduce, dim3(1), dim3(kBlock), 0, c->stream,
-                               c->part + 3 * (uint64_t)gt * (it & 1), gt, c->vtrace + 3 * (S - 1), c->ctrl, 1, it, 0,
-                               lim1, lim2);
+            with_tracers(c->ntr, [&](auto TT) {
+                constexpr int T = TT;
+                hipLaunchKernelGGL(k_mp_reduce<3 * T>, dim3(1), dim3(kBlock), 0, c->stream,
+                                   c->part + (uint64_t)nv * gt * (it & 1), gt, c->vtrace + nv * (S - 1), c->ctrl, 1,
+                                   it, 0, lim1, lim2);
+            });
         }
         LBX_TRY(hipGetLastError());
-        LBX_TRY(hipMemcpyAsync(vt.data(), c->vtrace, 3 * S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        LBX_TRY(hipMemcpyAsync(vt.data(), c->vtrace, nv * S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         LBX_TRY(read_ctrl(c, &h));
         int64_t ran = S;
         if (h.mp_stop) ran = std::min<int64_t>(S, h.mp_conv_step - c->mp_it);
         timer_end(c, LBX_KERNEL_MP, ran);
-        if (vacf_trace) std::memcpy(vacf_trace + 3 * done, vt.data(), 3 * ran * sizeof(double));
+        if (vacf_trace) std::memcpy(vacf_trace + nv * done, vt.data(), nv * ran * sizeof(double));
         c->mp_it += ran;
         if (ran & 1) c->mp_now ^= 1;
         done += ran;
@@ -1183,23 +1544,42 @@ int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* 
     return LBX_OK;
 }
 
-int lbx_mp_get(lbx_ctx* c, double* P, double* Pa) {
+int lbx_mp_tracers(lbx_ctx* c, int32_t* ntracers) {
+    if (!c || !ntracers) return LBX_ERR_ARG;
+    *ntracers = c->mp_ready ? c->ntr : 0;
+    return LBX_OK;
+}
+
+int lbx_mp_get_tracer(lbx_ctx* c, int32_t tracer, double* P, double* Pa) {
     if (!c) return LBX_ERR_ARG;
     if (!c->mp_ready) return LBX_ERR_STATE;
+    if (tracer < 0 || tracer >= c->ntr) return LBX_ERR_ARG;
     std::vector<double> full(3 * (size_t)c->N);
+    const uint64_t off = 3 * (uint64_t)tracer * c->Np;
     auto fetch = [&](const double* src, double* dst) -> int {
-        LBX_TRY(download_full(c, src, 3, full.data()));
+        LBX_TRY(download_full(c, src + off, 3, full.data()));
         for (uint32_t r = 0; r < c->N; ++r)
             for (int a = 0; a < 3; ++a) dst[3 * (size_t)r + a] = full[(size_t)a * c->N + r];
         return LBX_OK;
     };
     if (P) LBX_TRY(fetch(c->P[c->mp_now], P));
     if (Pa) {
-        if (c->ads)
+        if (c->tp.ads[tracer])
             LBX_TRY(fetch(c->Pa[c->mp_now], Pa));
         else
             std::memset(Pa, 0, 3 * (size_t)c->N * sizeof(double));
     }
+    return LBX_OK;
+}
+
+int lbx_mp_get(lbx_ctx* c, double* P, double* Pa) { return lbx_mp_get_tracer(c, 0, P, Pa); }
+
+int lbx_mp_deff_batch(lbx_ctx* c, double* deff) {
+    if (!c || !deff) return LBX_ERR_ARG;
+    if (!c->mp_ready) return LBX_ERR_STATE;
+    Ctrl h;
+    LBX_TRY(read_ctrl(c, &h));
+    for (int a = 0; a < 3 * c->ntr; ++a) deff[a] = h.deff_s[a] + h.deff_c[a];
     return LBX_OK;
 }
 
@@ -1277,6 +1657,13 @@ int lbx_kernel_time(lbx_ctx* c, int32_t which, double* total_ms, int64_t* launch
     if (!c || which < 0 || which > 1) return LBX_ERR_ARG;
     if (total_ms) *total_ms = c->t_ms[which];
     if (launches) *launches = c->t_n[which];
+    return LBX_OK;
+}
+
+int lbx_set_floor_window(lbx_ctx* c, int64_t steps) {
+    if (!c || steps < 0) return LBX_ERR_ARG;
+    if (steps > 0 && c->prec != LBX_PREC_FP32_SHIFT) return LBX_ERR_ARG;
+    c->floor_window = steps;
     return LBX_OK;
 }
 

@@ -31,15 +31,15 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
-template <int S>
-__device__ __forceinline__ double sgn(double x) {
+template <int S, typename V>
+__device__ __forceinline__ V sgn(V x) {
     static_assert(S == 1 || S == -1, "sign factor");
     if constexpr (S > 0) return x; else return -x;
 }
 
 // (a*x + b*y) + c*z for a, b, c in {-1, 0, 1}, zero terms dropped.
-template <int A, int B, int C>
-__device__ __forceinline__ double dot3(double x, double y, double z) {
+template <int A, int B, int C, typename V>
+__device__ __forceinline__ V dot3(V x, V y, V z) {
     if constexpr (A != 0) {
         double s = sgn<A>(x);
         if constexpr (B != 0) s = s + sgn<B>(y);
@@ -52,7 +52,7 @@ __device__ __forceinline__ double dot3(double x, double y, double z) {
     } else if constexpr (C != 0) {
         return sgn<C>(z);
     } else {
-        return 0.0;
+        return V(0);
     }
 }
 
@@ -86,6 +86,39 @@ __device__ __forceinline__ double collide_pop(double f, double rho, double jx, d
     double g = w1 * ((((double)cx - ux) * fx + ((double)cy - uy) * fy) + ((double)cz - uz) * fz);
     if constexpr (L != 0) g = g + ((2.0 * w2) * dot3<cx, cy, cz>(ux, uy, uz)) * dot3<cx, cy, cz>(fx, fy, fz);
     return (A * f + B * neq) + C * g;
+}
+
+// The same collision for the FP32-shift form, where a population is stored as
+// its deviation g = n - a0 rho0 from the rest state of density rho0:
+//   g' = n' - a0 rho0 = (1-1/tau) g + (1/tau) (neq - a0 rho0) + (1-1/(2 tau)) G
+// (because (1-1/tau) + 1/tau = 1), with neq - a0 rho0 = a0 drho + a1 (c.j) +
+// a2 sum_ab j_a u_b (c_a c_b - cs^2 delta_ab) and drho = rho - rho0 = sum g.
+// All of it in FP32: the deviations stay small, so their rounding error is
+// relative to the deviation, not to rho0.
+template <int L>
+__device__ __forceinline__ float collide_shift(float g, float drho, float jx, float jy, float jz, float ux,
+                                               float uy, float uz, float fx, float fy, float fz, float A, float B,
+                                               float C) {
+    constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
+    constexpr float w0 = (float)a0(L), w1 = (float)a1(L), w2 = (float)a2(L);
+    constexpr float Kx = (float)((double)(cx * cx) - kCsq);
+    constexpr float Ky = (float)((double)(cy * cy) - kCsq);
+    constexpr float Kz = (float)((double)(cz * cz) - kCsq);
+    float q = (jx * ux) * Kx;
+    if constexpr (cx * cy != 0) q = q + sgn<cx * cy>(jx * uy);
+    if constexpr (cx * cz != 0) q = q + sgn<cx * cz>(jx * uz);
+    if constexpr (cy * cx != 0) q = q + sgn<cy * cx>(jy * ux);
+    q = q + (jy * uy) * Ky;
+    if constexpr (cy * cz != 0) q = q + sgn<cy * cz>(jy * uz);
+    if constexpr (cz * cx != 0) q = q + sgn<cz * cx>(jz * ux);
+    if constexpr (cz * cy != 0) q = q + sgn<cz * cy>(jz * uy);
+    q = q + (jz * uz) * Kz;
+    float dneq = w0 * drho;
+    if constexpr (L != 0) dneq = dneq + w1 * dot3<cx, cy, cz>(jx, jy, jz);
+    dneq = dneq + w2 * q;
+    float G = w1 * ((((float)cx - ux) * fx + ((float)cy - uy) * fy) + ((float)cz - uz) * fz);
+    if constexpr (L != 0) G = G + ((2.0f * w2) * dot3<cx, cy, cz>(ux, uy, uz)) * dot3<cx, cy, cz>(fx, fy, fz);
+    return (A * g + B * dneq) + C * G;
 }
 
 // Division by a runtime-invariant divisor 1 <= d < 2^31 as a multiply-high
@@ -124,40 +157,49 @@ __device__ __forceinline__ void st(T* base, uint32_t idx, T v) {
 // SGPRs, so a lane needs one address register instead of a 64-bit pointer, and
 // an access outside [0, bytes) reads 0 / writes nothing instead of faulting.
 // 0x00020000 is the gfx9-family descriptor word 3 for raw 32-bit-dword data.
+// E is the element type: double (b64 accesses) or float (b32).
 constexpr int kBufWord3 = 0x00020000;
+template <typename E>
 struct Plane {
+    static_assert(std::is_same<E, double>::value || std::is_same<E, float>::value, "double or float elements");
     __amdgpu_buffer_rsrc_t r;
     __device__ __forceinline__ Plane(const void* p, uint64_t bytes)
         : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0,
                                               (int)(uint32_t)(bytes < 0xFFFFFFFFull ? bytes : 0xFFFFFFFFull),
                                               kBufWord3)) {}
-    __device__ __forceinline__ double ld(uint32_t idx) const {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8u, 0, 0));
+    __device__ __forceinline__ E ld(uint32_t idx) const {
+        if constexpr (sizeof(E) == 8)
+            return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8u, 0, 0));
+        else
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4u, 0, 0));
     }
-    __device__ __forceinline__ void st(uint32_t idx, double v) const {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
-                                              idx * 8u, 0, 0);
+    __device__ __forceinline__ void st(uint32_t idx, E v) const {
+        if constexpr (sizeof(E) == 8)
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
+                                                  r, idx * 8u, 0, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, idx * 4u, 0, 0);
     }
 };
 
-// A set of planes of doubles addressed by a 32-bit element index.  Narrow:
-// a buffer resource over the whole set (needs index*8 < 2^32); Wide: plain
-// 64-bit pointers (any size).
-template <bool Wide>
+// A set of planes of E addressed by a 32-bit element index.  Narrow: a
+// buffer resource over the whole set (needs index*sizeof(E) < 2^32); Wide:
+// plain 64-bit pointers (any size).
+template <bool Wide, typename E = double>
 struct View;
-template <>
-struct View<false> {
-    Plane p;
-    __device__ __forceinline__ View(const double* base, uint64_t n) : p(base, n * 8) {}
-    __device__ __forceinline__ double ld(uint32_t idx) const { return p.ld(idx); }
-    __device__ __forceinline__ void st(uint32_t idx, double v) const { p.st(idx, v); }
+template <typename E>
+struct View<false, E> {
+    Plane<E> p;
+    __device__ __forceinline__ View(const E* base, uint64_t n) : p(base, n * sizeof(E)) {}
+    __device__ __forceinline__ E ld(uint32_t idx) const { return p.ld(idx); }
+    __device__ __forceinline__ void st(uint32_t idx, E v) const { p.st(idx, v); }
 };
-template <>
-struct View<true> {
-    double* b;
-    __device__ __forceinline__ View(const double* base, uint64_t) : b(const_cast<double*>(base)) {}
-    __device__ __forceinline__ double ld(uint32_t idx) const { return b[(uint64_t)idx]; }
-    __device__ __forceinline__ void st(uint32_t idx, double v) const { b[(uint64_t)idx] = v; }
+template <typename E>
+struct View<true, E> {
+    E* b;
+    __device__ __forceinline__ View(const E* base, uint64_t) : b(const_cast<E*>(base)) {}
+    __device__ __forceinline__ E ld(uint32_t idx) const { return b[(uint64_t)idx]; }
+    __device__ __forceinline__ void st(uint32_t idx, E v) const { b[(uint64_t)idx] = v; }
 };
 
 // Periodic neighbour offsets of a node: X[1+d] is the x index of x+d, and Y,

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "liblbx.so")
 
 OK, ERR_NEG_POP, ERR_RESTPART, ERR_TAU, ERR_NOMEM, ERR_ARG, ERR_DEVICE, ERR_STATE = range(8)
 PREC_FP64, PREC_FP32_SHIFT = 0, 1
-EXIT_CONVERGED, EXIT_CAP = 1, 2
+EXIT_CONVERGED, EXIT_CAP, EXIT_FLOOR = 1, 2, 3
 KERNEL_LB, KERNEL_MP = 0, 1
 
 _p = ctypes.c_void_p
@@ -42,13 +42,18 @@ SIGNATURES = {
     "lbx_l2err_trace": (ctypes.c_int, [_p, _i64, _i64, _p, _p]),
     "lbx_mp_init": (ctypes.c_int, [_p, _p, _f64, _f64, _f64, _p]),
     "lbx_mp_init_from_flow": (ctypes.c_int, [_p] * 6 + [_f64] * 3 + [_p]),
+    "lbx_mp_init_batch": (ctypes.c_int, [_p, _p, _f64, _i32, _p, _p, _p]),
+    "lbx_mp_tracers": (ctypes.c_int, [_p, _p]),
     "lbx_mp_run": (ctypes.c_int, [_p, _i64, _i32, _p, _p, _p]),
     "lbx_mp_get": (ctypes.c_int, [_p, _p, _p]),
+    "lbx_mp_get_tracer": (ctypes.c_int, [_p, _i32, _p, _p]),
     "lbx_mp_deff": (ctypes.c_int, [_p, _p]),
+    "lbx_mp_deff_batch": (ctypes.c_int, [_p, _p]),
     "lbx_mp_free": (ctypes.c_int, [_p]),
     "lbx_collide": (ctypes.c_int, [_i32] * 3 + [_p, _f64] + [_p] * 8),
     "lbx_timing_enable": (ctypes.c_int, [_p, _i32]),
     "lbx_kernel_time": (ctypes.c_int, [_p, _i32, _p, _p]),
+    "lbx_set_floor_window": (ctypes.c_int, [_p, _i64]),
     "lbx_fluid_nodes": (_i64, [_p]),
     "lbx_sync": (ctypes.c_int, [_p]),
     "lbx_copy_bandwidth": (ctypes.c_int, [_i64, _i32, _p]),
@@ -110,8 +115,9 @@ def detect_interfacial(lx, ly, lz, nature):
     return itf
 
 
-def footprint_bytes(lx, ly, lz, precision=PREC_FP64, with_mp=True):
-    return int(lib().lbx_footprint_bytes(lx, ly, lz, precision, 1 if with_mp else 0))
+def footprint_bytes(lx, ly, lz, precision=PREC_FP64, with_mp=1):
+    """HBM bytes of a context; with_mp = number of tracers (True = 1)."""
+    return int(lib().lbx_footprint_bytes(lx, ly, lz, precision, int(with_mp)))
 
 
 KD_DEFAULT = 0.01
@@ -147,6 +153,7 @@ class Lattice:
     """One lattice resident in HBM (an lbx_ctx)."""
 
     def __init__(self, dims, nature, tau=1.0, precision=PREC_FP64):
+        self._batch = False
         self.dims = tuple(int(v) for v in dims)
         lx, ly, lz = self.dims
         self.N = lx * ly * lz
@@ -173,6 +180,11 @@ class Lattice:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_floor_window(self, steps):
+        """FP32-shift form: end a stage when l2err set no new minimum for
+        `steps` steps (0: off; default 1000 for FP32-shift contexts)."""
+        _check(lib().lbx_set_floor_window(self._h, int(steps)), "set_floor_window")
 
     @property
     def fluid_nodes(self):
@@ -222,23 +234,54 @@ class Lattice:
             arrs = [np.ascontiguousarray(a, np.float64) for a in (rho, j[0], j[1], j[2])]
             _check(lib().lbx_mp_init_from_flow(self._h, *[_ptr(a) for a in arrs], _ptr(fe), float(Db), float(ka),
                                                float(kd), _ptr(v0)), "mp_init_from_flow")
+        self._batch = False
         return v0
 
+    def mp_init_batch(self, fext, Db, ka, kd):
+        """Tracers with the given ka[t], kd[t] (up to 4) propagated together;
+        returns VACF(0) as (ntracers, 3)."""
+        fe = np.ascontiguousarray(fext, np.float64)
+        ka = np.ascontiguousarray(ka, np.float64).ravel()
+        kd = np.ascontiguousarray(kd, np.float64).ravel()
+        if ka.size != kd.size:
+            raise ValueError("ka and kd need one value per tracer")
+        v0 = np.zeros(3 * max(ka.size, 1))
+        _check(lib().lbx_mp_init_batch(self._h, _ptr(fe), float(Db), int(ka.size), _ptr(ka), _ptr(kd), _ptr(v0)),
+               "mp_init_batch")
+        self._batch = True
+        return v0.reshape(-1, 3)
+
+    @property
+    def mp_tracers(self):
+        n = _i32()
+        _check(lib().lbx_mp_tracers(self._h, ctypes.byref(n)), "mp_tracers")
+        return n.value
+
     def mp_run(self, nsteps, stop_on_convergence=False, check=True):
-        tr = np.zeros(3 * max(nsteps, 1))
+        """VACF per step: (steps, 3) after mp_init, (steps, ntracers, 3)
+        after mp_init_batch."""
+        nt = max(self.mp_tracers, 1)
+        tr = np.zeros(3 * nt * max(nsteps, 1))
         sd, cv = _i64(), _i32()
         st = lib().lbx_mp_run(self._h, int(nsteps), 1 if stop_on_convergence else 0, _ptr(tr), ctypes.byref(sd),
                               ctypes.byref(cv))
         if check and st != OK:
             raise LbxError(st, "mp_run")
-        return dict(status=st, steps=sd.value, converged=bool(cv.value), vacf=tr[:3 * sd.value].reshape(-1, 3))
+        v = tr[:3 * nt * sd.value]
+        v = v.reshape(-1, nt, 3) if self._batch else v.reshape(-1, 3)
+        return dict(status=st, steps=sd.value, converged=bool(cv.value), vacf=v)
 
-    def mp_get(self):
+    def mp_get(self, tracer=0):
         P, Pa = np.zeros(3 * self.N), np.zeros(3 * self.N)
-        _check(lib().lbx_mp_get(self._h, _ptr(P), _ptr(Pa)), "mp_get")
+        _check(lib().lbx_mp_get_tracer(self._h, int(tracer), _ptr(P), _ptr(Pa)), "mp_get_tracer")
         return P, Pa
 
     def mp_deff(self):
+        """D_eff: (3,) after mp_init, (ntracers, 3) after mp_init_batch."""
+        if self._batch:
+            d = np.zeros(3 * self.mp_tracers)
+            _check(lib().lbx_mp_deff_batch(self._h, _ptr(d)), "mp_deff_batch")
+            return d.reshape(-1, 3)
         d = np.zeros(3)
         _check(lib().lbx_mp_deff(self._h, _ptr(d)), "mp_deff")
         return d

@@ -32,6 +32,7 @@ extern "C" {
 /* exit reasons of lbref_equilibrate */
 #define LBREF_EXIT_CONVERGED 1  /* second convergence, equilibration.f90:373-374 */
 #define LBREF_EXIT_CAP       2  /* step cap reached (extension; the reference has none) */
+#define LBREF_EXIT_FLOOR     3  /* FP32-shift form: l2err stopped improving (lbshift_equilibrate) */
 
 void lbref_tables(int32_t c[19 * 3], double a0[19], double a1[19], double a2[19], int32_t inv[19]);
 
@@ -67,6 +68,17 @@ int lbref_mp_step(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, cons
 int lbref_mp_converged(int64_t it, const double vacf[3], int32_t lx, int32_t ly, int32_t lz, double Db);
 
 void lbref_deff(const double* vacf, int64_t nsteps, double deff[3]);
+
+/* lbshift.c: the FP32-shift precision form (not in the reference), in the
+ * per-node operation order of the library's FP32-shift kernels. */
+int lbshift_equilibrate(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double rho0,
+                        const double fext[3], double target_error, int64_t max_steps, int64_t floor_window,
+                        double* n, double* rho, double* jx, double* jy, double* jz, double* l2err_trace,
+                        int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason);
+
+int lbshift_mp(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, const int8_t* interfacial,
+               const double* rho, const double* jx, const double* jy, const double* jz, const double fext[3],
+               double Db, double ka, double kd, int64_t nsteps, double* P, double* Pa, double* vacf);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,7 @@ _LIB_PATH = os.path.join(_HERE, "liblbref.so")
 _lib = None
 
 OK, NEG_POP, RESTPART, BAD_TAU, BAD_ARG = 0, 1, 2, 3, 5
-EXIT_CONVERGED, EXIT_CAP = 1, 2
+EXIT_CONVERGED, EXIT_CAP, EXIT_FLOOR = 1, 2, 3
 
 _p = ctypes.c_void_p
 _i32, _i64, _f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -40,6 +40,8 @@ def lib():
         L.lbref_mp_step.argtypes = [_i32] * 3 + [_p] * 4 + [_f64] * 3 + [_i32] + [_p] * 5
         L.lbref_mp_converged.argtypes = [_i64, _p, _i32, _i32, _i32, _f64]
         L.lbref_deff.argtypes = [_p, _i64, _p]
+        L.lbshift_equilibrate.argtypes = [_i32] * 3 + [_p, _f64, _f64, _p, _f64, _i64, _i64] + [_p] * 6 + [_i64, _p, _p]
+        L.lbshift_mp.argtypes = [_i32] * 3 + [_p] * 7 + [_f64] * 3 + [_i64] + [_p] * 3
         _lib = L
     return _lib
 
@@ -166,3 +168,40 @@ def deff(vacf):
     d = np.zeros(3)
     lib().lbref_deff(_ptr(v), v.shape[0] - 1, _ptr(d))
     return d
+
+
+# ---------------------------------------------------------------- FP32-shift
+def equilibrate_shift(dims, nature, tau, rho0, fext, target_error, max_steps=0, floor_window=1000,
+                      trace_cap=1 << 20):
+    """lbshift_equilibrate: the FP32-shift form of equilibrate (same outputs,
+    FP64 n / rho / j built from the FP32 deviations)."""
+    lx, ly, lz = dims
+    N = lx * ly * lz
+    n = np.zeros(19 * N)
+    rho, jx, jy, jz = (np.zeros(N) for _ in range(4))
+    tr = np.zeros(trace_cap)
+    sd, er = _i64(), _i32()
+    fe = np.ascontiguousarray(fext, np.float64)
+    st = lib().lbshift_equilibrate(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)), tau, rho0, _ptr(fe),
+                                   target_error, max_steps, floor_window, _ptr(n), _ptr(rho), _ptr(jx), _ptr(jy),
+                                   _ptr(jz), _ptr(tr), trace_cap, ctypes.byref(sd), ctypes.byref(er))
+    steps = sd.value
+    nl2 = steps - 1 if st == NEG_POP else steps
+    return dict(status=st, steps=steps, exit_reason=er.value, n=n, rho=rho, j=np.stack([jx, jy, jz]),
+                l2err=tr[:min(nl2, trace_cap)].copy())
+
+
+def mp_shift(dims, nature, interfacial, rho, j, fext, Db, ka, kd, nsteps):
+    """lbshift_mp: FP32-shift moment propagation; returns (status, vacf
+    (nsteps+1, 3), P, Pa)."""
+    lx, ly, lz = dims
+    N = lx * ly * lz
+    P, Pa = np.zeros(3 * N), np.zeros(3 * N)
+    v = np.zeros(3 * (nsteps + 1))
+    js = [np.ascontiguousarray(j[a], np.float64) for a in range(3)]
+    fe = np.ascontiguousarray(fext, np.float64)
+    st = lib().lbshift_mp(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)),
+                          _ptr(np.ascontiguousarray(interfacial, np.int8)),
+                          _ptr(np.ascontiguousarray(rho, np.float64)), *[_ptr(a) for a in js], _ptr(fe),
+                          float(Db), float(ka), float(kd), int(nsteps), _ptr(P), _ptr(Pa), _ptr(v))
+    return st, v.reshape(-1, 3), P, Pa

@@ -102,6 +102,51 @@ def test_mp_from_host_flow_equals_resident_flow():
         assert np.array_equal(v_res, v_host) and np.array_equal(a, b)
 
 
+def test_mp_batch_equals_single_tracer_runs():
+    """Three tracers propagated together (lbx_mp_init_batch) give, tracer by
+    tracer, the same bits as three single-tracer runs: VACF, P, Pa, D_eff."""
+    c = Case("bcc12_ads")
+    ka = [c.ka, 0.0, lbx.ka_kd_from_dF(-2.0)[0]]
+    kd = [c.kd, 0.0, lbx.KD_DEFAULT]
+    steps = 60
+    say("batch", c.dims, ka, kd)
+    with lbx.Lattice(c.dims, c.nature, c.tau) as L:
+        L.init_rest(c.rho0)
+        L.equilibrate(c.fext, c.target)
+        single = []
+        for t in range(3):
+            v0 = L.mp_init(c.fext, c.Db, ka[t], kd[t])
+            v = L.mp_run(steps)["vacf"]
+            P, Pa = L.mp_get()
+            single.append((v0, v, P, Pa, L.mp_deff()))
+        v0b = L.mp_init_batch(c.fext, c.Db, ka, kd)
+        assert L.mp_tracers == 3
+        vb = L.mp_run(steps)["vacf"]
+        assert vb.shape == (steps, 3, 3)
+        db = L.mp_deff()
+        for t in range(3):
+            P, Pa = L.mp_get(t)
+            s0, sv, sP, sPa, sd = single[t]
+            assert np.array_equal(v0b[t], s0)
+            assert np.array_equal(vb[:, t, :], sv)
+            assert np.array_equal(P, sP) and np.array_equal(Pa, sPa)
+            assert np.array_equal(db[t], sd)
+        # one call per step gives the same as one run
+        L.mp_init_batch(c.fext, c.Db, ka, kd)
+        one = np.concatenate([L.mp_run(1)["vacf"] for _ in range(steps)])
+        assert np.array_equal(one, vb)
+        # the convergence test is per tracer: a batch cannot stop on it
+        r = L.mp_run(5, stop_on_convergence=True, check=False)
+        assert r["status"] == lbx.ERR_ARG
+        for bad in ([], [0.0] * 5):
+            with pytest.raises(lbx.LbxError) as e:
+                L.mp_init_batch(c.fext, c.Db, bad, bad)
+            assert e.value.status == lbx.ERR_ARG
+        # any tracer whose rest probability drops below epsilon stops the batch
+        L.mp_init_batch(c.fext, c.Db, [0.0, 2.0], [0.0, 0.01])
+        assert L.mp_run(3, check=False)["status"] == lbx.ERR_RESTPART
+
+
 def test_error_statuses():
     nat = lbx.build_geometry(1, 3, 3, 6)
     with pytest.raises(lbx.LbxError) as e:
