@@ -606,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs<W> a) {
                 const double wx = a.j[q] + a.fx, wy = a.j[a.Np + q] + a.fy, wz = a.j[2 * a.Np + q] + a.fz;
                 const double ntn = a0(Li) * rn + a1(Li) * dot3<-cx, -cy, -cz>(wx, wy, wz);
                 const double sp = as_coef<W>(ntn / rn - a0(Li) + a.lambda * a0(Li) * 0.5);
-                a.win[(uint64_t)(L - 1) * a.Np + i] = (W)sp;
+                a.win[comp_index(18, L - 1, i, a.Np)] = (W)sp;
 #pragma unroll
                 for (int t = 0; t < T; ++t) {
                     if constexpr (cx != 0) p1[3 * t + 0] = p1[3 * t + 0] + sgn<-cx>(sp) * a.tp.bw[t];
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs<W> a) {
                     if constexpr (cz != 0) p1[3 * t + 2] = p1[3 * t + 2] + sgn<-cz>(sp) * a.tp.bw[t];
                 }
             } else {
-                a.win[(uint64_t)(L - 1) * a.Np + i] = W(0);
+                a.win[comp_index(18, L - 1, i, a.Np)] = W(0);
             }
         });
         a.rem[i] = remv;
@@ -625,9 +625,9 @@ __global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs<W> a) {
             low = low || rt < a.eps;
         }
         if (low) a.ctrl->restpart = 1;
-        for (int c = 0; c < 3; ++c) a.ustar[c * a.Np + i] = (W)us[c];
+        for (int c = 0; c < 3; ++c) a.ustar[comp_index(3, c, i, a.Np)] = (W)us[c];
 #pragma unroll
-        for (int k = 0; k < 3 * T; ++k) a.P0[(uint64_t)k * a.Np + i] = p1[k];
+        for (int k = 0; k < 3 * T; ++k) a.P0[comp_index(3 * T, k, i, a.Np)] = p1[k];
     }
     block_partial<3 * T>(vp, a.partials, lds);
 }
@@ -651,6 +651,7 @@ struct MpStepArgs {
     FastDiv dlx, dly;
     uint32_t lz, Nf;
     uint64_t Np;
+    int32_t any_ads;                 // some tracer of the batch adsorbs (Pan, Pax allocated)
     TracerParams tp;
 };
 
@@ -660,60 +661,73 @@ struct MpStepArgs {
 // VACF(t) = sum_r P(r,t) . u*(r).
 template <bool Wide, int T, typename W>
 __global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs<W> a) {
-    __shared__ double lds[kBlock / 64 * 3 * T];
+    constexpr int K = 3 * T;  // moment components of the batch
+    __shared__ double lds[kBlock / 64 * K];
     if (vload(&a.ctrl->mp_stop)) return;
     if (a.prev_partials && blockIdx.x == 0)
-        mp_reduce_block<3 * T>(a.prev_partials, gridDim.x, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
+        mp_reduce_block<K>(a.prev_partials, gridDim.x, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
     const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
-    double vp[3 * T];
+    double vp[K];
 #pragma unroll
-    for (int k = 0; k < 3 * T; ++k) vp[k] = 0.0;
+    for (int k = 0; k < K; ++k) vp[k] = 0.0;
     if (i < a.Nf) {
         const uint32_t Np = (uint32_t)a.Np;
         const View<Wide, W> win(a.win, 18 * a.Np), us(a.ustar, 3 * a.Np);
-        const View<Wide> Pn(a.Pn, 3 * T * a.Np), Px(a.Px, 3 * T * a.Np), rem(a.rem, a.Np);
+        const View<Wide> Pn(a.Pn, K * a.Np), Px(a.Px, K * a.Np), rem(a.rem, a.Np);
         const uint32_t e = ld(a.nodes, i);
         Neighbours nb;
         nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
-        double p[3 * T], acc[3 * T];
+        double p[K], acc[K];
+        load_comps<K>(Pn, Np, i, p);
 #pragma unroll
-        for (int k = 0; k < 3 * T; ++k) {
-            p[k] = Pn.ld((uint32_t)k * Np + i);
-            acc[k] = 0.0;
-        }
-        double u[3];
-        for (int c = 0; c < 3; ++c) u[c] = (double)us.ld(c * Np + i);
+        for (int k = 0; k < K; ++k) acc[k] = 0.0;
+        W uw[3];
+        load_comps<3>(us, Np, i, uw);
         const double rm = rem.ld(i);
+        Vec2<W> wp;  // the probabilities of directions L, L+1 (L odd) from one paired load
         static_for<1, kQ>([&](auto I) {
             constexpr int L = I;
+            if constexpr (L & 1) wp = win.ld2((uint32_t)((L - 1) / 2) * Np + i);
             const uint32_t q = nb.q[L];
             if (q != kSolidSlot) {
-                const double w = (double)win.ld((uint32_t)(L - 1) * Np + i);
+                const double w = (double)wp[(L - 1) & 1];
+                double pq[K];
+                load_comps<K>(Pn, Np, q, pq);
 #pragma unroll
-                for (int k = 0; k < 3 * T; ++k) acc[k] = acc[k] + Pn.ld((uint32_t)k * Np + q) * w;
+                for (int k = 0; k < K; ++k) acc[k] = acc[k] + pq[k] * w;
             }
         });
 #pragma unroll
         for (int t = 0; t < T; ++t)
-            for (int c = 0; c < 3; ++c) vp[3 * t + c] = p[3 * t + c] * u[c];
-        const bool itf = e & kNodeInterfacial;
+            for (int c = 0; c < 3; ++c) vp[3 * t + c] = p[3 * t + c] * (double)uw[c];
+        double px[K];
+        if (!(a.any_ads && (e & kNodeInterfacial))) {
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-            if (!a.tp.ads[t] || !itf) {
-                for (int c = 0; c < 3; ++c) Px.st((uint32_t)(3 * t + c) * Np + i, acc[3 * t + c] + rm * p[3 * t + c]);
-            } else {
-                const View<Wide> Pan(a.Pan, 3 * T * a.Np), Pax(a.Pax, 3 * T * a.Np);
+            for (int k = 0; k < K; ++k) px[k] = acc[k] + rm * p[k];
+            store_comps<K>(Px, Np, i, px);
+        } else {
+            const View<Wide> Pan(a.Pan, K * a.Np), Pax(a.Pax, K * a.Np);
+            double pa[K], pax[K];
+            load_comps<K>(Pan, Np, i, pa);
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
                 const double rt = rm - a.tp.ka[t];
                 for (int c = 0; c < 3; ++c) {
-                    const uint32_t k = (uint32_t)(3 * t + c) * Np + i;
-                    const double pa = Pan.ld(k);
-                    Px.st(k, (acc[3 * t + c] + rt * p[3 * t + c]) + pa * a.tp.kd[t]);
-                    Pax.st(k, pa * a.tp.one_minus_kd[t] + p[3 * t + c] * a.tp.ka[t]);
+                    const int k = 3 * t + c;
+                    if (a.tp.ads[t]) {
+                        px[k] = (acc[k] + rt * p[k]) + pa[k] * a.tp.kd[t];
+                        pax[k] = pa[k] * a.tp.one_minus_kd[t] + p[k] * a.tp.ka[t];
+                    } else {  // a tracer without adsorption keeps Pa = 0
+                        px[k] = acc[k] + rm * p[k];
+                        pax[k] = pa[k];
+                    }
                 }
             }
+            store_comps<K>(Px, Np, i, px);
+            store_comps<K>(Pax, Np, i, pax);
         }
     }
-    block_partial<3 * T>(vp, a.partials, lds);
+    block_partial<K>(vp, a.partials, lds);
 }
 
 // Plain streaming copy (16 B per lane per iteration, grid-stride): the
@@ -1079,6 +1093,7 @@ void launch_mp_step(lbx_ctx* c, int now, int64_t it, int64_t k, int32_t stop_on_
     a.lz = c->lz;
     a.Nf = c->Nf;
     a.Np = c->Np;
+    a.any_ads = c->any_ads;
     a.tp = c->tp;
     with_tracers(c->ntr, [&](auto TT) {
         constexpr int T = TT;
@@ -1181,6 +1196,11 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
         return LBX_ERR_ARG;
     }
     c->Np = ((uint64_t)c->Nf + 63) / 64 * 64;
+    // element indices of the population set (19*Np) are 32-bit
+    if (kQ * c->Np >= (1ull << 32)) {
+        delete c;
+        return LBX_ERR_ARG;
+    }
     c->wide = kQ * c->Np * 8 >= (1ull << 32);
     c->ntiles = nblocks(c->Nf);
     int st = LBX_OK;
@@ -1554,12 +1574,18 @@ int lbx_mp_get_tracer(lbx_ctx* c, int32_t tracer, double* P, double* Pa) {
     if (!c) return LBX_ERR_ARG;
     if (!c->mp_ready) return LBX_ERR_STATE;
     if (tracer < 0 || tracer >= c->ntr) return LBX_ERR_ARG;
-    std::vector<double> full(3 * (size_t)c->N);
-    const uint64_t off = 3 * (uint64_t)tracer * c->Np;
+    const int K = 3 * c->ntr;
+    std::vector<double> packed((size_t)K * c->Np);
+    // the tracer's three components (comp_index layout) into (3,lx,ly,lz), 0 at solids
     auto fetch = [&](const double* src, double* dst) -> int {
-        LBX_TRY(download_full(c, src + off, 3, full.data()));
-        for (uint32_t r = 0; r < c->N; ++r)
-            for (int a = 0; a < 3; ++a) dst[3 * (size_t)r + a] = full[(size_t)a * c->N + r];
+        LBX_TRY(hipMemcpyAsync(packed.data(), src, packed.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        LBX_TRY(hipStreamSynchronize(c->stream));
+        std::memset(dst, 0, 3 * (size_t)c->N * sizeof(double));
+        for (uint32_t i = 0; i < c->Nf; ++i) {
+            const size_t r = c->nodes_h[i] & kIndexMask;
+            for (int a = 0; a < 3; ++a) dst[3 * r + a] = packed[comp_index(K, 3 * tracer + a, i, c->Np)];
+        }
         return LBX_OK;
     };
     if (P) LBX_TRY(fetch(c->P[c->mp_now], P));

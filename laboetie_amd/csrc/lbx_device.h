@@ -160,6 +160,8 @@ __device__ __forceinline__ void st(T* base, uint32_t idx, T v) {
 // E is the element type: double (b64 accesses) or float (b32).
 constexpr int kBufWord3 = 0x00020000;
 template <typename E>
+using Vec2 = E __attribute__((ext_vector_type(2)));
+template <typename E>
 struct Plane {
     static_assert(std::is_same<E, double>::value || std::is_same<E, float>::value, "double or float elements");
     __amdgpu_buffer_rsrc_t r;
@@ -180,6 +182,21 @@ struct Plane {
         else
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, idx * 4u, 0, 0);
     }
+    // elements 2*pidx and 2*pidx+1 in one access (b128 for double, b64 for float)
+    __device__ __forceinline__ Vec2<E> ld2(uint32_t pidx) const {
+        if constexpr (sizeof(E) == 8)
+            return __builtin_bit_cast(Vec2<E>, __builtin_amdgcn_raw_buffer_load_b128(r, pidx * 16u, 0, 0));
+        else
+            return __builtin_bit_cast(Vec2<E>, __builtin_amdgcn_raw_buffer_load_b64(r, pidx * 8u, 0, 0));
+    }
+    __device__ __forceinline__ void st2(uint32_t pidx, Vec2<E> v) const {
+        if constexpr (sizeof(E) == 8)
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v),
+                                                   r, pidx * 16u, 0, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
+                                                  r, pidx * 8u, 0, 0);
+    }
 };
 
 // A set of planes of E addressed by a 32-bit element index.  Narrow: a
@@ -193,6 +210,8 @@ struct View<false, E> {
     __device__ __forceinline__ View(const E* base, uint64_t n) : p(base, n * sizeof(E)) {}
     __device__ __forceinline__ E ld(uint32_t idx) const { return p.ld(idx); }
     __device__ __forceinline__ void st(uint32_t idx, E v) const { p.st(idx, v); }
+    __device__ __forceinline__ Vec2<E> ld2(uint32_t pidx) const { return p.ld2(pidx); }
+    __device__ __forceinline__ void st2(uint32_t pidx, Vec2<E> v) const { p.st2(pidx, v); }
 };
 template <typename E>
 struct View<true, E> {
@@ -200,7 +219,39 @@ struct View<true, E> {
     __device__ __forceinline__ View(const E* base, uint64_t) : b(const_cast<E*>(base)) {}
     __device__ __forceinline__ E ld(uint32_t idx) const { return b[(uint64_t)idx]; }
     __device__ __forceinline__ void st(uint32_t idx, E v) const { b[(uint64_t)idx] = v; }
+    __device__ __forceinline__ Vec2<E> ld2(uint32_t pidx) const { return ((const Vec2<E>*)b)[(uint64_t)pidx]; }
+    __device__ __forceinline__ void st2(uint32_t pidx, Vec2<E> v) const { ((Vec2<E>*)b)[(uint64_t)pidx] = v; }
 };
+
+// K per-node values of Np nodes (a moment field, the scattering
+// probabilities) laid out so that one 16-byte (double) or 8-byte (float)
+// access reads two of them: values 2p and 2p+1 (p < K/2) side by side in pair
+// plane p, an odd last value alone in a plane after the pairs.  A node's K
+// values then take ceil(K/2) loads instead of K.
+__host__ __device__ constexpr uint64_t comp_index(int K, int k, uint64_t i, uint64_t Np) {
+    return k < (K & ~1) ? ((uint64_t)(k >> 1) * Np + i) * 2 + (uint64_t)(k & 1) : (uint64_t)(K & ~1) * Np + i;
+}
+
+template <int K, typename V, typename E>
+__device__ __forceinline__ void load_comps(const V& v, uint32_t Np, uint32_t i, E (&out)[K]) {
+    static_for<0, K / 2>([&](auto P) {
+        const Vec2<E> x = v.ld2((uint32_t)P * Np + i);
+        out[2 * P] = x[0];
+        out[2 * P + 1] = x[1];
+    });
+    if constexpr (K & 1) out[K - 1] = v.ld((uint32_t)(K - 1) * Np + i);
+}
+
+template <int K, typename V, typename E>
+__device__ __forceinline__ void store_comps(const V& v, uint32_t Np, uint32_t i, const E (&in)[K]) {
+    static_for<0, K / 2>([&](auto P) {
+        Vec2<E> x;
+        x[0] = in[2 * P];
+        x[1] = in[2 * P + 1];
+        v.st2((uint32_t)P * Np + i, x);
+    });
+    if constexpr (K & 1) v.st((uint32_t)(K - 1) * Np + i, in[K - 1]);
+}
 
 // Periodic neighbour offsets of a node: X[1+d] is the x index of x+d, and Y,
 // Z the row / plane offsets of y+d, z+d (module_system.f90:99-112: only a
