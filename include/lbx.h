@@ -77,6 +77,27 @@ int lbx_build_geometry(int32_t label, int32_t lx, int32_t ly, int32_t lz,
 /* detectInterfacialNodes, supercell_definition.f90:194-226. */
 int lbx_detect_interfacial(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, int8_t* interfacial);
 
+/* Seeded synthetic media (no counterpart in the reference, which reads such
+ * media from geom.in files).  Random numbers: SplitMix64(seed).
+ *
+ * lbx_gaussian_field: i.i.d. N(0,1) per node (Box-Muller pairs, x fastest)
+ * convolved periodically with exp(-d^2/(2 sigma^2)) along each axis
+ * (|d| <= min(ceil(4 sigma), (n-1)/2)); field is (lx,ly,lz).
+ * lbx_geometry_random: that field thresholded so that exactly
+ * round(porosity*N) nodes are fluid (the lowest values; ties by index);
+ * solid clusters are all kept.
+ * lbx_geometry_disks: ndisks solid disks/spheres of the given radius at
+ * seeded centres, uniform in [1, n+1) per axis (1 on an axis of length 1),
+ * no two closer than 2*radius (periodic minimum image); a node is solid when
+ * its per-axis minimum-image distance to a centre is <= radius, the rule of
+ * tools/chromato_generator.f90:40-64.  centres (may be NULL) receives 3*ndisks
+ * 1-based coordinates. */
+int lbx_gaussian_field(int32_t lx, int32_t ly, int32_t lz, double sigma, uint64_t seed, double* field);
+int lbx_geometry_random(int32_t lx, int32_t ly, int32_t lz, double sigma, double porosity, uint64_t seed,
+                        int8_t* nature);
+int lbx_geometry_disks(int32_t lx, int32_t ly, int32_t lz, int32_t ndisks, double radius, uint64_t seed,
+                       int8_t* nature, double* centres);
+
 /* --------------------------------------------------------------- context */
 
 /* HBM bytes a context of this shape needs (admission control for batching

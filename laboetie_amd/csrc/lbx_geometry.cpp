@@ -128,3 +128,164 @@ extern "C" int lbx_detect_interfacial(int32_t lx, int32_t ly, int32_t lz, const 
             }
     return LBX_OK;
 }
+
+// ------------------------------------------------------------ generators
+// Seeded synthetic media for the workloads the reference runs on files it
+// does not generate itself.  Both draw from SplitMix64 (Steele, Lea, Flood,
+// "Fast splittable pseudorandom number generators", OOPSLA 2014) so that a
+// seed gives the same medium on every machine with an IEEE libm.
+namespace lbx {
+
+struct SplitMix64 {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    // uniform in [0, 1) with 53 random bits
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+}  // namespace lbx
+
+#include <algorithm>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// out[i] = sum_{d=-R..R} w[d+R] * in[(i+d) mod n] along one axis of a
+// (lx,ly,lz) x-fastest field, d in increasing order; planes split over threads.
+void conv_axis(const std::vector<double>& in, std::vector<double>& out, int32_t lx, int32_t ly, int32_t lz,
+               int axis, const std::vector<double>& w, int R) {
+    const int32_t n[3] = {lx, ly, lz};
+    const int64_t stride[3] = {1, (int64_t)lx, (int64_t)lx * ly};
+    const int32_t len = n[axis];
+    auto work = [&](int32_t k0, int32_t k1) {
+        for (int32_t k = k0; k < k1; ++k)
+            for (int32_t j = 0; j < ly; ++j)
+                for (int32_t i = 0; i < lx; ++i) {
+                    const int32_t c[3] = {i, j, k};
+                    const int64_t base = (int64_t)i + (int64_t)lx * ((int64_t)j + (int64_t)ly * k) -
+                                         (int64_t)c[axis] * stride[axis];
+                    double s = 0.0;
+                    for (int d = -R; d <= R; ++d) {
+                        int32_t p = (c[axis] + d) % len;
+                        if (p < 0) p += len;
+                        s = s + w[d + R] * in[base + (int64_t)p * stride[axis]];
+                    }
+                    out[base + (int64_t)c[axis] * stride[axis]] = s;
+                }
+    };
+    const int32_t nt = (int32_t)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    const int32_t per = (lz + nt - 1) / nt;
+    for (int32_t t = 0; t < nt; ++t) {
+        const int32_t k0 = t * per, k1 = std::min(lz, k0 + per);
+        if (k0 < k1) pool.emplace_back(work, k0, k1);
+    }
+    for (auto& th : pool) th.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbx_gaussian_field(int32_t lx, int32_t ly, int32_t lz, double sigma, uint64_t seed, double* field) {
+    if (lx <= 0 || ly <= 0 || lz <= 0 || !field || !(sigma > 0.0)) return LBX_ERR_ARG;
+    const int64_t N = (int64_t)lx * ly * lz;
+    // i.i.d. N(0,1) per node, x fastest, by Box-Muller pairs
+    lbx::SplitMix64 rng{seed};
+    std::vector<double> a((size_t)N), b((size_t)N);
+    for (int64_t r = 0; r < N; r += 2) {
+        const double u1 = 1.0 - rng.uniform();  // (0, 1]
+        const double u2 = rng.uniform();
+        const double m = std::sqrt(-2.0 * std::log(u1));
+        a[(size_t)r] = m * std::cos(2.0 * M_PI * u2);
+        if (r + 1 < N) a[(size_t)r + 1] = m * std::sin(2.0 * M_PI * u2);
+    }
+    // separable periodic convolution with exp(-d^2 / (2 sigma^2)), |d| <= R,
+    // R = min(ceil(4 sigma), (n-1)/2) per axis so that no image is counted twice
+    const int32_t n[3] = {lx, ly, lz};
+    for (int axis = 0; axis < 3; ++axis) {
+        const int R = (int)std::min<double>(std::ceil(4.0 * sigma), (double)((n[axis] - 1) / 2));
+        std::vector<double> w((size_t)(2 * R + 1));
+        for (int d = -R; d <= R; ++d) w[(size_t)(d + R)] = std::exp(-(double)d * d / (2.0 * sigma * sigma));
+        conv_axis(a, b, lx, ly, lz, axis, w, R);
+        a.swap(b);
+    }
+    std::memcpy(field, a.data(), (size_t)N * sizeof(double));
+    return LBX_OK;
+}
+
+int lbx_geometry_random(int32_t lx, int32_t ly, int32_t lz, double sigma, double porosity, uint64_t seed,
+                        int8_t* nature) {
+    if (!nature || !(porosity > 0.0) || !(porosity <= 1.0)) return LBX_ERR_ARG;
+    const int64_t N = (int64_t)lx * ly * lz;
+    if (lx <= 0 || ly <= 0 || lz <= 0) return LBX_ERR_ARG;
+    std::vector<double> f((size_t)N);
+    int st = lbx_gaussian_field(lx, ly, lz, sigma, seed, f.data());
+    if (st) return st;
+    // the round(porosity N) lowest values are fluid (ties: lower index first);
+    // every solid cluster is kept
+    const int64_t nfluid = std::llround(porosity * (double)N);
+    if (nfluid <= 0) return LBX_ERR_ARG;
+    // (value, index) pairs in lexicographic order: the nfluid-th smallest is the cut
+    std::vector<std::pair<double, int64_t>> v((size_t)N);
+    for (int64_t r = 0; r < N; ++r) v[(size_t)r] = {f[(size_t)r], r};
+    std::nth_element(v.begin(), v.begin() + (nfluid - 1), v.end());
+    const std::pair<double, int64_t> cut = v[(size_t)(nfluid - 1)];
+    for (int64_t r = 0; r < N; ++r) nature[r] = std::make_pair(f[(size_t)r], r) <= cut ? 0 : 1;
+    return LBX_OK;
+}
+
+int lbx_geometry_disks(int32_t lx, int32_t ly, int32_t lz, int32_t ndisks, double radius, uint64_t seed,
+                       int8_t* nature, double* centres) {
+    if (lx <= 0 || ly <= 0 || lz <= 0 || !nature || ndisks < 0 || !(radius > 0.0)) return LBX_ERR_ARG;
+    const int32_t n[3] = {lx, ly, lz};
+    lbx::SplitMix64 rng{seed};
+    std::vector<double> c((size_t)3 * ndisks);
+    // per-axis periodic minimum-image distance, tools/chromato_generator.f90:53-61
+    auto axis_d = [&](double p, double q, int a) {
+        double d = std::fabs(p - q);
+        return std::min(d, (double)n[a] - d);
+    };
+    const double dia = 2.0 * radius;
+    for (int32_t k = 0; k < ndisks; ++k) {
+        for (int tries = 0;; ++tries) {
+            if (tries > 100000) return LBX_ERR_ARG;  // no room for another disk
+            // centre uniform in [1, n+1) per axis (a flat axis of length 1 at 1),
+            // chromato_generator.f90:20-24
+            for (int a = 0; a < 3; ++a) c[(size_t)(3 * k + a)] = n[a] == 1 ? 1.0 : rng.uniform() * n[a] + 1.0;
+            bool ok = true;
+            for (int32_t m = 0; m < k && ok; ++m) {
+                const double dx = axis_d(c[(size_t)3 * k], c[(size_t)3 * m], 0);
+                const double dy = axis_d(c[(size_t)3 * k + 1], c[(size_t)3 * m + 1], 1);
+                const double dz = axis_d(c[(size_t)3 * k + 2], c[(size_t)3 * m + 2], 2);
+                ok = std::sqrt(dx * dx + dy * dy + dz * dz) > dia;  // disks neither overlap nor touch
+            }
+            if (ok) break;
+        }
+    }
+    // solid where the distance to a centre is <= radius, chromato_generator.f90:40-64
+    // (1-based node coordinates)
+    for (int32_t k = 0; k < lz; ++k)
+        for (int32_t j = 0; j < ly; ++j)
+            for (int32_t i = 0; i < lx; ++i) {
+                int8_t v = 0;
+                for (int32_t m = 0; m < ndisks && !v; ++m) {
+                    const double dx = axis_d((double)(i + 1), c[(size_t)3 * m], 0);
+                    const double dy = axis_d((double)(j + 1), c[(size_t)3 * m + 1], 1);
+                    const double dz = axis_d((double)(k + 1), c[(size_t)3 * m + 2], 2);
+                    v = std::sqrt(dx * dx + dy * dy + dz * dz) <= radius;
+                }
+                nature[(int64_t)i + (int64_t)lx * ((int64_t)j + (int64_t)ly * k)] = v;
+            }
+    if (centres) std::memcpy(centres, c.data(), c.size() * sizeof(double));
+    return LBX_OK;
+}
+
+}  // extern "C"

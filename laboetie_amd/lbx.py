@@ -32,6 +32,9 @@ SIGNATURES = {
     "lbx_d3q19_tables": (ctypes.c_int, [_p] * 5),
     "lbx_build_geometry": (ctypes.c_int, [_i32] * 4 + [_p, _i64, _p]),
     "lbx_detect_interfacial": (ctypes.c_int, [_i32] * 3 + [_p, _p]),
+    "lbx_gaussian_field": (ctypes.c_int, [_i32] * 3 + [_f64, ctypes.c_uint64, _p]),
+    "lbx_geometry_random": (ctypes.c_int, [_i32] * 3 + [_f64, _f64, ctypes.c_uint64, _p]),
+    "lbx_geometry_disks": (ctypes.c_int, [_i32] * 4 + [_f64, ctypes.c_uint64, _p, _p]),
     "lbx_footprint_bytes": (_i64, [_i32] * 5),
     "lbx_create": (ctypes.c_int, [_i32] * 3 + [_p, _f64, _i32, _p]),
     "lbx_destroy": (ctypes.c_int, [_p]),
@@ -113,6 +116,31 @@ def detect_interfacial(lx, ly, lz, nature):
     _check(lib().lbx_detect_interfacial(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)), _ptr(itf)),
            "detect_interfacial")
     return itf
+
+
+def gaussian_field(lx, ly, lz, sigma, seed):
+    """Periodic Gaussian-filtered white noise (lbx_gaussian_field), (lx,ly,lz) x fastest."""
+    f = np.zeros(lx * ly * lz)
+    _check(lib().lbx_gaussian_field(lx, ly, lz, float(sigma), int(seed), _ptr(f)), "gaussian_field")
+    return f
+
+
+def geometry_random(lx, ly, lz, sigma=8.0, porosity=0.6, seed=20261202):
+    """Gaussian random porous medium with exactly round(porosity*N) fluid nodes."""
+    nat = np.zeros(lx * ly * lz, np.int8)
+    _check(lib().lbx_geometry_random(lx, ly, lz, float(sigma), float(porosity), int(seed), _ptr(nat)),
+           "geometry_random")
+    return nat
+
+
+def geometry_disks(lx, ly, lz, ndisks, radius=5.0, seed=1):
+    """Seeded non-overlapping disks (spheres) of the chromatography inputs;
+    returns (nature, centres (ndisks, 3), 1-based)."""
+    nat = np.zeros(lx * ly * lz, np.int8)
+    c = np.zeros(3 * max(ndisks, 1))
+    _check(lib().lbx_geometry_disks(lx, ly, lz, int(ndisks), float(radius), int(seed), _ptr(nat), _ptr(c)),
+           "geometry_disks")
+    return nat, c[:3 * ndisks].reshape(-1, 3)
 
 
 def footprint_bytes(lx, ly, lz, precision=PREC_FP64, with_mp=1):
