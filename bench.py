@@ -16,7 +16,10 @@ also carries the roofline of the dominant kernel (algorithmic HBM bytes per
 launch / measured mean launch duration, HIP events on the library's stream)
 and the reference program timed on this machine's host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-fp32-shift]
+
+The same job also runs in the FP32-shift precision form; its throughput,
+rooflines and D_eff error against the FP64 run go under "fp32_shift".
 """
 import argparse
 import json
@@ -38,6 +41,8 @@ LB_BYTES_PER_FLUID = 19 * 8 * 2                 # read + write of 19 FP64 popula
 MP_COEF_BYTES_PER_FLUID = 19 * 8 + 8            # coefficient source (n, rho), shared by the tracers of a batch
 MP_P_BYTES_PER_FLUID = 2 * 3 * 8                # P read + write, per tracer
 MP_BYTES_PER_ADS_INTERFACIAL = 2 * 3 * 8        # Pa read + write, per adsorbing tracer
+LB32_BYTES_PER_FLUID = 19 * 4 * 2               # FP32-shift: read + write of 19 FP32 deviations
+MP32_COEF_BYTES_PER_FLUID = 19 * 4 + 4          # FP32-shift: coefficient source in FP32
 
 WORKLOAD = dict(label=3, dims=(128, 128, 128), tau=1.0, rho0=1.0, fext=(0.0, 0.0, 1e-6), target=1e-10,
                 Db=0.01, kd=0.01, dF=(0.0, -1.0, -2.0), mp_steps=10000)
@@ -73,6 +78,41 @@ def solve(L, w, batch=True):
             deffs.append(L.mp_deff().tolist())
     L.sync()
     return r["steps"], mp_steps, deffs
+
+
+def fp32_shift_leg(w, nat, args, nf, nfi):
+    """The same job in the FP32-shift precision form (populations as FP32
+    deviations from a0*rho0, FP32 collision, FP32 MP probabilities; FP64
+    moments, P, VACF and D_eff), timed the same way."""
+    with lbx.Lattice(w["dims"], nat, w["tau"], precision=lbx.PREC_FP32_SHIFT) as L:
+        for _ in range(args.warmup):
+            solve(L, w, not args.mp_one_at_a_time)
+        L.timing(True)
+        L.sync()
+        t0 = time.perf_counter()
+        results = [solve(L, w, not args.mp_one_at_a_time) for _ in range(args.steps)]
+        L.sync()
+        dt = time.perf_counter() - t0
+        lb_ms, lb_n = L.kernel_time(lbx.KERNEL_LB)
+        mp_ms, mp_n = L.kernel_time(lbx.KERNEL_MP)
+    lx, ly, lz = w["dims"]
+    N = lx * ly * lz
+    lb_steps, mp_steps, deffs = results[-1]
+    T = 1 if args.mp_one_at_a_time else len(w["dF"])
+    lb_gbs = LB32_BYTES_PER_FLUID * nf / (lb_ms / lb_n / 1e3) / 1e9
+    mp_bytes = (MP32_COEF_BYTES_PER_FLUID * nf + MP_P_BYTES_PER_FLUID * nf * T
+                + MP_BYTES_PER_ADS_INTERFACIAL * nfi * T)
+    mp_gbs = mp_bytes / (mp_ms / mp_n / 1e3) / 1e9
+    print("bench: fp32-shift leg done", file=sys.stderr, flush=True)
+    return {"value": round(N * sum(r[0] + r[1] for r in results) / dt / 1e6, 2), "unit": "MLUPS",
+            "ms_per_step": round(dt / args.steps * 1e3, 3), "lb_steps": lb_steps, "mp_steps": mp_steps,
+            "roofline_lb": {"achieved": round(lb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(lb_gbs / HBM_PEAK_GBS, 4), "bytes_per_fluid_node": LB32_BYTES_PER_FLUID,
+                            "avg_launch_us": round(lb_ms / lb_n * 1e3, 2)},
+            "roofline_mp": {"achieved": round(mp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(mp_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": round(mp_bytes),
+                            "avg_launch_us": round(mp_ms / mp_n * 1e3, 2)},
+            "deff": deffs}
 
 
 def read_traffic():
@@ -207,6 +247,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3, help="timed full solves")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32-shift", action="store_true", help="skip the FP32-shift leg")
     ap.add_argument("--mp-one-at-a-time", action="store_true",
                     help="propagate the dF tracers one after the other instead of together")
     ap.add_argument("--copy-probe", action="store_true",
@@ -225,7 +266,7 @@ def main():
     nat = lbx.build_geometry(w["label"], lx, ly, lz)
     itf = lbx.detect_interfacial(lx, ly, lz, nat)
     nfi = int(((nat == 0) & (itf == 1)).sum())
-    with lbx.Lattice(w["dims"], nat, w["tau"]) as L:
+    with lbx.Lattice(w["dims"], nat, w["tau"], precision=lbx.PREC_FP64) as L:
         nf = L.fluid_nodes
         for _ in range(args.warmup):
             solve(L, w, not args.mp_one_at_a_time)
@@ -238,6 +279,7 @@ def main():
         dt = time.perf_counter() - t0
         lb_ms, lb_n = L.kernel_time(lbx.KERNEL_LB)
         mp_ms, mp_n = L.kernel_time(lbx.KERNEL_MP)
+    shift = None if args.no_fp32_shift else fp32_shift_leg(w, nat, args, nf, nfi)
     copy_gbs = lbx.copy_bandwidth(2 << 30, 20)  # measured ceiling, outside the timed region
     lb_steps, mp_steps, deffs = results[-1]
     updates = N * sum(r[0] + r[1] for r in results)
@@ -281,6 +323,12 @@ def main():
         "mp_tracers_together": per_launch,
         "deff": deffs,
     }
+    if shift is not None:
+        d64 = np.array(deffs)
+        d32 = np.array(shift.pop("deff"))
+        shift["deff_rel_error_vs_fp64"] = float(f"{np.abs(d32 - d64).max() / np.abs(d64).max():.3e}")
+        shift["vs_fp64_value"] = round(shift["value"] / value, 3)
+        line["fp32_shift"] = shift
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, lb_steps, mp_steps)
     print(json.dumps(line), flush=True)
