@@ -238,6 +238,68 @@ int lbx_copy_bandwidth(int64_t bytes, int32_t iters, double* gb_per_s);
 /* Block until the context's stream is idle. */
 int lbx_sync(lbx_ctx* ctx);
 
+/* ----------------------------------------------------------- batch engine
+ * Many independent lattices ("instances", each an lb.in-style solve: rest
+ * start, two-stage equilibration, then moment propagation of one tracer)
+ * resident in HBM at once, advanced together: one LB launch steps every
+ * instance in its LB phase, one MP launch every instance in its MP phase,
+ * over one packed fluid-node list with per-instance offsets (lbx_batch.hip).
+ * Each instance's results are bit-identical to the single-lattice path with
+ * the same input (LB fields, P; VACF with 256-node blocks).  No counterpart
+ * in the reference (it solves one lattice per process); it serves batched
+ * and continuously batched workloads.  A batch is used by one thread. */
+typedef struct lbxb lbxb;
+
+/* HBM bytes of a batch of this capacity (nodes of the packed list, full-box
+ * map entries, instance slots). */
+int64_t lbxb_footprint_bytes(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
+                             int32_t max_instances);
+int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box, int32_t max_instances,
+                lbxb** out);
+int lbxb_destroy(lbxb* b);
+
+/* Admit one instance (HBM admission: LBX_ERR_NOMEM when its fluid nodes,
+ * full-box map or instance slot do not fit now).  It starts from rest at
+ * rho0; equilibrates with the reference's two-stage loop to target_error
+ * (max_lb_steps > 0 caps it), then runs mp_steps moment-propagation steps of
+ * a tracer (Db, ka, kd); *id receives its slot. */
+int lbxb_add(lbxb* b, int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double rho0,
+             const double fext[3], double target_error, int64_t max_lb_steps, double Db, double ka, double kd,
+             int64_t mp_steps, int32_t* id);
+/* Free an instance's slot and nodes (any phase). */
+int lbxb_remove(lbxb* b, int32_t id);
+
+/* One round: up to lb_steps LB launches and mp_steps MP launches
+ * (each <= 1024), then phase changes (stage switch, end of LB -> MP set-up,
+ * end of MP).  *active (may be NULL) = instances not yet done or failed. */
+int lbxb_round(lbxb* b, int32_t lb_steps, int32_t mp_steps, int32_t* active);
+
+/* Instance state: phase 1 LB, 2 LB stage switch pending, 3 LB done, 4 MP,
+ * 5 done, 6 failed (status = the lbx status); LB steps taken and exit reason,
+ * MP steps done. */
+#define LBXB_PHASE_LB     1
+#define LBXB_PHASE_DONE   5
+#define LBXB_PHASE_FAILED 6
+int lbxb_info(lbxb* b, int32_t id, int32_t* phase, int32_t* status, int64_t* lb_steps, int32_t* exit_reason,
+              int64_t* mp_steps_done);
+/* D_eff, the VACF trace (first vacf_rows rows of 3, row 0 = VACF(0)) and the
+ * final flow (full box (lx,ly,lz), 0 at solids); any pointer may be NULL. */
+int lbxb_result(lbxb* b, int32_t id, double deff[3], double* vacf, int64_t vacf_rows, double* rho, double* jx,
+                double* jy, double* jz);
+
+/* Block size (128 or 256 nodes) of the LB and MP launches; the autotuner
+ * times both on the current state (without changing it) and keeps the
+ * faster; times[4] = mean ms per launch of LB-128, LB-256, MP-128, MP-256. */
+int lbxb_set_block(lbxb* b, int32_t lb_block, int32_t mp_block);
+int lbxb_autotune(lbxb* b, int32_t launches, int32_t* lb_block, int32_t* mp_block, double* times);
+/* Rounds replay captured HIP graphs of their launches (on by default). */
+int lbxb_set_graphs(lbxb* b, int32_t on);
+/* Device time (HIP events around each phase of each round) and work done. */
+int lbxb_stats(lbxb* b, double* lb_ms, int64_t* lb_box_updates, int64_t* lb_fluid_updates, double* mp_ms,
+               int64_t* mp_box_updates, int64_t* mp_fluid_updates);
+/* Largest free run of 256-node tiles (admission headroom). */
+int64_t lbxb_free_tiles(lbxb* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,714 +31,13 @@
 
 #include "../../include/lbx.h"
 #include "lbx_device.h"
+#include "lbx_kernels.h"
 #include "lbx_tables.h"
 
 #pragma clang fp contract(off)
 
 using namespace lbx;
 
-namespace {
-
-constexpr int kBlock = 256;          // one tile = 256 consecutive packed nodes
-constexpr int kSlots = 64;           // striped l2err slots per step
-constexpr int kRing = 1024;          // l2err ring slots (>= 2 chunks)
-constexpr int kChunk = 256;          // LB steps launched between host checks
-constexpr int kMpChunk = 512;        // MP steps per host check
-constexpr uint32_t kSolidSlot = 0xFFFFFFFFu;
-constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
-constexpr uint32_t kNodeInterfacial = 0x80000000u;
-
-// Device-side control block.  Every field has a single writer per launch
-// (block 0 of a step kernel, or the one-block reduction kernel), and is read
-// by later launches on the same stream, so plain stores suffice.
-struct Ctrl {
-    int32_t stop;                // LB: a step kernel saw the previous step converge
-    int32_t restpart;            // MP: a rest probability < eps (set by mp prep)
-    int64_t exit_step;           // LB: that converged step
-    int64_t negpop;              // LB: first step with a negative population (-1: none)
-    int32_t mp_stop;             // MP: VACF convergence reached
-    int32_t pad;
-    int64_t mp_conv_step;
-    int32_t floor_hit;           // LB (FP32-shift): the stop at exit_step was the l2err floor test
-    int32_t pad2;
-    double l2min[2];             // LB floor test: running minimum of l2err of the stage up to step s,
-    int64_t l2min_step[2];       // and the step that set it, in slot s & 1
-    double vacf0[12];            // 3 per tracer (at most 4)
-    double deff_s[12], deff_c[12]; // Neumaier running sums of the VACF integrals
-};
-
-__device__ __forceinline__ int32_t vload(const int32_t* p) { return *(const volatile int32_t*)p; }
-__device__ __forceinline__ int64_t vload(const int64_t* p) { return *(const volatile int64_t*)p; }
-__device__ __forceinline__ double vload(const double* p) { return *(const volatile double*)p; }
-
-// packed indices of the 18 neighbours r + c_l (l = 1..18) of full-box node r
-struct Neighbours {
-    uint32_t q[kQ];
-    __device__ __forceinline__ void load(const uint32_t* __restrict__ map, uint32_t r, const FastDiv& dlx,
-                                         const FastDiv& dly, uint32_t lz) {
-        Nbr nb;
-        nb.init(r, dlx, dly, lz);
-        static_for<1, kQ>([&](auto I) {
-            constexpr int L = I;
-            q[L] = ld(map, nb.at<kCX[L], kCY[L], kCZ[L]>());
-        });
-    }
-};
-
-// ---------------------------------------------------------------- LB kernels
-
-// Populations, flux and step constants are of type E: double for the FP64
-// form, float for the FP32-shift form (populations stored as deviations
-// g = n - a0 rho0 from the rest state).
-template <typename E>
-struct StepArgs {
-    const E* __restrict__ fin;
-    E* __restrict__ fout;
-    E* __restrict__ j;
-    const uint32_t* __restrict__ nodes;
-    const uint32_t* __restrict__ map;
-    unsigned long long* slots;            // [3][kSlots] striped per-step max|dj| bit patterns
-    double* l2ring;
-    Ctrl* ctrl;
-    FastDiv dlx, dly;
-    uint32_t lz, Nf;
-    uint64_t Np;
-    E A, B, C;
-    E fmx, fmy, fmz;   // F(t)/2, the force of step t in j(t) (equilibration.f90:293-295)
-    E fcx, fcy, fcz;   // F(t+1), the force of the collision that opens step t+1
-    E rho0;            // FP32-shift: the rest density of the shift
-    int64_t t;
-    int64_t check_after;    // stop check on l2err(t-1) only when t-1 > check_after
-    int64_t floor_window;   // > 0: also stop when l2err set no new minimum for this many steps
-    double target;
-    int32_t write_l2;       // 0 for a redo of a step whose l2err is already known
-    int32_t have_prev;      // step t-1 ran in this run: reduce its l2err first
-};
-
-// A negative population among a block's nodes is reported as +inf in place of
-// its max|dj| (max over bit patterns of non-negative doubles = max of values).
-constexpr unsigned long long kNegMark = 0x7FF0000000000000ull;
-
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    return v;
-}
-
-// max over the kSlots slots of one step (called by the first wave of a block)
-__device__ __forceinline__ unsigned long long slots_max(const unsigned long long* set) {
-    return wave_max_u64(*(const volatile unsigned long long*)&set[threadIdx.x & (kSlots - 1)]);
-}
-
-// Pull-streaming with half-way bounce-back (equilibration.f90:204-243): the
-// population arriving at node i in direction L left r - c_L, or, when that
-// node is solid, left i itself in direction inv(L).
-// The two candidate sources differ only in the lane's element index, so one
-// load per direction serves both.
-template <int L, bool Wide, typename E>
-__device__ __forceinline__ E pull(const View<Wide, E>& fin, uint32_t Np, uint32_t i, const Neighbours& nb) {
-    if constexpr (L == 0) {
-        return fin.ld(i);
-    } else {
-        const uint32_t q = nb.q[kInv[L]];  // r - c_L = r + c_inv(L)
-        return fin.ld(q != kSolidSlot ? (uint32_t)L * Np + q : (uint32_t)kInv[L] * Np + i);
-    }
-}
-
-// One time step, fused: pull-streaming with bounce-back of the post-collision
-// populations of step t, the negativity check (equilibration.f90:248),
-// density (:254) and flux (:293-300), the l2err contribution (:339-344), then
-// the collision that opens step t+1 (module_collision.f90:87-108).
-// E = float: the FP32-shift form; the l2err is still a max of FP64 values.
-#ifndef LBX_LB_WAVES
-#define LBX_LB_WAVES 1
-#endif
-template <bool Wide, typename E>
-__global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs<E> a) {
-    constexpr bool kShift = std::is_same<E, float>::value;
-    if (vload(&a.ctrl->stop) || vload(&a.ctrl->negpop) >= 0) return;
-    if (a.have_prev) {
-        // l2err of step t-1 (equilibration.f90:339-344) from its striped
-        // slots, then the negativity (:248) and convergence (:346) verdicts
-        __shared__ unsigned long long prev_bits;
-        if (threadIdx.x < 64) {
-            const unsigned long long v = slots_max(a.slots + ((a.t - 1) % 3) * kSlots);
-            if (threadIdx.x == 0) prev_bits = v;
-        }
-        __syncthreads();
-        const bool first = blockIdx.x == 0 && threadIdx.x == 0;
-        if (prev_bits == kNegMark) {
-            if (first) a.ctrl->negpop = a.t - 1;
-            return;
-        }
-        const double prev = __longlong_as_double((long long)prev_bits);
-        if (first) a.l2ring[(a.t - 1) % kRing] = prev;
-        bool floor = false;
-        if (a.floor_window > 0 && a.t - 1 > a.check_after) {
-            // record of the stage up to step t-1 from the one up to t-2 (every
-            // block derives the same record; block 0 stores it for step t+1)
-            const int s0 = (int)((a.t - 2) & 1), s1 = (int)((a.t - 1) & 1);
-            double m = vload(&a.ctrl->l2min[s0]);
-            int64_t ms = vload(&a.ctrl->l2min_step[s0]);
-            if (prev < m) {
-                m = prev;
-                ms = a.t - 1;
-            }
-            if (first) {
-                a.ctrl->l2min[s1] = m;
-                a.ctrl->l2min_step[s1] = ms;
-            }
-            floor = (a.t - 1) - ms >= a.floor_window;
-        }
-        if (a.t - 1 > a.check_after && (prev <= a.target || floor)) {
-            if (first) {
-                a.ctrl->exit_step = a.t - 1;
-                a.ctrl->floor_hit = prev <= a.target ? 0 : 1;
-                a.ctrl->stop = 1;
-            }
-            return;
-        }
-    }
-    // clear the slots of step t+1 (last used by step t-2, read by step t-1)
-    if (a.write_l2 && blockIdx.x == 0 && threadIdx.x < kSlots) a.slots[((a.t + 1) % 3) * kSlots + threadIdx.x] = 0ull;
-    const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
-    double l2 = 0.0;
-    bool neg = false;
-    if (i < a.Nf) {
-        const uint32_t Np = (uint32_t)a.Np;
-        const View<Wide, E> fin(a.fin, kQ * a.Np), fout(a.fout, kQ * a.Np), jv(a.j, 3 * a.Np);
-        Neighbours nb;
-        nb.load(a.map, ld(a.nodes, i) & kIndexMask, a.dlx, a.dly, a.lz);
-        E jo[3] = {E(0), E(0), E(0)};
-        if (a.write_l2) {  // j_old, issued before the populations so its latency overlaps theirs
-            jo[0] = jv.ld(i);
-            jo[1] = jv.ld(Np + i);
-            jo[2] = jv.ld(2 * Np + i);
-        }
-        E f[kQ];
-        static_for<0, kQ>([&](auto I) { f[I] = pull<I, Wide, E>(fin, Np, i, nb); });
-        if constexpr (kShift)  // n < 0  <=>  g < -a0 rho0
-            static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < -((float)a0(I) * a.rho0)); });
-        else
-            static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < 0.0); });
-        E rho = f[0];  // FP32-shift: drho = rho - rho0
-        static_for<1, kQ>([&](auto I) { rho = rho + f[I]; });
-        E jx = a.fmx, jy = a.fmy, jz = a.fmz;
-        static_for<0, kQ>([&](auto I) {
-            constexpr int L = I;
-            if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(f[L]);
-            if constexpr (kCY[L] != 0) jy = jy + sgn<kCY[L]>(f[L]);
-            if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(f[L]);
-        });
-        if (a.write_l2) {
-            const E dx = fabs(jx - jo[0]), dy = fabs(jy - jo[1]), dz = fabs(jz - jo[2]);
-            l2 = (double)fmax(fmax(dx, dy), dz);
-        }
-        jv.st(i, jx);
-        jv.st(Np + i, jy);
-        jv.st(2 * Np + i, jz);
-        if constexpr (kShift) {
-            const float drho = rho, full = a.rho0 + drho;
-            const float ux = jx / full, uy = jy / full, uz = jz / full;
-            static_for<0, kQ>([&](auto I) {
-                constexpr int L = I;
-                fout.st((uint32_t)L * Np + i, collide_shift<L>(f[L], drho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy,
-                                                               a.fcz, a.A, a.B, a.C));
-            });
-        } else {
-            const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
-            static_for<0, kQ>([&](auto I) {
-                constexpr int L = I;
-                fout.st((uint32_t)L * Np + i,
-                        collide_pop<L>(f[L], rho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy, a.fcz, a.A, a.B, a.C));
-            });
-        }
-    }
-    if (a.write_l2) {
-        __shared__ double lds[kBlock / 64];
-        l2 = wave_max(l2);
-        const bool wneg = __ballot(neg) != 0ull;
-        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = wneg ? -1.0 : l2;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double m = lds[0];
-            bool bneg = m < 0.0;
-            for (int w = 1; w < kBlock / 64; ++w) {
-                bneg = bneg || lds[w] < 0.0;
-                m = fmax(m, lds[w]);
-            }
-            if (bneg || m > 0.0)
-                atomicMax(&a.slots[(a.t % 3) * kSlots + (blockIdx.x & (kSlots - 1))],
-                          bneg ? kNegMark : (unsigned long long)__double_as_longlong(m));
-        }
-    }
-}
-
-// l2err of the last step of a launch run, which no later step kernel reads.
-__global__ __launch_bounds__(64) void k_lb_finalize(const unsigned long long* slots, double* l2ring, Ctrl* ctrl,
-                                                    int64_t t) {
-    if (vload(&ctrl->stop) || vload(&ctrl->negpop) >= 0) return;
-    const unsigned long long v = slots_max(slots + (t % 3) * kSlots);
-    if (threadIdx.x == 0) {
-        if (v == kNegMark)
-            ctrl->negpop = t;
-        else
-            l2ring[t % kRing] = __longlong_as_double((long long)v);
-    }
-}
-
-// Streaming only: the post-streaming state f_s(t) of the step whose input is
-// fin, and its density SUM(n,4) (equilibration.f90:254).
-__global__ __launch_bounds__(kBlock) void k_extract(const double* __restrict__ fin, double* __restrict__ nout,
-                                                    double* __restrict__ rho_out, const uint32_t* __restrict__ nodes,
-                                                    const uint32_t* __restrict__ map, FastDiv dlx, FastDiv dly,
-                                                    uint32_t lz, uint32_t Nf, uint64_t Np) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= Nf) return;
-    Neighbours nb;
-    nb.load(map, nodes[i] & kIndexMask, dlx, dly, lz);
-    const View<true> fv(fin, kQ * Np);
-    double rho = 0.0;
-    static_for<0, kQ>([&](auto I) {
-        constexpr int L = I;
-        const double v = pull<L, true>(fv, (uint32_t)Np, i, nb);
-        nout[(uint64_t)L * Np + i] = v;
-        rho = (L == 0) ? v : rho + v;
-    });
-    rho_out[i] = rho;
-}
-
-// The same for the FP32-shift form: the post-streaming deviations g into
-// gout, and the FP64 density rho0 + sum g and flux F/2 + sum g c, summed in
-// FP64 from the FP32 deviations (the flow that moment propagation reads).
-__global__ __launch_bounds__(kBlock) void k_extract_shift(const float* __restrict__ gin, float* __restrict__ gout,
-                                                          double* __restrict__ rho_out, double* __restrict__ j_out,
-                                                          const uint32_t* __restrict__ nodes,
-                                                          const uint32_t* __restrict__ map, FastDiv dlx, FastDiv dly,
-                                                          uint32_t lz, uint32_t Nf, uint64_t Np, double rho0,
-                                                          double fmx, double fmy, double fmz) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= Nf) return;
-    Neighbours nb;
-    nb.load(map, nodes[i] & kIndexMask, dlx, dly, lz);
-    const View<true, float> gv(gin, kQ * Np);
-    double drho = 0.0, jx = fmx, jy = fmy, jz = fmz;
-    static_for<0, kQ>([&](auto I) {
-        constexpr int L = I;
-        const float v = pull<L, true>(gv, (uint32_t)Np, i, nb);
-        gout[(uint64_t)L * Np + i] = v;
-        const double d = v;
-        drho = (L == 0) ? d : drho + d;
-        if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(d);
-        if constexpr (kCY[L] != 0) jy = jy + sgn<kCY[L]>(d);
-        if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(d);
-    });
-    rho_out[i] = rho0 + drho;
-    j_out[i] = jx;
-    j_out[Np + i] = jy;
-    j_out[2 * Np + i] = jz;
-}
-
-// The collision on packed fields with a uniform force (the opening collision
-// of a run), in place: module_collision.f90:87-108.
-__global__ __launch_bounds__(kBlock) void k_collide_packed(double* f, const double* __restrict__ rho,
-                                                           const double* __restrict__ j, double fx, double fy,
-                                                           double fz, uint32_t Nf, uint64_t Np, double A, double B,
-                                                           double C) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= Nf) return;
-    const double d = rho[i], jx = j[i], jy = j[Np + i], jz = j[2 * Np + i];
-    const double ux = jx / d, uy = jy / d, uz = jz / d;
-    static_for<0, kQ>([&](auto I) {
-        constexpr int L = I;
-        const double v = f[(uint64_t)L * Np + i];
-        f[(uint64_t)L * Np + i] = collide_pop<L>(v, d, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
-    });
-}
-
-// The same for the FP32-shift form: the FP64 density and flux of the state
-// become the FP32 drho = rho - rho0 and j of the collision (and j_old of the
-// first step).
-__global__ __launch_bounds__(kBlock) void k_collide_packed_shift(float* g, const double* __restrict__ rho,
-                                                                 const double* __restrict__ j,
-                                                                 float* __restrict__ j32, float fx, float fy,
-                                                                 float fz, uint32_t Nf, uint64_t Np, float A,
-                                                                 float B, float C, double rho0) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= Nf) return;
-    const float drho = (float)(rho[i] - rho0), full = (float)rho0 + drho;
-    const float jx = (float)j[i], jy = (float)j[Np + i], jz = (float)j[2 * Np + i];
-    j32[i] = jx;  // j_old of the first step
-    j32[Np + i] = jy;
-    j32[2 * Np + i] = jz;
-    const float ux = jx / full, uy = jy / full, uz = jz / full;
-    static_for<0, kQ>([&](auto I) {
-        constexpr int L = I;
-        const float v = g[(uint64_t)L * Np + i];
-        g[(uint64_t)L * Np + i] = collide_shift<L>(v, drho, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
-    });
-}
-
-// collide() on whole full-box fields with per-node forces, in place
-// (module_collision.f90:15-130, fluid nodes only): the stateless entry point.
-__global__ __launch_bounds__(kBlock) void k_collide_box(double* f, const double* __restrict__ rho,
-                                                        const double* __restrict__ j,
-                                                        const double* __restrict__ F,
-                                                        const int8_t* __restrict__ nature, uint32_t N, double A,
-                                                        double B, double C) {
-    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r >= N || nature[r] != 0) return;
-    const uint64_t n = N;
-    const double d = rho[r], jx = j[r], jy = j[n + r], jz = j[2 * n + r];
-    const double fx = F[r], fy = F[n + r], fz = F[2 * n + r];
-    const double ux = jx / d, uy = jy / d, uz = jz / d;
-    static_for<0, kQ>([&](auto I) {
-        constexpr int L = I;
-        const double v = f[(uint64_t)L * n + r];
-        f[(uint64_t)L * n + r] = collide_pop<L>(v, d, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
-    });
-}
-
-// Rest state, init_simu.f90:31-39: n = rho0*a0 at fluid nodes, j = 0; in the
-// FP32-shift form (g != NULL) the deviations g = 0 and the FP32 flux j32 = 0.
-__global__ __launch_bounds__(kBlock) void k_init_rest(double* __restrict__ f, float* __restrict__ g,
-                                                      float* __restrict__ j32, double* __restrict__ rho,
-                                                      double* __restrict__ j, uint32_t Nf, uint64_t Np, double rho0) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= Nf) return;
-    static_for<0, kQ>([&](auto I) {
-        constexpr int L = I;
-        if (f) f[(uint64_t)L * Np + i] = rho0 * a0(L);
-        if (g) g[(uint64_t)L * Np + i] = 0.0f;
-    });
-    rho[i] = rho0;
-    for (int c = 0; c < 3; ++c) {
-        j[c * Np + i] = 0.0;
-        if (j32) j32[c * Np + i] = 0.0f;
-    }
-}
-
-// ---------------------------------------------------------------- MP kernels
-//
-// Several tracers that share the flow and Db (so the scattering
-// probabilities win, rem and u*) but differ in ka, kd can be propagated in one
-// pass: the shared coefficients are read once per node for all of them.  The
-// arithmetic of each tracer is exactly that of a single-tracer run.
-
-constexpr int kMaxTracers = 4;
-
-// Per-block sums of NV values in a fixed order, written by thread 0 to
-// part[NV*blockIdx.x ...].
-template <int NV>
-__device__ __forceinline__ void block_partial(const double (&v)[NV], double* part, double* lds /* [4*NV] */) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int a = 0; a < NV; ++a) {
-        const double s = wave_sum(v[a]);
-        if (lane == 0) lds[w * NV + a] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int a = 0; a < NV; ++a) {
-            double s = lds[a];
-            for (int k = 1; k < kBlock / 64; ++k) s = s + lds[k * NV + a];
-            part[NV * blockIdx.x + a] = s;
-        }
-    }
-    __syncthreads();
-}
-
-// VACF of one moment-propagation step (NV = 3 per tracer) from the per-block
-// partial sums, in a fixed order that does not depend on which kernel runs it
-// (so every run gives the same bits), then the running D_eff sums and,
-// optionally, the convergence test of module_moment_propagation.f90:284-288
-// (all NV components below both limits).  Called by all kBlock threads of one
-// block.  mode 0: VACF(0) of mp init; mode 1: VACF(it) of a step.
-template <int NV>
-__device__ __forceinline__ void mp_reduce_block(const double* part, uint32_t nparts, double* vtrace, Ctrl* ctrl,
-                                                int32_t mode, int64_t it, int32_t check_conv, double lim1,
-                                                double lim2, double* lds /* [4*NV] */) {
-    double s[NV];
-#pragma unroll
-    for (int a = 0; a < NV; ++a) s[a] = 0.0;
-    for (uint32_t k = threadIdx.x; k < nparts; k += kBlock)
-#pragma unroll
-        for (int a = 0; a < NV; ++a) {
-            const double x = *(const volatile double*)&part[(uint64_t)NV * k + a];
-            s[a] = (k == threadIdx.x) ? x : s[a] + x;
-        }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int a = 0; a < NV; ++a) {
-        const double t = wave_sum(s[a]);
-        if (lane == 0) lds[w * NV + a] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int a = 0; a < NV; ++a) {
-            double t = lds[a];
-            for (int k = 1; k < kBlock / 64; ++k) t = t + lds[k * NV + a];
-            s[a] = t;
-        }
-        if (mode == 0) {
-#pragma unroll
-            for (int a = 0; a < NV; ++a) {
-                ctrl->vacf0[a] = s[a];
-                ctrl->deff_s[a] = 0.5 * s[a];
-                ctrl->deff_c[a] = 0.0;
-            }
-        } else {
-#pragma unroll
-            for (int a = 0; a < NV; ++a) {
-                vtrace[a] = s[a];
-                // Neumaier update of D = VACF(0)/2 + sum VACF(t)
-                const double d = ctrl->deff_s[a], x = s[a], u = d + x;
-                ctrl->deff_c[a] += (fabs(d) >= fabs(x)) ? (d - u) + x : (x - u) + d;
-                ctrl->deff_s[a] = u;
-            }
-            if (check_conv && it > 2) {
-                bool conv = true;
-#pragma unroll
-                for (int a = 0; a < NV; ++a) conv = conv && fabs(s[a]) < lim1 && fabs(s[a]) < lim2;
-                if (conv) {
-                    ctrl->mp_conv_step = it;
-                    ctrl->mp_stop = 1;
-                }
-            }
-        }
-    }
-    __syncthreads();
-}
-
-template <int NV>
-__global__ __launch_bounds__(kBlock) void k_mp_reduce(const double* __restrict__ part, uint32_t nparts,
-                                                      double* vtrace, Ctrl* ctrl, int32_t mode, int64_t it,
-                                                      int32_t check_conv, double lim1, double lim2) {
-    __shared__ double lds[kBlock / 64 * NV];
-    if (mode == 1 && vload(&ctrl->mp_stop)) return;
-    mp_reduce_block<NV>(part, nparts, vtrace, ctrl, mode, it, check_conv, lim1, lim2, lds);
-}
-
-struct TracerParams {
-    double bw[kMaxTracers];    // 1/Pstat (module_moment_propagation.f90:97,108)
-    double ka[kMaxTracers], kd[kMaxTracers], one_minus_kd[kMaxTracers];
-    int32_t ads[kMaxTracers];  // adsorption on (K = ka/kd > eps, :46-56)
-};
-
-// W is the type the scattering probabilities win and u* are stored in:
-// double, or float in the FP32-shift form.  There each probability is rounded
-// to FP32 once, and every quantity built from it (the rest probability
-// rem = 1 - sum s, which stays FP64, u*, P(r,1), VACF(0)) uses the rounded
-// value, so the propagator still conserves sum_r P exactly up to FP64 rounding.
-template <typename W>
-__device__ __forceinline__ double as_coef(double v) {
-    return (double)(W)v;
-}
-
-template <typename W>
-struct MpPrepArgs {
-    const double* __restrict__ rho;
-    const double* __restrict__ j;
-    const uint32_t* __restrict__ nodes;
-    const uint32_t* __restrict__ map;
-    W* __restrict__ win;
-    double* __restrict__ rem;
-    W* __restrict__ ustar;
-    double* __restrict__ P0;   // [T][3][Np]
-    double* partials;
-    Ctrl* ctrl;
-    FastDiv dlx, dly;
-    uint32_t lz, Nf;
-    uint64_t Np;
-    double fx, fy, fz;   // lb.in f_ext (drop_tracers.f90:92)
-    double lambda, eps;
-    TracerParams tp;
-};
-
-// update_tracer_population (drop_tracers.f90:97-100) fused with
-// moment_propagation::init (module_moment_propagation.f90:105-137) and the
-// static part of propagate (:218-249): per fluid node, the scattering
-// probabilities s(r,l) = n_l/rho - w_l + lambda w_l / 2 towards fluid
-// neighbours, the rest probability rem, u* = sum s c, the incoming
-// probabilities win(r,l) = s(r+c_l, inv l), and per tracer P(r,1) and the
-// VACF(0) contribution.
-template <int T, typename W>
-__global__ __launch_bounds__(kBlock) void k_mp_prep(MpPrepArgs<W> a) {
-    __shared__ double lds[kBlock / 64 * 3 * T];
-    const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
-    double vp[3 * T];
-#pragma unroll
-    for (int k = 0; k < 3 * T; ++k) vp[k] = 0.0;
-    if (i < a.Nf) {
-        const uint32_t e = a.nodes[i];
-        Neighbours nb;
-        nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
-        const double rr = a.rho[i];
-        const double vx = a.j[i] + a.fx, vy = a.j[a.Np + i] + a.fy, vz = a.j[2 * a.Np + i] + a.fz;
-        double us[3] = {0.0, 0.0, 0.0};
-        double p1[3 * T];
-#pragma unroll
-        for (int k = 0; k < 3 * T; ++k) p1[k] = 0.0;
-        double remv = 1.0;
-        static_for<1, kQ>([&](auto I) {
-            constexpr int L = I;
-            constexpr int Li = kInv[L];
-            constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
-            const uint32_t q = nb.q[L];
-            if (q != kSolidSlot) {
-                const double nt = a0(L) * rr + a1(L) * dot3<cx, cy, cz>(vx, vy, vz);
-                const double s = as_coef<W>(nt / rr - a0(L) + a.lambda * a0(L) * 0.5);
-                remv = remv - s;
-                if constexpr (cx != 0) us[0] = us[0] + sgn<cx>(s);
-                if constexpr (cy != 0) us[1] = us[1] + sgn<cy>(s);
-                if constexpr (cz != 0) us[2] = us[2] + sgn<cz>(s);
-#pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    if constexpr (cx != 0) vp[3 * t + 0] = vp[3 * t + 0] + a.tp.bw[t] * s;
-                    if constexpr (cy != 0) vp[3 * t + 1] = vp[3 * t + 1] + a.tp.bw[t] * s;
-                    if constexpr (cz != 0) vp[3 * t + 2] = vp[3 * t + 2] + a.tp.bw[t] * s;
-                }
-                const double rn = a.rho[q];
-                const double wx = a.j[q] + a.fx, wy = a.j[a.Np + q] + a.fy, wz = a.j[2 * a.Np + q] + a.fz;
-                const double ntn = a0(Li) * rn + a1(Li) * dot3<-cx, -cy, -cz>(wx, wy, wz);
-                const double sp = as_coef<W>(ntn / rn - a0(Li) + a.lambda * a0(Li) * 0.5);
-                a.win[comp_index(18, L - 1, i, a.Np)] = (W)sp;
-#pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    if constexpr (cx != 0) p1[3 * t + 0] = p1[3 * t + 0] + sgn<-cx>(sp) * a.tp.bw[t];
-                    if constexpr (cy != 0) p1[3 * t + 1] = p1[3 * t + 1] + sgn<-cy>(sp) * a.tp.bw[t];
-                    if constexpr (cz != 0) p1[3 * t + 2] = p1[3 * t + 2] + sgn<-cz>(sp) * a.tp.bw[t];
-                }
-            } else {
-                a.win[comp_index(18, L - 1, i, a.Np)] = W(0);
-            }
-        });
-        a.rem[i] = remv;
-        bool low = false;
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            const double rt = (a.tp.ads[t] && (e & kNodeInterfacial)) ? remv - a.tp.ka[t] : remv;
-            low = low || rt < a.eps;
-        }
-        if (low) a.ctrl->restpart = 1;
-        for (int c = 0; c < 3; ++c) a.ustar[comp_index(3, c, i, a.Np)] = (W)us[c];
-#pragma unroll
-        for (int k = 0; k < 3 * T; ++k) a.P0[comp_index(3 * T, k, i, a.Np)] = p1[k];
-    }
-    block_partial<3 * T>(vp, a.partials, lds);
-}
-
-template <typename W>
-struct MpStepArgs {
-    const W* __restrict__ win;
-    const double* __restrict__ rem;
-    const W* __restrict__ ustar;
-    const uint32_t* __restrict__ nodes;
-    const uint32_t* __restrict__ map;
-    const double* __restrict__ Pn;   // [T][3][Np]
-    double* __restrict__ Px;
-    const double* __restrict__ Pan;
-    double* __restrict__ Pax;
-    double* partials;                // this step's per-block VACF partials
-    const double* prev_partials;     // the previous step's, reduced by block 0 (NULL: none)
-    double* prev_vtrace;             // where VACF(it-1) goes
-    int64_t it;
-    Ctrl* ctrl;
-    FastDiv dlx, dly;
-    uint32_t lz, Nf;
-    uint64_t Np;
-    int32_t any_ads;                 // some tracer of the batch adsorbs (Pan, Pax allocated)
-    TracerParams tp;
-};
-
-// One moment-propagation step of T tracers, module_moment_propagation.f90:207-253:
-// P(r,t+1) = sum_l P(r+c_l,t) s(r+c_l, inv l) + rem(r) P(r,t) [+ kd Pa],
-// Pa(r,t+1) = (1-kd) Pa(r,t) + ka P(r,t) at adsorbing interfacial nodes,
-// VACF(t) = sum_r P(r,t) . u*(r).
-template <bool Wide, int T, typename W>
-__global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs<W> a) {
-    constexpr int K = 3 * T;  // moment components of the batch
-    __shared__ double lds[kBlock / 64 * K];
-    if (vload(&a.ctrl->mp_stop)) return;
-    if (a.prev_partials && blockIdx.x == 0)
-        mp_reduce_block<K>(a.prev_partials, gridDim.x, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
-    const uint32_t i = xcd_tile() * kBlock + threadIdx.x;
-    double vp[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) vp[k] = 0.0;
-    if (i < a.Nf) {
-        const uint32_t Np = (uint32_t)a.Np;
-        const View<Wide, W> win(a.win, 18 * a.Np), us(a.ustar, 3 * a.Np);
-        const View<Wide> Pn(a.Pn, K * a.Np), Px(a.Px, K * a.Np), rem(a.rem, a.Np);
-        const uint32_t e = ld(a.nodes, i);
-        Neighbours nb;
-        nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
-        double p[K], acc[K];
-        load_comps<K>(Pn, Np, i, p);
-#pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = 0.0;
-        W uw[3];
-        load_comps<3>(us, Np, i, uw);
-        const double rm = rem.ld(i);
-        Vec2<W> wp;  // the probabilities of directions L, L+1 (L odd) from one paired load
-        static_for<1, kQ>([&](auto I) {
-            constexpr int L = I;
-            if constexpr (L & 1) wp = win.ld2((uint32_t)((L - 1) / 2) * Np + i);
-            const uint32_t q = nb.q[L];
-            if (q != kSolidSlot) {
-                const double w = (double)wp[(L - 1) & 1];
-                double pq[K];
-                load_comps<K>(Pn, Np, q, pq);
-#pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] = acc[k] + pq[k] * w;
-            }
-        });
-#pragma unroll
-        for (int t = 0; t < T; ++t)
-            for (int c = 0; c < 3; ++c) vp[3 * t + c] = p[3 * t + c] * (double)uw[c];
-        double px[K];
-        if (!(a.any_ads && (e & kNodeInterfacial))) {
-#pragma unroll
-            for (int k = 0; k < K; ++k) px[k] = acc[k] + rm * p[k];
-            store_comps<K>(Px, Np, i, px);
-        } else {
-            const View<Wide> Pan(a.Pan, K * a.Np), Pax(a.Pax, K * a.Np);
-            double pa[K], pax[K];
-            load_comps<K>(Pan, Np, i, pa);
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                const double rt = rm - a.tp.ka[t];
-                for (int c = 0; c < 3; ++c) {
-                    const int k = 3 * t + c;
-                    if (a.tp.ads[t]) {
-                        px[k] = (acc[k] + rt * p[k]) + pa[k] * a.tp.kd[t];
-                        pax[k] = pa[k] * a.tp.one_minus_kd[t] + p[k] * a.tp.ka[t];
-                    } else {  // a tracer without adsorption keeps Pa = 0
-                        px[k] = acc[k] + rm * p[k];
-                        pax[k] = pa[k];
-                    }
-                }
-            }
-            store_comps<K>(Px, Np, i, px);
-            store_comps<K>(Pax, Np, i, pax);
-        }
-    }
-    block_partial<K>(vp, a.partials, lds);
-}
-
-// Plain streaming copy (16 B per lane per iteration, grid-stride): the
-// measured ceiling that the roofline fractions are read against, and the
-// known-size kernel that calibrates the HBM traffic counters.
-__global__ __launch_bounds__(kBlock) void k_copy(const double2* __restrict__ a, double2* __restrict__ b, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
-        b[i] = a[i];
-}
-
-}  // namespace
 
 // ====================================================================== host
 
@@ -1055,6 +354,7 @@ void launch_mp_prep(lbx_ctx* c, const double fext[3], double eps) {
     a.fz = fext[2];
     a.lambda = c->lambda;
     a.eps = eps;
+    a.tile0 = 0;
     a.tp = c->tp;
     const uint32_t gt = c->ntiles;
     with_tracers(c->ntr, [&](auto TT) {
@@ -1256,7 +556,7 @@ int lbx_init_rest(lbx_ctx* c, double rho0) {
     if (!c) return LBX_ERR_ARG;
     if (c->prec == LBX_PREC_FP32_SHIFT) c->rho0 = rho0;  // the shift follows the rest state
     hipLaunchKernelGGL(k_init_rest, dim3(c->ntiles), dim3(kBlock), 0, c->stream, c->f[0], c->g[0], c->j32, c->rho,
-                       c->j, c->Nf, c->Np, rho0);
+                       c->j, 0u, c->Nf, c->Np, rho0);
     c->pre = 0;
     LBX_TRY(hipGetLastError());
     return hip_status(hipStreamSynchronize(c->stream));
@@ -1320,10 +620,11 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
                                c->stream));
     if (shift) {
         hipLaunchKernelGGL(k_collide_packed_shift, dim3(gt), dim3(kBlock), 0, c->stream, c->g[0], c->rho, c->j,
-                           c->j32, 0.0f, 0.0f, 0.0f, c->Nf, c->Np, (float)c->A, (float)c->B, (float)c->C, c->rho0);
+                           c->j32, 0.0f, 0.0f, 0.0f, 0u, c->Nf, c->Np, (float)c->A, (float)c->B, (float)c->C,
+                           c->rho0);
     } else {
         hipLaunchKernelGGL(k_collide_packed, dim3(gt), dim3(kBlock), 0, c->stream, c->f[0], c->rho, c->j, 0.0, 0.0,
-                           0.0, c->Nf, c->Np, c->A, c->B, c->C);
+                           0.0, 0u, c->Nf, c->Np, c->A, c->B, c->C);
     }
     LBX_TRY(hipGetLastError());
     LBX_TRY(reset_ctrl(c));
@@ -1409,11 +710,11 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
             // carries F(t)/2 as in the step kernel
             const double* Ft = t > switch_step ? F : zero;
             hipLaunchKernelGGL(k_extract_shift, dim3(gt), dim3(kBlock), 0, c->stream, c->g[src], c->g[dst], c->rho,
-                               c->j, c->nodes, c->map, c->dlx, c->dly, c->lz, c->Nf, c->Np, c->rho0, Ft[0] / 2.0,
-                               Ft[1] / 2.0, Ft[2] / 2.0);
+                               c->j, c->nodes, c->map, c->dlx, c->dly, c->lz, 0u, c->Nf, c->Np, c->rho0,
+                               Ft[0] / 2.0, Ft[1] / 2.0, Ft[2] / 2.0);
         } else {
             hipLaunchKernelGGL(k_extract, dim3(gt), dim3(kBlock), 0, c->stream, c->f[src], c->f[dst], c->rho,
-                               c->nodes, c->map, c->dlx, c->dly, c->lz, c->Nf, c->Np);
+                               c->nodes, c->map, c->dlx, c->dly, c->lz, 0u, c->Nf, c->Np);
         }
         LBX_TRY(hipGetLastError());
         c->pre = dst;

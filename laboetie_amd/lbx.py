@@ -59,6 +59,19 @@ SIGNATURES = {
     "lbx_set_floor_window": (ctypes.c_int, [_p, _i64]),
     "lbx_fluid_nodes": (_i64, [_p]),
     "lbx_sync": (ctypes.c_int, [_p]),
+    "lbxb_footprint_bytes": (_i64, [_i32, _i64, _i64, _i32]),
+    "lbxb_create": (ctypes.c_int, [_i32, _i64, _i64, _i32, _p]),
+    "lbxb_destroy": (ctypes.c_int, [_p]),
+    "lbxb_add": (ctypes.c_int, [_p, _i32, _i32, _i32, _p, _f64, _f64, _p, _f64, _i64, _f64, _f64, _f64, _i64, _p]),
+    "lbxb_remove": (ctypes.c_int, [_p, _i32]),
+    "lbxb_round": (ctypes.c_int, [_p, _i32, _i32, _p]),
+    "lbxb_info": (ctypes.c_int, [_p, _i32, _p, _p, _p, _p, _p]),
+    "lbxb_result": (ctypes.c_int, [_p, _i32, _p, _p, _i64, _p, _p, _p, _p]),
+    "lbxb_set_block": (ctypes.c_int, [_p, _i32, _i32]),
+    "lbxb_autotune": (ctypes.c_int, [_p, _i32, _p, _p, _p]),
+    "lbxb_set_graphs": (ctypes.c_int, [_p, _i32]),
+    "lbxb_stats": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
+    "lbxb_free_tiles": (_i64, [_p]),
     "lbx_copy_bandwidth": (ctypes.c_int, [_i64, _i32, _p]),
 }
 
@@ -327,3 +340,108 @@ class Lattice:
 
     def sync(self):
         _check(lib().lbx_sync(self._h), "sync")
+
+
+# ------------------------------------------------------------- batch engine
+PHASE_LB, PHASE_SWITCH, PHASE_LB_DONE, PHASE_MP, PHASE_DONE, PHASE_FAILED = 1, 2, 3, 4, 5, 6
+
+
+class Batch:
+    """Many lattices resident in HBM, advanced together (an lbxb)."""
+
+    def __init__(self, capacity_nodes, capacity_box, max_instances, precision=PREC_FP64):
+        h = _p()
+        _check(lib().lbxb_create(precision, int(capacity_nodes), int(capacity_box), int(max_instances),
+                                 ctypes.byref(h)), "lbxb_create")
+        self._h = h
+        self.precision = precision
+        self.dims = {}
+
+    @staticmethod
+    def footprint_bytes(capacity_nodes, capacity_box, max_instances, precision=PREC_FP64):
+        return int(lib().lbxb_footprint_bytes(precision, int(capacity_nodes), int(capacity_box), int(max_instances)))
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().lbxb_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, dims, nature, tau, rho0, fext, target_error, Db, ka, kd, mp_steps, max_lb_steps=0):
+        """Admit an instance; returns its id, or None when it does not fit now."""
+        lx, ly, lz = (int(v) for v in dims)
+        nat = np.ascontiguousarray(nature, np.int8)
+        fe = np.ascontiguousarray(fext, np.float64)
+        i = _i32()
+        st = lib().lbxb_add(self._h, lx, ly, lz, _ptr(nat), float(tau), float(rho0), _ptr(fe), float(target_error),
+                            int(max_lb_steps), float(Db), float(ka), float(kd), int(mp_steps), ctypes.byref(i))
+        if st == ERR_NOMEM:
+            return None
+        _check(st, "lbxb_add")
+        self.dims[i.value] = (lx, ly, lz)
+        return i.value
+
+    def remove(self, iid):
+        _check(lib().lbxb_remove(self._h, int(iid)), "lbxb_remove")
+        self.dims.pop(iid, None)
+
+    def round(self, lb_steps=64, mp_steps=64):
+        n = _i32()
+        _check(lib().lbxb_round(self._h, int(lb_steps), int(mp_steps), ctypes.byref(n)), "lbxb_round")
+        return n.value
+
+    def info(self, iid):
+        ph, st, er = _i32(), _i32(), _i32()
+        lb, mp = _i64(), _i64()
+        _check(lib().lbxb_info(self._h, int(iid), ctypes.byref(ph), ctypes.byref(st), ctypes.byref(lb),
+                               ctypes.byref(er), ctypes.byref(mp)), "lbxb_info")
+        return dict(phase=ph.value, status=st.value, lb_steps=lb.value, exit_reason=er.value, mp_steps=mp.value)
+
+    def result(self, iid, vacf_rows=0, flow=False):
+        d = np.zeros(3)
+        v = np.zeros(3 * max(vacf_rows, 1))
+        lx, ly, lz = self.dims[iid]
+        N = lx * ly * lz
+        fl = [np.zeros(N) for _ in range(4)] if flow else [None] * 4
+        _check(lib().lbxb_result(self._h, int(iid), _ptr(d), _ptr(v) if vacf_rows else None, int(vacf_rows),
+                                 *[_ptr(a) for a in fl]), "lbxb_result")
+        out = dict(deff=d, vacf=v[:3 * vacf_rows].reshape(-1, 3))
+        if flow:
+            out["rho"], out["j"] = fl[0], np.stack(fl[1:])
+        return out
+
+    def set_block(self, lb_block, mp_block):
+        _check(lib().lbxb_set_block(self._h, int(lb_block), int(mp_block)), "lbxb_set_block")
+
+    def autotune(self, launches=10):
+        lb, mp = _i32(), _i32()
+        t = np.zeros(4)
+        _check(lib().lbxb_autotune(self._h, int(launches), ctypes.byref(lb), ctypes.byref(mp), _ptr(t)),
+               "lbxb_autotune")
+        return dict(lb_block=lb.value, mp_block=mp.value, ms={"lb128": t[0], "lb256": t[1], "mp128": t[2],
+                                                               "mp256": t[3]})
+
+    def set_graphs(self, on):
+        _check(lib().lbxb_set_graphs(self._h, 1 if on else 0), "lbxb_set_graphs")
+
+    def stats(self):
+        lbms, mpms = _f64(), _f64()
+        a, b_, c, d = _i64(), _i64(), _i64(), _i64()
+        _check(lib().lbxb_stats(self._h, ctypes.byref(lbms), ctypes.byref(a), ctypes.byref(b_), ctypes.byref(mpms),
+                                ctypes.byref(c), ctypes.byref(d)), "lbxb_stats")
+        return dict(lb_ms=lbms.value, lb_box_updates=a.value, lb_fluid_updates=b_.value, mp_ms=mpms.value,
+                    mp_box_updates=c.value, mp_fluid_updates=d.value)
+
+    def free_tiles(self):
+        return int(lib().lbxb_free_tiles(self._h))

@@ -1,0 +1,118 @@
+"""The batch engine (lbxb_*): instances advanced together in one packed
+fluid-node list give the same results as the single-lattice path, alone or
+in a mix (the equivalence a served request relies on).
+
+Bar: with 256-node blocks, bit-identical LB step counts, exit reasons, flow
+fields, VACF(t) and D_eff; with 128-node blocks, identical flow fields and
+VACF within 1e-12 of |VACF(0)| (the per-block VACF partials group nodes
+differently).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from golden_cases import Case
+from laboetie_amd import lbx
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["bcc12_ads", "cyl9_tau08", "random_3d", "chromat_2disks", "cfg1_slit"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def single(c, prec, mp_steps):
+    with lbx.Lattice(c.dims, c.nature, c.tau, precision=prec) as L:
+        L.init_rest(c.rho0)
+        r = L.equilibrate(c.fext, c.target, max_steps=20000)
+        _, rho, j = L.get_state()
+        v0 = L.mp_init(c.fext, c.Db, c.ka, c.kd)
+        m = L.mp_run(mp_steps)
+        return dict(steps=r["steps"], exit_reason=r["exit_reason"], rho=rho, j=j,
+                    vacf=np.vstack([v0, m["vacf"]]), deff=L.mp_deff())
+
+
+def run_batch(cases, prec, mp_steps, block=256, graphs=True, lb_round=64, mp_round=64):
+    cap = sum(((int((c.nature == 0).sum()) + 255) // 256) * 256 for c in cases) + 4096
+    box = sum(int(np.prod(c.dims)) for c in cases) + 4096
+    with lbx.Batch(cap, box, len(cases) + 2, precision=prec) as B:
+        B.set_block(block, block)
+        B.set_graphs(graphs)
+        ids = [B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, mp_steps,
+                     max_lb_steps=20000) for c in cases]
+        assert None not in ids
+        rounds = 0
+        while B.round(lb_round, mp_round) > 0:
+            rounds += 1
+            assert rounds < 2000
+        out = []
+        for iid in ids:
+            inf = B.info(iid)
+            assert inf["phase"] == lbx.PHASE_DONE, inf
+            res = B.result(iid, vacf_rows=mp_steps + 1, flow=True)
+            res.update(inf)
+            out.append(res)
+        return out, B.stats()
+
+
+@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT])
+def test_batch_matches_single_lattice_bitwise(prec):
+    cases = [Case(n) for n in CASES]
+    mp_steps = 150
+    say("batch vs single", prec, [c.name for c in cases])
+    got, stats = run_batch(cases, prec, mp_steps)
+    for c, g in zip(cases, got):
+        s = single(c, prec, mp_steps)
+        say(c.name, "steps", s["steps"], g["lb_steps"], "exit", s["exit_reason"], g["exit_reason"])
+        assert (g["lb_steps"], g["exit_reason"]) == (s["steps"], s["exit_reason"])
+        assert np.array_equal(g["rho"], s["rho"]) and np.array_equal(g["j"], s["j"])
+        assert np.array_equal(g["vacf"], s["vacf"])
+        assert np.array_equal(g["deff"], s["deff"])
+    assert stats["lb_fluid_updates"] > 0 and stats["mp_fluid_updates"] > 0
+
+
+def test_batch_block128_no_graphs_and_mixed_rounds():
+    cases = [Case(n) for n in ["bcc12_ads", "random_3d", "cyl9_tau08"]]
+    mp_steps = 100
+    ref, _ = run_batch(cases, lbx.PREC_FP64, mp_steps)
+    got, _ = run_batch(cases, lbx.PREC_FP64, mp_steps, block=128, graphs=False, lb_round=37, mp_round=23)
+    for c, r, g in zip(cases, ref, got):
+        assert g["lb_steps"] == r["lb_steps"]
+        assert np.array_equal(g["rho"], r["rho"]) and np.array_equal(g["j"], r["j"])
+        scale = np.abs(r["vacf"][0]).max()
+        assert np.all(np.abs(g["vacf"] - r["vacf"]) <= 1e-12 * scale)
+
+
+def test_batch_admission_autotune_and_slot_reuse():
+    c = Case("bcc12_ads")
+    nf = int((c.nature == 0).sum())
+    tiles = (nf + 255) // 256
+    box = int(np.prod(c.dims))
+    with lbx.Batch(2 * tiles * 256, 2 * box, 3) as B:
+        a = B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, 80)
+        b = B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, 80)
+        assert a is not None and b is not None
+        # no room for a third lattice of this size: admission refuses it
+        assert B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, 80) is None
+        B.round(16, 0)  # stage 1 ends at t = 3
+        B.round(16, 0)  # force switch, then stage 2
+        assert B.info(a)["phase"] == lbx.PHASE_LB
+        t = B.autotune(3)
+        assert t["lb_block"] in (128, 256) and t["ms"]["lb128"] > 0 and t["ms"]["lb256"] > 0
+        while B.round(64, 64) > 0:
+            pass
+        ra = B.result(a, vacf_rows=81, flow=True)
+        B.remove(a)
+        # the freed slot and nodes take a new instance, which gives the same results
+        a2 = B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, 80)
+        assert a2 is not None
+        while B.round(64, 64) > 0:
+            pass
+        r2 = B.result(a2, vacf_rows=81, flow=True)
+        assert np.array_equal(ra["j"], r2["j"]) and np.array_equal(ra["rho"], r2["rho"])
+        scale = np.abs(ra["vacf"][0]).max()
+        assert np.all(np.abs(ra["vacf"] - r2["vacf"]) <= 1e-12 * scale)
