@@ -218,7 +218,8 @@ int lbx_timing_enable(lbx_ctx* ctx, int32_t on);
 int lbx_kernel_time(lbx_ctx* ctx, int32_t which, double* total_ms, int64_t* launches);
 
 /* FP32-shift form: a stage of lbx_equilibrate also ends when l2err has not
- * set a new minimum for `steps` consecutive steps (its FP32 rounding floor;
+ * set a new minimum for `steps` consecutive steps after falling at least
+ * 100-fold below the stage's largest l2err (its FP32 rounding floor;
  * a stage-1 floor switches the force on like a convergence, a stage-2 floor
  * exits with LBX_EXIT_FLOOR).  0 disables the test.  Default 1000 for
  * FP32-shift contexts; FP64 contexts keep the reference's semantics and

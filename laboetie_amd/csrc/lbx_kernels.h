@@ -44,7 +44,8 @@ struct Ctrl {
     int32_t floor_hit;           // LB (FP32-shift): the stop at exit_step was the l2err floor test
     int32_t pad2;
     double l2min[2];             // LB floor test: running minimum of l2err of the stage up to step s,
-    int64_t l2min_step[2];       // and the step that set it, in slot s & 1
+    int64_t l2min_step[2];       // the step that set it, in slot s & 1,
+    double l2max[2];             // and the running maximum
     double vacf0[12];            // 3 per tracer (at most 4)
     double deff_s[12], deff_c[12]; // Neumaier running sums of the VACF integrals
 };
@@ -100,6 +101,9 @@ struct StepArgs {
 // A negative population among a block's nodes is reported as +inf in place of
 // its max|dj| (max over bit patterns of non-negative doubles = max of values).
 constexpr unsigned long long kNegMark = 0x7FF0000000000000ull;
+
+// the floor test needs l2err this far below the stage's maximum
+constexpr double kFloorDrop = 1e-2;
 
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
 #pragma unroll
@@ -168,17 +172,22 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E>& a, uint32_t tile
             // record of the stage up to step t-1 from the one up to t-2 (every
             // block derives the same record; block 0 stores it for step t+1)
             const int s0 = (int)((a.t - 2) & 1), s1 = (int)((a.t - 1) & 1);
-            double m = vload(&a.ctrl->l2min[s0]);
+            double m = vload(&a.ctrl->l2min[s0]), mx = vload(&a.ctrl->l2max[s0]);
             int64_t ms = vload(&a.ctrl->l2min_step[s0]);
             if (prev < m) {
                 m = prev;
                 ms = a.t - 1;
             }
+            mx = fmax(mx, prev);
             if (first) {
                 a.ctrl->l2min[s1] = m;
                 a.ctrl->l2min_step[s1] = ms;
+                a.ctrl->l2max[s1] = mx;
             }
-            floor = (a.t - 1) - ms >= a.floor_window;
+            // a floor: no new minimum for floor_window steps after l2err has
+            // fallen 100-fold below the stage's largest value (a plateau of a
+            // still-accelerating flow is not one)
+            floor = (a.t - 1) - ms >= a.floor_window && m <= kFloorDrop * mx;
         }
         if (a.t - 1 > a.check_after && (prev <= a.target || floor)) {
             if (first) {

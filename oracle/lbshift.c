@@ -114,7 +114,8 @@ static float collide_shift(int l, float g, float drho, float jx, float jy, float
  * density rho0 (g = 0, j = 0).  Same stage logic as lbref_equilibrate; the
  * l2err of a step is max|j - j_old| of the FP32 fluxes.  floor_window > 0:
  * a stage also ends when l2err has set no new minimum for that many steps
- * (the FP32 rounding floor; a stage-2 floor exits with LBREF_EXIT_FLOOR).  On return n (FP64,
+ * after falling 100-fold below the stage's largest value (the FP32 rounding
+ * floor; a stage-2 floor exits with LBREF_EXIT_FLOOR).  On return n (FP64,
  * (lx,ly,lz,19)) = a0 rho0 + g of the post-streaming state of the last step
  * (0 at solids), rho = rho0 + sum g and j = F/2 + sum g c, both summed in
  * FP64 from the FP32 deviations. */
@@ -144,8 +145,9 @@ int lbshift_equilibrate(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature
     int conv_without = 0;
     int64_t switch_step = INT64_MAX;
     int64_t check_after = 2;              /* stop tests apply to steps after this one */
-    double rec_min = HUGE_VAL;            /* floor test: the stage's smallest l2err so far */
-    int64_t rec_step = check_after;       /* and the step that set it */
+    double rec_min = HUGE_VAL;            /* floor test: the stage's smallest l2err so far, */
+    int64_t rec_step = check_after;       /* the step that set it */
+    double rec_max = 0.0;                 /* and the stage's largest */
     double F[3] = {0.0, 0.0, 0.0}; /* force of the current step */
     int64_t t;
     for (t = 1;; ++t) {
@@ -194,7 +196,8 @@ int lbshift_equilibrate(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature
                 rec_min = l2;
                 rec_step = t;
             }
-            floor = t - rec_step >= floor_window;
+            rec_max = fmax(rec_max, l2);
+            floor = t - rec_step >= floor_window && rec_min <= 1e-2 * rec_max;
         }
         if (t > check_after && (l2 <= target_error || floor)) {
             if (!conv_without) {
@@ -203,6 +206,7 @@ int lbshift_equilibrate(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature
                 check_after = t;
                 rec_min = HUGE_VAL;
                 rec_step = t;
+                rec_max = 0.0;
                 F[0] = fext[0];
                 F[1] = fext[1];
                 F[2] = fext[2];

@@ -116,3 +116,31 @@ def test_batch_admission_autotune_and_slot_reuse():
         assert np.array_equal(ra["j"], r2["j"]) and np.array_equal(ra["rho"], r2["rho"])
         scale = np.abs(ra["vacf"][0]).max()
         assert np.all(np.abs(ra["vacf"] - r2["vacf"]) <= 1e-12 * scale)
+
+
+def test_served_requests_equal_alone():
+    """Requests arriving while others run (continuous batching) give the same
+    D_eff, bit for bit, as each request solved alone on the single-lattice
+    path."""
+    from laboetie_amd import serving
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(6):
+        N = int(rng.integers(8, 17))
+        kind = k % 3
+        nat = lbx.build_geometry(3 if kind == 0 else 2, N, N, N) if kind < 2 else lbx.geometry_random(N, N, N, 2.0,
+                                                                                                    0.6, k + 1)
+        dF = float(rng.uniform(-3, 0))
+        reqs.append(dict(dims=(N, N, N), nature=nat, tau=1.0, rho0=1.0, fext=(0.0, 0.0, 2e-6), target=1e-9, Db=0.01,
+                         ka=0.01 * np.exp(-dF), kd=0.01, mp_steps=120, max_lb=3000))
+    arrivals = np.array([0.0, 0.0, 0.002, 0.004, 0.006, 0.008])
+    cap, box = serving.capacity_for(reqs[:3])
+    s = serving.serve(reqs, arrivals, lbx.PREC_FP64, cap, box, max_instances=4, lb_round=32, mp_round=32)
+    assert np.all(s["status"] == 0) and np.all(np.isfinite(s["latency"]))
+    for r, k in zip(reqs, range(len(reqs))):
+        with lbx.Lattice(r["dims"], r["nature"], r["tau"]) as L:
+            L.init_rest(r["rho0"])
+            L.equilibrate(r["fext"], r["target"], max_steps=r["max_lb"])
+            L.mp_init(r["fext"], r["Db"], r["ka"], r["kd"])
+            L.mp_run(r["mp_steps"])
+            assert np.array_equal(L.mp_deff(), s["deff"][k]), k
