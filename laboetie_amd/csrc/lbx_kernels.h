@@ -139,6 +139,9 @@ __device__ __forceinline__ E pull(const View<Wide, E>& fin, uint32_t Np, uint32_
 // density (:254) and flux (:293-300), the l2err contribution (:339-344), then
 // the collision that opens step t+1 (module_collision.f90:87-108).
 // E = float: the FP32-shift form; the l2err is still a max of FP64 values.
+#ifndef LBX_MP_WAVES
+#define LBX_MP_WAVES 1
+#endif
 #ifndef LBX_LB_WAVES
 #define LBX_LB_WAVES 1
 #endif
@@ -752,7 +755,7 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
 }
 
 template <bool Wide, int T, typename W>
-__global__ __launch_bounds__(kBlock) void k_mp_step(MpStepArgs<W> a) {
+__global__ __launch_bounds__(kBlock, LBX_MP_WAVES) void k_mp_step(MpStepArgs<W> a) {
     mp_step_tile<kBlock, Wide, T, W>(a, xcd_tile(), blockIdx.x == 0, gridDim.x);
 }
 

@@ -134,7 +134,8 @@ def run_one_at_a_time(reqs, precision):
 
 
 # ------------------------------------------------------------------ serving
-def serve(reqs, arrivals, precision, capacity_nodes, capacity_box, max_instances=256, lb_round=64, mp_round=64):
+def serve(reqs, arrivals, precision, capacity_nodes, capacity_box, max_instances=256, lb_round=64, mp_round=64,
+          hold=5.0):
     """Continuous batching: requests arrive at `arrivals` (s, from start);
     each is admitted as soon as its nodes fit, runs alongside the others, and
     is removed when done.  Returns per-request latency (s)."""
@@ -151,17 +152,24 @@ def serve(reqs, arrivals, precision, capacity_nodes, capacity_box, max_instances
             now = time.perf_counter() - t0
             while pending and arrivals[pending[0]] <= now:
                 queue.append(pending.pop(0))
-            # admission in arrival order, as HBM allows
-            while queue:
-                iid = _add(B, reqs[queue[0]])
+            # admission in arrival order as HBM allows; a request that does not
+            # fit yet keeps its place while later ones that fit go ahead
+            # (backfilling), unless it has waited longer than `hold` seconds
+            waiting = []
+            for k in queue:
+                if waiting and now - arrivals[waiting[0]] > hold:
+                    waiting.append(k)
+                    continue
+                iid = _add(B, reqs[k])
                 if iid is None:
                     if not running:  # larger than the whole batch: refused
-                        k = queue.pop(0)
                         status[k] = lbx.ERR_NOMEM
                         lat[k] = time.perf_counter() - t0 - arrivals[k]
                         continue
-                    break
-                running[iid] = queue.pop(0)
+                    waiting.append(k)
+                else:
+                    running[iid] = k
+            queue = waiting
             if not running:
                 if pending and not queue:
                     time.sleep(max(0.0, min(0.01, arrivals[pending[0]] - now)))
@@ -230,9 +238,9 @@ def main(argv=None):
     rng = np.random.default_rng(args.seed)
     reqs = [serving_request(rng) for _ in range(args.requests)]
     prec = _prec("fp32_shift" if args.precision == "both" else args.precision)
-    cap_nodes, cap_box = capacity_for(reqs[:12], 1.0)
-    cap_nodes = max(cap_nodes, 16 << 20)
-    cap_box = max(cap_box, 24 << 20)
+    # HBM budget: 48 M fluid nodes (~20 GB in the FP32-shift form), room for
+    # several of the largest requests at once
+    cap_nodes, cap_box = 48 << 20, 64 << 20
     # saturation throughput: a closed batch of the first requests, all admitted at t = 0
     sat_reqs = reqs[:12]
     t = serve(sat_reqs, np.zeros(len(sat_reqs)), prec, cap_nodes, cap_box)
