@@ -432,6 +432,21 @@ __global__ __launch_bounds__(kBlock) void k_init_rest(double* __restrict__ f, fl
 }
 
 // ---------------------------------------------------------------- MP kernels
+
+// packed nodes on either side of an MP block whose P the block stages in LDS
+#ifndef LBX_MP_HALO
+#define LBX_MP_HALO 128
+#endif
+constexpr int kMpHalo = LBX_MP_HALO;
+
+// a block's LDS array with View's element / pair access (load_comps, store_comps)
+struct LdsView {
+    double* b;
+    __device__ __forceinline__ double ld(uint32_t idx) const { return b[idx]; }
+    __device__ __forceinline__ void st(uint32_t idx, double v) const { b[idx] = v; }
+    __device__ __forceinline__ Vec2<double> ld2(uint32_t pidx) const { return ((const Vec2<double>*)b)[pidx]; }
+    __device__ __forceinline__ void st2(uint32_t pidx, Vec2<double> v) const { ((Vec2<double>*)b)[pidx] = v; }
+};
 //
 // Several tracers that share the flow and Db (so the scattering
 // probabilities win, rem and u*) but differ in ka, kd can be propagated in one
@@ -686,23 +701,39 @@ template <int BLOCK, bool Wide, int T, typename W>
 __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t tile, bool first_block,
                                              uint32_t nparts) {
     constexpr int K = 3 * T;  // moment components of the batch
+    constexpr int WIN = BLOCK + 2 * kMpHalo;
     __shared__ double lds[BLOCK / 64 * K];
+    __shared__ double pwin[K * WIN];  // P(now) of packed nodes [w0, w0 + WIN), comp_index layout
     if (vload(&a.ctrl->mp_stop)) return;
     if (a.prev_partials && first_block)
         mp_reduce_block<K, BLOCK>(a.prev_partials, nparts, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
     const uint32_t i = tile * BLOCK + threadIdx.x;
+    const uint32_t Np = (uint32_t)a.Np;
+    const View<Wide> Pn(a.Pn, K * a.Np);
+    // The block's nodes and kMpHalo on either side: neighbours along x, and
+    // along y where a row holds fewer fluid nodes than that, are read from LDS
+    const int64_t w0 = (int64_t)tile * BLOCK - kMpHalo;
+    const LdsView pw{pwin};
+    for (int n = threadIdx.x; n < WIN; n += BLOCK) {
+        const int64_t g = w0 + n;
+        if (g >= 0 && g < (int64_t)Np) {
+            double v[K];
+            load_comps<K>(Pn, Np, (uint32_t)g, v);
+            store_comps<K>(pw, WIN, (uint32_t)n, v);
+        }
+    }
+    __syncthreads();
     double vp[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) vp[k] = 0.0;
     if (i < a.Nf) {
-        const uint32_t Np = (uint32_t)a.Np;
         const View<Wide, W> win(a.win, 18 * a.Np), us(a.ustar, 3 * a.Np);
-        const View<Wide> Pn(a.Pn, K * a.Np), Px(a.Px, K * a.Np), rem(a.rem, a.Np);
+        const View<Wide> Px(a.Px, K * a.Np), rem(a.rem, a.Np);
         const uint32_t e = ld(a.nodes, i);
         Neighbours nb;
         nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
         double p[K], acc[K];
-        load_comps<K>(Pn, Np, i, p);
+        load_comps<K>(pw, WIN, (uint32_t)(kMpHalo + threadIdx.x), p);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] = 0.0;
         W uw[3];
@@ -716,7 +747,11 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
             if (q != kSolidSlot) {
                 const double w = (double)wp[(L - 1) & 1];
                 double pq[K];
-                load_comps<K>(Pn, Np, q, pq);
+                const uint32_t d = (uint32_t)((int64_t)q - w0);
+                if (d < (uint32_t)WIN)
+                    load_comps<K>(pw, WIN, d, pq);
+                else
+                    load_comps<K>(Pn, Np, q, pq);
 #pragma unroll
                 for (int k = 0; k < K; ++k) acc[k] = acc[k] + pq[k] * w;
             }
