@@ -301,10 +301,16 @@ def main():
     roof_lb = {"bound": "hbm", "achieved": round(lb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(lb_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("k_lb_step"),
                "kernel": "k_lb_step", "bytes_per_launch": lb_bytes, "avg_launch_us": round(lb_avg_s * 1e6, 2)}
+    # the same launch priced per tracer-step, as the reference reads its coefficients once per
+    # tracer: 208 B per fluid node and tracer (+48 B Pa at adsorbing interfacial nodes)
+    mp_bytes_tr = (MP_COEF_BYTES_PER_FLUID + MP_P_BYTES_PER_FLUID) * nf * per_launch \
+        + MP_BYTES_PER_ADS_INTERFACIAL * nfi * n_ads
     roof_mp = {"bound": "hbm", "achieved": round(mp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(mp_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("k_mp_step"),
                "kernel": "k_mp_step", "tracers_per_launch": per_launch, "bytes_per_launch": round(mp_bytes),
-               "avg_launch_us": round(mp_avg_s * 1e6, 2)}
+               "avg_launch_us": round(mp_avg_s * 1e6, 2),
+               "per_tracer_step_bytes": round(mp_bytes_tr),
+               "per_tracer_step_achieved": round(mp_bytes_tr / mp_avg_s / 1e9, 1)}
     line = {
         "metric": "D3Q19+MP MLUPS & HBM roofline %, FP64 vs FP32-shift; p50/p99 solve latency",
         "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
