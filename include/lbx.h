@@ -288,13 +288,23 @@ int lbxb_info(lbxb* b, int32_t id, int32_t* phase, int32_t* status, int64_t* lb_
 int lbxb_result(lbxb* b, int32_t id, double deff[3], double* vacf, int64_t vacf_rows, double* rho, double* jx,
                 double* jy, double* jz);
 
-/* Block size (128 or 256 nodes) of the LB and MP launches; the autotuner
- * times both on the current state (without changing it) and keeps the
- * faster; times[4] = mean ms per launch of LB-128, LB-256, MP-128, MP-256. */
+/* Block size (128 or 256 nodes) of the LB and MP launches. */
 int lbxb_set_block(lbxb* b, int32_t lb_block, int32_t mp_block);
+/* How a round's steps are launched: one launch per step on the stream, the
+ * same launches replayed from a captured HIP graph (default), or one
+ * persistent cooperative launch per phase that loops over the steps with a
+ * grid-wide barrier between them (LB block size for both phases).  All give
+ * the same bits. */
+#define LBXB_LAUNCH_STREAM     0
+#define LBXB_LAUNCH_GRAPH      1
+#define LBXB_LAUNCH_PERSISTENT 2
+int lbxb_set_launch_mode(lbxb* b, int32_t mode);
+int lbxb_launch_mode(lbxb* b);
+/* Autotuner: times 128- and 256-node blocks per phase on the current state
+ * without changing it and keeps the faster, then keeps the persistent mode
+ * if it beats per-step launches in every phase; times[6] = mean ms per step
+ * of LB-128, LB-256, MP-128, MP-256, LB-persistent, MP-persistent. */
 int lbxb_autotune(lbxb* b, int32_t launches, int32_t* lb_block, int32_t* mp_block, double* times);
-/* Rounds replay captured HIP graphs of their launches (on by default). */
-int lbxb_set_graphs(lbxb* b, int32_t on);
 /* Device time (HIP events around each phase of each round) and work done. */
 int lbxb_stats(lbxb* b, double* lb_ms, int64_t* lb_box_updates, int64_t* lb_fluid_updates, double* mp_ms,
                int64_t* mp_box_updates, int64_t* mp_fluid_updates);

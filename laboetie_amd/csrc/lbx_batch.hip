@@ -21,6 +21,7 @@
 // Phases (host-driven between rounds, as the single-lattice host loop does):
 // LB stage 1 -> (redo of the switch step with the force on) -> LB stage 2 ->
 // extract + MP prep -> MP steps -> done.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -63,6 +64,7 @@ struct BInst {
     // this round's MP work
     int32_t mp_on, any_ads;
     int64_t mp_it0, mp_n;         // launch k runs MP step mp_it0 + k + 1 while k < mp_n
+    int32_t mp_repeat, pad_;      // every launch recomputes step mp_it0 + 1 (timing)
     TracerParams tp;
 };
 
@@ -81,14 +83,14 @@ struct BatchView {
     uint64_t Np;
 };
 
+// LB launch k of a round on one BLOCK-node tile (a redo repeats step lb_t0)
 template <int BLOCK, bool Wide, typename E>
-__global__ __launch_bounds__(BLOCK) void kb_lb_step(BatchView<E> v, int64_t k) {
-    const uint32_t tile = xcd_tile();
+__device__ __forceinline__ void kb_lb_tile(const BatchView<E>& v, uint32_t tile, int64_t k) {
     const uint32_t id = v.tile_inst[tile / (kTile / BLOCK)];
     if (id == kNoInst) return;
     const BInst& in = v.insts[id];
     if (!in.lb_on || k >= in.lb_n) return;
-    const int64_t t = in.lb_t0 + k;
+    const int64_t t = in.lb_t0 + (in.lb_redo ? 0 : k);
     StepArgs<E> a;
     a.fin = (t - 1) & 1 ? v.f1 : v.f0;
     a.fout = t & 1 ? v.f1 : v.f0;
@@ -123,13 +125,23 @@ __global__ __launch_bounds__(BLOCK) void kb_lb_step(BatchView<E> v, int64_t k) {
     lb_step_tile<BLOCK, Wide, E>(a, tile, tile - first, tile == first);
 }
 
+template <int BLOCK, bool Wide, typename E>
+__global__ __launch_bounds__(BLOCK) void kb_lb_step(BatchView<E> v, int64_t k) {
+    kb_lb_tile<BLOCK, Wide, E>(v, xcd_tile(), k);
+}
+
 // l2err of each LB instance's last step of the round (one wave per instance slot)
-__global__ __launch_bounds__(64) void kb_lb_finalize(const BInst* insts, unsigned long long* slots, double* l2ring,
-                                                     Ctrl* ctrl) {
-    const uint32_t id = blockIdx.x;
+// (called by the first wave of a block)
+__device__ __forceinline__ void kb_lb_finalize_inst(const BInst* insts, unsigned long long* slots, double* l2ring,
+                                                    Ctrl* ctrl, uint32_t id) {
     const BInst& in = insts[id];
     if (!in.lb_on || in.lb_redo || in.lb_n <= 0) return;
     lb_finalize(slots + (uint64_t)id * 3 * kSlots, l2ring + (uint64_t)id * kRing, ctrl + id, in.lb_t0 + in.lb_n - 1);
+}
+
+__global__ __launch_bounds__(64) void kb_lb_finalize(const BInst* insts, unsigned long long* slots, double* l2ring,
+                                                     Ctrl* ctrl) {
+    kb_lb_finalize_inst(insts, slots, l2ring, ctrl, blockIdx.x);
 }
 
 template <typename W>
@@ -153,14 +165,15 @@ struct BatchMpView {
     uint64_t Np;
 };
 
+// MP launch k of a round on one BLOCK-node tile (mp_repeat: every launch
+// recomputes step mp_it0 + 1, for timing)
 template <int BLOCK, bool Wide, typename W>
-__global__ __launch_bounds__(BLOCK) void kb_mp_step(BatchMpView<W> v, int64_t k, int32_t reduce_prev) {
-    const uint32_t tile = xcd_tile();
+__device__ __forceinline__ void kb_mp_tile(const BatchMpView<W>& v, uint32_t tile, int64_t k, int32_t reduce_prev) {
     const uint32_t id = v.tile_inst[tile / (kTile / BLOCK)];
     if (id == kNoInst) return;
     const BInst& in = v.insts[id];
     if (!in.mp_on || k >= in.mp_n) return;
-    const int64_t it = in.mp_it0 + k + 1;  // P(it) -> P(it+1); P(m) lives in buffer (m-1)&1
+    const int64_t it = in.mp_it0 + (in.mp_repeat ? 0 : k) + 1;  // P(it) -> P(it+1); P(m) lives in buffer (m-1)&1
     const uint32_t first = in.tile_first * (kTile / BLOCK);
     MpStepArgs<W> a;
     a.win = v.win;
@@ -173,7 +186,9 @@ __global__ __launch_bounds__(BLOCK) void kb_mp_step(BatchMpView<W> v, int64_t k,
     a.Pan = (it - 1) & 1 ? v.Pa1 : v.Pa0;
     a.Pax = it & 1 ? v.Pa1 : v.Pa0;
     a.partials = it & 1 ? v.part1 : v.part0;
-    a.prev_partials = (reduce_prev && k > 0) ? ((it - 1) & 1 ? v.part1 : v.part0) + 3 * (uint64_t)first : nullptr;
+    a.prev_partials = (reduce_prev && !in.mp_repeat && k > 0)
+                          ? ((it - 1) & 1 ? v.part1 : v.part0) + 3 * (uint64_t)first
+                          : nullptr;
     a.prev_vtrace = v.vtrace + ((uint64_t)(k - 1) * v.vt_stride + id) * 3;
     a.it = it;
     a.ctrl = v.ctrl + id;
@@ -187,20 +202,66 @@ __global__ __launch_bounds__(BLOCK) void kb_mp_step(BatchMpView<W> v, int64_t k,
     mp_step_tile<BLOCK, Wide, 1, W>(a, tile, tile == first, in.ntiles * (kTile / BLOCK));
 }
 
-// VACF of each MP instance's last step of the round (one block per instance slot)
+template <int BLOCK, bool Wide, typename W>
+__global__ __launch_bounds__(BLOCK) void kb_mp_step(BatchMpView<W> v, int64_t k, int32_t reduce_prev) {
+    kb_mp_tile<BLOCK, Wide, W>(v, xcd_tile(), k, reduce_prev);
+}
+
+// VACF of an MP instance's last step of the round: partials of TILE-node
+// tiles, reduced by the NT threads of one block
+template <int TILE, int NT>
+__device__ __forceinline__ void kb_mp_reduce_inst(const BInst* insts, const double* part0, const double* part1,
+                                                  double* vtrace, uint32_t vt_stride, Ctrl* ctrl, uint32_t id) {
+    __shared__ double lds[NT / 64 * 3];
+    const BInst& in = insts[id];
+    if (!in.mp_on || in.mp_n <= 0 || in.mp_repeat) return;
+    const int64_t it = in.mp_it0 + in.mp_n;
+    const uint32_t first = in.tile_first * (kTile / TILE);
+    const double* part = (it & 1 ? part1 : part0) + 3 * (uint64_t)first;
+    mp_reduce_block<3, NT>(part, in.ntiles * (kTile / TILE), vtrace + ((uint64_t)(in.mp_n - 1) * vt_stride + id) * 3,
+                           ctrl + id, 1, it, 0, 0.0, 0.0, lds);
+}
+
+// (one block per instance slot)
 template <int BLOCK>
 __global__ __launch_bounds__(kBlock) void kb_mp_reduce_last(const BInst* insts, const double* part0,
                                                             const double* part1, double* vtrace, uint32_t vt_stride,
                                                             Ctrl* ctrl) {
-    __shared__ double lds[kBlock / 64 * 3];
-    const uint32_t id = blockIdx.x;
-    const BInst& in = insts[id];
-    if (!in.mp_on || in.mp_n <= 0) return;
-    const int64_t it = in.mp_it0 + in.mp_n;
-    const uint32_t first = in.tile_first * (kTile / BLOCK);
-    const double* part = (it & 1 ? part1 : part0) + 3 * (uint64_t)first;
-    mp_reduce_block<3, kBlock>(part, in.ntiles * (kTile / BLOCK), vtrace + ((uint64_t)(in.mp_n - 1) * vt_stride + id) * 3,
-                               ctrl + id, 1, it, 0, 0.0, 0.0, lds);
+    kb_mp_reduce_inst<BLOCK, kBlock>(insts, part0, part1, vtrace, vt_stride, ctrl, blockIdx.x);
+}
+
+// A whole round in one cooperative launch: S_lb LB steps, the LB closing
+// reduction, S_mp MP steps and the MP closing reduction, every block looping
+// over its share of the tiles with a grid-wide barrier between steps.  The
+// per-tile work and its order of partial sums are those of the per-step
+// launches, so the results are the same bits.  Every block runs every
+// barrier: the loops have grid-uniform bounds and the per-tile functions
+// only return from the tile, never from the kernel.
+template <int BLOCK, bool Wide, typename E, typename W>
+__global__ __launch_bounds__(BLOCK) void kb_round(BatchView<E> lv, BatchMpView<W> mv, int64_t S_lb, int64_t S_mp,
+                                                  uint32_t ntiles, uint32_t ninst, int32_t lb_redo_only) {
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    // the tiles of XCD x (blocks b = x mod 8) are one contiguous range
+    const uint32_t nb = gridDim.x, b = blockIdx.x, x = b & 7u, local = b >> 3;
+    const uint32_t nblk = (nb >> 3) + (x < (nb & 7u) ? 1u : 0u);
+    const uint32_t per = (ntiles + 7u) / 8u;
+    const uint32_t t0 = x * per, t1 = min(ntiles, t0 + per);
+    for (int64_t k = 0; k < S_lb; ++k) {
+        for (uint32_t tile = t0 + local; tile < t1; tile += nblk) kb_lb_tile<BLOCK, Wide, E>(lv, tile, k);
+        grid.sync();
+    }
+    if (S_lb > 0 && !lb_redo_only) {
+        for (uint32_t id = b; id < ninst; id += nb)
+            if (threadIdx.x < 64) kb_lb_finalize_inst(lv.insts, lv.slots, lv.l2ring, lv.ctrl, id);
+        grid.sync();
+    }
+    for (int64_t k = 0; k < S_mp; ++k) {
+        for (uint32_t tile = t0 + local; tile < t1; tile += nblk) kb_mp_tile<BLOCK, Wide, W>(mv, tile, k, 1);
+        grid.sync();
+    }
+    if (S_mp > 0)
+        for (uint32_t id = b; id < ninst; id += nb)
+            kb_mp_reduce_inst<BLOCK, BLOCK>(mv.insts, mv.part0, mv.part1, mv.vtrace, mv.vt_stride, mv.ctrl, id);
 }
 
 // ------------------------------------------------------------------ host
@@ -303,6 +364,8 @@ struct lbxb {
     uint32_t cap_inst = 0, ntiles_cap = 0;
     int32_t lb_block = 256, mp_block = 256;
     bool graphs = true;
+    bool persistent = false;  // a round's launches of each phase as one cooperative launch
+    int32_t num_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     // device arrays
@@ -537,6 +600,43 @@ int run_phase(lbxb* b, int phase /*0 LB, 1 MP*/, int64_t S) {
     return hip_status(hipGraphLaunch(it->second.exec, b->stream));
 }
 
+// One phase of a round as a single cooperative launch of kb_round (grid: as
+// many blocks as can be resident at once, at most one per tile)
+int run_phase_persistent(lbxb* b, int phase, int64_t S, int32_t redo_only) {
+    if (S <= 0) return LBX_OK;
+    const int block = b->lb_block;
+    const uint32_t ntiles = grid_of(b, block);
+    if (!ntiles) return LBX_OK;
+    auto go = [&](auto etag, auto wtag, auto btag, auto wide) -> int {
+        using E = decltype(etag);
+        using W = decltype(wtag);
+        constexpr int BL = decltype(btag)::value;
+        constexpr bool WD = decltype(wide)::value;
+        auto kern = kb_round<BL, WD, E, W>;
+        int per_cu = 0;
+        LBX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BL, 0));
+        if (per_cu <= 0) return LBX_ERR_DEVICE;
+        const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)per_cu * (uint32_t)b->num_cu);
+        BatchView<E> lv = lb_view<E>(b);
+        BatchMpView<W> mv = mp_view<W>(b);
+        int64_t S_lb = phase == 0 ? S : 0, S_mp = phase == 1 ? S : 0;
+        uint32_t nt = ntiles, ni = b->cap_inst;
+        int32_t ro = redo_only;
+        void* args[] = {&lv, &mv, &S_lb, &S_mp, &nt, &ni, &ro};
+        return hip_status(hipLaunchCooperativeKernel((const void*)kern, dim3(grid), dim3(BL), args, 0, b->stream));
+    };
+    using I128 = std::integral_constant<int, 128>;
+    using I256 = std::integral_constant<int, 256>;
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    if (shift(b)) {
+        if (block == 128) return b->wide ? go(float{}, float{}, I128{}, T_{}) : go(float{}, float{}, I128{}, F_{});
+        return b->wide ? go(float{}, float{}, I256{}, T_{}) : go(float{}, float{}, I256{}, F_{});
+    }
+    if (block == 128) return b->wide ? go(double{}, double{}, I128{}, T_{}) : go(double{}, double{}, I128{}, F_{});
+    return b->wide ? go(double{}, double{}, I256{}, T_{}) : go(double{}, double{}, I256{}, F_{});
+}
+
 template <typename T>
 int download(lbxb* b, const T* src, uint64_t count, T* dst) {
     LBX_TRY(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, b->stream));
@@ -668,6 +768,11 @@ int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
     b->cap_inst = (uint32_t)max_instances;
     b->ntiles_cap = (uint32_t)(Np / kTile);
     b->wide = kQ * Np * 8 >= (1ull << 32);
+    {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&b->num_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    }
     const bool sh = precision == LBX_PREC_FP32_SHIFT;
     int st = LBX_OK;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = LBX_ERR_DEVICE;
@@ -917,7 +1022,7 @@ int lbxb_round(lbxb* b, int32_t lb_steps, int32_t mp_steps, int32_t* active) {
     LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
     if (S_lb > 0) {
         LBX_TRY(hipEventRecord(b->ev[0], b->stream));
-        LBX_TRY(run_phase(b, 0, S_lb));
+        LBX_TRY(b->persistent ? run_phase_persistent(b, 0, S_lb, 0) : run_phase(b, 0, S_lb));
         LBX_TRY(hipEventRecord(b->ev[1], b->stream));
         LBX_TRY(hipEventSynchronize(b->ev[1]));
         float ms = 0.f;
@@ -926,7 +1031,7 @@ int lbxb_round(lbxb* b, int32_t lb_steps, int32_t mp_steps, int32_t* active) {
     }
     if (S_mp > 0) {
         LBX_TRY(hipEventRecord(b->ev[0], b->stream));
-        LBX_TRY(run_phase(b, 1, S_mp));
+        LBX_TRY(b->persistent ? run_phase_persistent(b, 1, S_mp, 0) : run_phase(b, 1, S_mp));
         LBX_TRY(hipEventRecord(b->ev[1], b->stream));
         LBX_TRY(hipEventSynchronize(b->ev[1]));
         float ms = 0.f;
@@ -1042,19 +1147,28 @@ int lbxb_set_block(lbxb* b, int32_t lb_block, int32_t mp_block) {
     return LBX_OK;
 }
 
-int lbxb_set_graphs(lbxb* b, int32_t on) {
-    if (!b) return LBX_ERR_ARG;
-    b->graphs = on != 0;
+int lbxb_set_launch_mode(lbxb* b, int32_t mode) {
+    if (!b || mode < LBXB_LAUNCH_STREAM || mode > LBXB_LAUNCH_PERSISTENT) return LBX_ERR_ARG;
+    b->graphs = mode == LBXB_LAUNCH_GRAPH;
+    b->persistent = mode == LBXB_LAUNCH_PERSISTENT;
     if (!b->graphs) clear_graphs(b);
     return LBX_OK;
+}
+
+int lbxb_launch_mode(lbxb* b) {
+    if (!b) return -1;
+    return b->persistent ? LBXB_LAUNCH_PERSISTENT : (b->graphs ? LBXB_LAUNCH_GRAPH : LBXB_LAUNCH_STREAM);
 }
 
 // Times `launches` launches of each block size on the current state without
 // changing it: LB launches redo each LB instance's last completed step (same
 // inputs, same outputs), MP launches recompute its last MP step without the
-// VACF reduction.  Picks the faster size per phase; times[4] (may be NULL)
-// receives the mean launch time in ms of LB-128, LB-256, MP-128, MP-256 (0
-// when the phase had no instance to time).
+// VACF reduction.  Picks the faster size per phase, then times the same
+// repeated steps in one cooperative launch per phase and keeps that
+// (persistent) mode when it is faster in every phase, else HIP graphs.
+// times[6] (may be NULL) receives mean ms per step of LB-128, LB-256,
+// MP-128, MP-256, LB-persistent, MP-persistent (0 when a phase had no
+// instance to time).
 int lbxb_autotune(lbxb* b, int32_t launches, int32_t* lb_block, int32_t* mp_block, double* times) {
     if (!b || launches <= 0) return LBX_ERR_ARG;
     const uint32_t ni = b->cap_inst;
@@ -1099,14 +1213,49 @@ int lbxb_autotune(lbxb* b, int32_t launches, int32_t* lb_block, int32_t* mp_bloc
             tm[2 * phase + s] = ms / launches;
         }
     }
-    for (uint32_t id = 0; id < ni; ++id) b->tab[id].lb_on = b->tab[id].mp_on = b->tab[id].lb_redo = 0;
-    LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
     if (any_lb) b->lb_block = tm[0] < tm[1] ? 128 : 256;
     if (any_mp) b->mp_block = tm[2] < tm[3] ? 128 : 256;
+    // one cooperative launch of `launches` repeated steps per phase, with the
+    // LB block size (kb_round runs both phases with one size)
+    double tp[2] = {0, 0};
+    for (uint32_t id = 0; id < ni; ++id) {
+        BInst& in = b->tab[id];
+        if (in.lb_on) in.lb_n = launches;  // lb_redo: every launch recomputes step t
+        if (in.mp_on) in.mp_n = launches, in.mp_repeat = 1;
+    }
+    LBX_TRY(hipStreamSynchronize(b->stream));
+    LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
+    for (int phase = 0; phase < 2; ++phase) {
+        if ((phase == 0 && !any_lb) || (phase == 1 && !any_mp)) continue;
+        LBX_TRY(run_phase_persistent(b, phase, 1, 1));  // warm
+        LBX_TRY(hipEventRecord(b->ev[0], b->stream));
+        LBX_TRY(run_phase_persistent(b, phase, launches, 1));
+        LBX_TRY(hipEventRecord(b->ev[1], b->stream));
+        LBX_TRY(hipEventSynchronize(b->ev[1]));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+        tp[phase] = ms / launches;
+    }
+    for (uint32_t id = 0; id < ni; ++id) {
+        BInst& in = b->tab[id];
+        in.lb_on = in.mp_on = in.lb_redo = in.mp_repeat = 0;
+    }
+    LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
+    // persistent when it beats the per-step launches in every phase that was timed
+    const double per_lb = any_lb ? std::min(tm[0], tm[1]) : 0.0, per_mp = any_mp ? std::min(tm[2], tm[3]) : 0.0;
+    if (any_lb || any_mp) {
+        const bool pers = (!any_lb || tp[0] < per_lb) && (!any_mp || tp[1] < per_mp);
+        b->persistent = pers;
+        b->graphs = !pers;
+    }
     clear_graphs(b);
     if (lb_block) *lb_block = b->lb_block;
     if (mp_block) *mp_block = b->mp_block;
-    if (times) std::memcpy(times, tm, sizeof(tm));
+    if (times) {
+        std::memcpy(times, tm, sizeof(tm));
+        times[4] = tp[0];
+        times[5] = tp[1];
+    }
     return hip_status(hipStreamSynchronize(b->stream));
 }
 

@@ -54,6 +54,17 @@ __device__ __forceinline__ int32_t vload(const int32_t* p) { return *(const vola
 __device__ __forceinline__ int64_t vload(const int64_t* p) { return *(const volatile int64_t*)p; }
 __device__ __forceinline__ double vload(const double* p) { return *(const volatile double*)p; }
 
+// A control flag as thread 0 of the block read it, the same for every wave:
+// another block may write the flag while this block's waves read it, and a
+// block whose waves disagreed would split at its next barrier.
+__device__ __forceinline__ bool block_flag(bool v) {
+    __shared__ int32_t flag;
+    __syncthreads();  // the previous use of `flag` by this block is over
+    if (threadIdx.x == 0) flag = v ? 1 : 0;
+    __syncthreads();
+    return flag != 0;
+}
+
 // packed indices of the 18 neighbours r + c_l (l = 1..18) of full-box node r
 struct Neighbours {
     uint32_t q[kQ];
@@ -153,7 +164,7 @@ __device__ __forceinline__ E pull(const View<Wide, E>& fin, uint32_t Np, uint32_
 template <int BLOCK, bool Wide, typename E>
 __device__ __forceinline__ void lb_step_tile(const StepArgs<E>& a, uint32_t tile, uint32_t stripe, bool first_block) {
     constexpr bool kShift = std::is_same<E, float>::value;
-    if (vload(&a.ctrl->stop) || vload(&a.ctrl->negpop) >= 0) return;
+    if (block_flag(vload(&a.ctrl->stop) || vload(&a.ctrl->negpop) >= 0)) return;
     if (a.have_prev) {
         // l2err of step t-1 (equilibration.f90:339-344) from its striped
         // slots, then the negativity (:248) and convergence (:346) verdicts
@@ -704,7 +715,7 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     constexpr int WIN = BLOCK + 2 * kMpHalo;
     __shared__ double lds[BLOCK / 64 * K];
     __shared__ double pwin[K * WIN];  // P(now) of packed nodes [w0, w0 + WIN), comp_index layout
-    if (vload(&a.ctrl->mp_stop)) return;
+    if (block_flag(vload(&a.ctrl->mp_stop) != 0)) return;
     if (a.prev_partials && first_block)
         mp_reduce_block<K, BLOCK>(a.prev_partials, nparts, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
     const uint32_t i = tile * BLOCK + threadIdx.x;

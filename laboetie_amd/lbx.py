@@ -69,7 +69,8 @@ SIGNATURES = {
     "lbxb_result": (ctypes.c_int, [_p, _i32, _p, _p, _i64, _p, _p, _p, _p]),
     "lbxb_set_block": (ctypes.c_int, [_p, _i32, _i32]),
     "lbxb_autotune": (ctypes.c_int, [_p, _i32, _p, _p, _p]),
-    "lbxb_set_graphs": (ctypes.c_int, [_p, _i32]),
+    "lbxb_set_launch_mode": (ctypes.c_int, [_p, _i32]),
+    "lbxb_launch_mode": (ctypes.c_int, [_p]),
     "lbxb_stats": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
     "lbxb_free_tiles": (_i64, [_p]),
     "lbx_copy_bandwidth": (ctypes.c_int, [_i64, _i32, _p]),
@@ -344,6 +345,7 @@ class Lattice:
 
 # ------------------------------------------------------------- batch engine
 PHASE_LB, PHASE_SWITCH, PHASE_LB_DONE, PHASE_MP, PHASE_DONE, PHASE_FAILED = 1, 2, 3, 4, 5, 6
+LAUNCH_STREAM, LAUNCH_GRAPH, LAUNCH_PERSISTENT = 0, 1, 2
 
 
 class Batch:
@@ -426,14 +428,22 @@ class Batch:
 
     def autotune(self, launches=10):
         lb, mp = _i32(), _i32()
-        t = np.zeros(4)
+        t = np.zeros(6)
         _check(lib().lbxb_autotune(self._h, int(launches), ctypes.byref(lb), ctypes.byref(mp), _ptr(t)),
                "lbxb_autotune")
-        return dict(lb_block=lb.value, mp_block=mp.value, ms={"lb128": t[0], "lb256": t[1], "mp128": t[2],
-                                                               "mp256": t[3]})
+        return dict(lb_block=lb.value, mp_block=mp.value, mode=self.launch_mode,
+                    ms={"lb128": t[0], "lb256": t[1], "mp128": t[2], "mp256": t[3], "lb_persistent": t[4],
+                        "mp_persistent": t[5]})
+
+    def set_launch_mode(self, mode):
+        _check(lib().lbxb_set_launch_mode(self._h, int(mode)), "lbxb_set_launch_mode")
+
+    @property
+    def launch_mode(self):
+        return int(lib().lbxb_launch_mode(self._h))
 
     def set_graphs(self, on):
-        _check(lib().lbxb_set_graphs(self._h, 1 if on else 0), "lbxb_set_graphs")
+        self.set_launch_mode(LAUNCH_GRAPH if on else LAUNCH_STREAM)
 
     def stats(self):
         lbms, mpms = _f64(), _f64()

@@ -103,6 +103,7 @@ def test_batch_admission_autotune_and_slot_reuse():
         assert B.info(a)["phase"] == lbx.PHASE_LB
         t = B.autotune(3)
         assert t["lb_block"] in (128, 256) and t["ms"]["lb128"] > 0 and t["ms"]["lb256"] > 0
+        assert t["ms"]["lb_persistent"] > 0 and t["mode"] in (lbx.LAUNCH_GRAPH, lbx.LAUNCH_PERSISTENT)
         while B.round(64, 64) > 0:
             pass
         ra = B.result(a, vacf_rows=81, flow=True)
@@ -144,3 +145,28 @@ def test_served_requests_equal_alone():
             L.mp_init(r["fext"], r["Db"], r["ka"], r["kd"])
             L.mp_run(r["mp_steps"])
             assert np.array_equal(L.mp_deff(), s["deff"][k]), k
+
+
+@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT])
+def test_batch_persistent_launches_same_bits(prec):
+    """A round's steps as one cooperative launch per phase (grid-wide barrier
+    between steps) give the same bits as per-step launches."""
+    cases = [Case(n) for n in ["bcc12_ads", "random_3d", "cyl9_tau08", "cfg1_slit"]]
+    mp_steps = 90
+    ref, _ = run_batch(cases, prec, mp_steps)
+    cap = sum(((int((c.nature == 0).sum()) + 255) // 256) * 256 for c in cases) + 4096
+    box = sum(int(np.prod(c.dims)) for c in cases) + 4096
+    with lbx.Batch(cap, box, len(cases) + 2, precision=prec) as B:
+        B.set_launch_mode(lbx.LAUNCH_PERSISTENT)
+        assert B.launch_mode == lbx.LAUNCH_PERSISTENT
+        ids = [B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, mp_steps,
+                     max_lb_steps=20000) for c in cases]
+        while B.round(64, 64) > 0:
+            pass
+        for c, iid, r in zip(cases, ids, ref):
+            inf = B.info(iid)
+            g = B.result(iid, vacf_rows=mp_steps + 1, flow=True)
+            say("persistent", c.name, inf["lb_steps"], r["lb_steps"])
+            assert (inf["lb_steps"], inf["exit_reason"]) == (r["lb_steps"], r["exit_reason"])
+            assert np.array_equal(g["rho"], r["rho"]) and np.array_equal(g["j"], r["j"])
+            assert np.array_equal(g["vacf"], r["vacf"]) and np.array_equal(g["deff"], r["deff"])
