@@ -91,14 +91,18 @@ def run_batch(reqs, precision, lb_round=128, mp_round=128, autotune=True):
     with lbx.Batch(cap, box, len(reqs), precision=precision) as B:
         ids = [_add(B, r) for r in reqs]
         assert None not in ids, "capacity"
-        tuned = None
+        tuned = []
         t0 = time.perf_counter()
         rounds = 0
+        mp_seen = False
         while True:
             active = B.round(lb_round, mp_round)
             rounds += 1
-            if autotune and tuned is None and rounds == 2:
-                tuned = B.autotune(5)
+            if autotune and rounds == 2:
+                tuned.append(B.autotune(5))  # LB phase
+            if autotune and not mp_seen and B.stats()["mp_fluid_updates"] > 0:
+                mp_seen = True
+                tuned.append(B.autotune(5))  # once MP has started
             if active == 0:
                 break
         wall = time.perf_counter() - t0
