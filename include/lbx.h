@@ -118,7 +118,9 @@ int lbx_destroy(lbx_ctx* ctx);
 int lbx_init_rest(lbx_ctx* ctx, double rho0);
 
 /* Arbitrary state: n (post-streaming populations, (lx,ly,lz,19)), and the
- * density and flux that the next collision uses (collide's arguments). */
+ * density and flux that the next collision uses (collide's arguments).  In
+ * the FP32-shift form the shift rho0 becomes the density of the first fluid
+ * node (lbx_init_rest sets it to its rho0). */
 int lbx_set_state(lbx_ctx* ctx, const double* n, const double* rho,
                   const double* jx, const double* jy, const double* jz);
 

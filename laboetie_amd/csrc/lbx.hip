@@ -566,6 +566,9 @@ int lbx_set_state(lbx_ctx* c, const double* n, const double* rho, const double* 
                   const double* jz) {
     if (!c || !n || !rho || !jx || !jy || !jz) return LBX_ERR_ARG;
     if (c->prec == LBX_PREC_FP32_SHIFT) {
+        // the shift: the density of the first fluid node (the uniform rest
+        // density when the state is a rest state)
+        c->rho0 = rho[c->nodes_h[0] & kIndexMask];
         LBX_TRY(upload_shift(c, n, c->g[0]));
     } else {
         const double* planes[kQ];

@@ -8,7 +8,7 @@ module lbx_bind
 
     integer(c_int), parameter :: LBX_OK = 0, LBX_ERR_NEG_POP = 1, LBX_ERR_RESTPART = 2, LBX_ERR_TAU = 3, &
                                  LBX_ERR_NOMEM = 4, LBX_ERR_ARG = 5, LBX_ERR_DEVICE = 6, LBX_ERR_STATE = 7
-    integer(c_int32_t), parameter :: LBX_PREC_FP64 = 0
+    integer(c_int32_t), parameter :: LBX_PREC_FP64 = 0, LBX_PREC_FP32_SHIFT = 1
     integer(c_int32_t), parameter :: LBX_EXIT_CONVERGED = 1, LBX_EXIT_CAP = 2
 
     type(c_ptr), save :: lbx_lattice = c_null_ptr
@@ -143,18 +143,34 @@ contains
     end subroutine lbx_nature
 
     ! the process-wide lattice context, created on first use
+    ! The lattice, created on first use.  One lb.in key beyond the
+    ! reference's selects the precision form: gpu_precision = fp64 (default,
+    ! the reference's arithmetic) or fp32_shift.
     subroutine lbx_ensure_lattice(tau)
         use system, only: supercell
+        use module_input, only: getinput
         real(c_double), intent(in) :: tau
         integer(c_int8_t), allocatable :: nature(:)
         integer :: lx, ly, lz
+        integer(c_int32_t) :: prec
+        character(len=:), allocatable :: pname
         if (c_associated(lbx_lattice)) return
         lx = supercell%geometry%dimensions%indiceMax(1)
         ly = supercell%geometry%dimensions%indiceMax(2)
         lz = supercell%geometry%dimensions%indiceMax(3)
+        pname = trim(adjustl(getinput%char("gpu_precision", defaultvalue="fp64")))
+        select case (pname)
+        case ("fp64")
+            prec = LBX_PREC_FP64
+        case ("fp32_shift")
+            prec = LBX_PREC_FP32_SHIFT
+        case default
+            print *, "gpu_precision must be fp64 or fp32_shift, found: ", pname
+            stop
+        end select
         call lbx_nature(nature)
         call lbx_check(lbx_create(int(lx, c_int32_t), int(ly, c_int32_t), int(lz, c_int32_t), nature, tau, &
-                                  LBX_PREC_FP64, lbx_lattice), "lbx_create")
+                                  prec, lbx_lattice), "lbx_create")
     end subroutine lbx_ensure_lattice
 
 end module lbx_bind

@@ -59,3 +59,28 @@ def test_dropin_matches_reference_program(name, mp_steps):
         assert vr.shape == vg.shape == (mp_steps + 1, 4)
         assert np.array_equal(vr[:, 0], vg[:, 0])
         assert np.all(np.abs(vr[:, 1:] - vg[:, 1:]) <= 1e-12 * np.abs(vr[0, 1:]).max())
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_BIN) and os.path.exists(GPU_BIN)),
+                    reason="drop-in or reference binary not built (needs the reference sources at build time)")
+def test_dropin_fp32_shift_key():
+    """`gpu_precision = fp32_shift` in lb.in runs the same program in the
+    FP32-shift form: VACF within 1e-6 of VACF(0) of the reference program's
+    (FP64) and an unknown value stops with a message."""
+    g = load("bcc12_ads")
+    mp_steps = 300
+    lb_in = str(g["lb_in"]).replace(
+        [ln for ln in str(g["lb_in"]).splitlines() if ln.startswith("maximum_moment_propagation_steps")][0],
+        f"maximum_moment_propagation_steps = {mp_steps}")
+    with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg, \
+            tempfile.TemporaryDirectory() as db:
+        rc_r, out_r = run_case(REF_BIN, lb_in, str(g["geom_in"]), dr)
+        rc_g, out_g = run_case(GPU_BIN, lb_in + "\ngpu_precision = fp32_shift\n", str(g["geom_in"]), dg)
+        assert rc_r == 0 and rc_g == 0, out_g
+        vr = np.loadtxt(os.path.join(dr, "output", "vacf.dat"), comments="#")
+        vg = np.loadtxt(os.path.join(dg, "output", "vacf.dat"), comments="#")
+        assert vr.shape == vg.shape
+        scale = np.abs(vr[0, 1:]).max()
+        assert np.all(np.abs(vg[:, 1:] - vr[:, 1:]) <= 1e-6 * scale)
+        rc_b, out_b = run_case(GPU_BIN, lb_in + "\ngpu_precision = fp16\n", str(g["geom_in"]), db)
+        assert "gpu_precision must be fp64 or fp32_shift" in out_b
