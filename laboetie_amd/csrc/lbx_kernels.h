@@ -164,16 +164,27 @@ __device__ __forceinline__ E pull(const View<Wide, E>& fin, uint32_t Np, uint32_
 template <int BLOCK, bool Wide, typename E>
 __device__ __forceinline__ void lb_step_tile(const StepArgs<E>& a, uint32_t tile, uint32_t stripe, bool first_block) {
     constexpr bool kShift = std::is_same<E, float>::value;
-    if (block_flag(vload(&a.ctrl->stop) || vload(&a.ctrl->negpop) >= 0)) return;
+    const uint32_t i = tile * BLOCK + threadIdx.x;
+    // Each wave's work is a chain of dependent memory round trips, so
+    // independent requests go out together: the node word, the stop flags
+    // and the previous step's l2err slots (read whether or not a flag is set).
+    const uint32_t node = i < a.Nf ? ld(a.nodes, i) : 0u;
+    __shared__ int32_t stop_flag;
+    __shared__ unsigned long long prev_bits;
+    __syncthreads();  // a previous step of this block (persistent launches) has read both
+    if (threadIdx.x < 64) {
+        const bool stop = (vload(&a.ctrl->stop) != 0) | (vload(&a.ctrl->negpop) >= 0);
+        const unsigned long long v = a.have_prev ? slots_max(a.slots + ((a.t - 1) % 3) * kSlots) : 0ull;
+        if (threadIdx.x == 0) {
+            stop_flag = stop ? 1 : 0;
+            prev_bits = v;
+        }
+    }
+    __syncthreads();
+    if (stop_flag) return;
     if (a.have_prev) {
         // l2err of step t-1 (equilibration.f90:339-344) from its striped
         // slots, then the negativity (:248) and convergence (:346) verdicts
-        __shared__ unsigned long long prev_bits;
-        if (threadIdx.x < 64) {
-            const unsigned long long v = slots_max(a.slots + ((a.t - 1) % 3) * kSlots);
-            if (threadIdx.x == 0) prev_bits = v;
-        }
-        __syncthreads();
         const bool first = first_block && threadIdx.x == 0;
         if (prev_bits == kNegMark) {
             if (first) a.ctrl->negpop = a.t - 1;
@@ -214,14 +225,13 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E>& a, uint32_t tile
     }
     // clear the slots of step t+1 (last used by step t-2, read by step t-1)
     if (a.write_l2 && first_block && threadIdx.x < kSlots) a.slots[((a.t + 1) % 3) * kSlots + threadIdx.x] = 0ull;
-    const uint32_t i = tile * BLOCK + threadIdx.x;
     double l2 = 0.0;
     bool neg = false;
     if (i < a.Nf) {
         const uint32_t Np = (uint32_t)a.Np;
         const View<Wide, E> fin(a.fin, kQ * a.Np), fout(a.fout, kQ * a.Np), jv(a.j, 3 * a.Np);
         Neighbours nb;
-        nb.load(a.map, ld(a.nodes, i) & kIndexMask, a.dlx, a.dly, a.lz);
+        nb.load(a.map, node & kIndexMask, a.dlx, a.dly, a.lz);
         E jo[3] = {E(0), E(0), E(0)};
         if (a.write_l2) {  // j_old, issued before the populations so its latency overlaps theirs
             jo[0] = jv.ld(i);
@@ -715,12 +725,28 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     constexpr int WIN = BLOCK + 2 * kMpHalo;
     __shared__ double lds[BLOCK / 64 * K];
     __shared__ double pwin[K * WIN];  // P(now) of packed nodes [w0, w0 + WIN), comp_index layout
+    const uint32_t i = tile * BLOCK + threadIdx.x;
+    const bool valid = i < a.Nf;
+    // Each wave's work is a chain of dependent memory round trips (stop flag,
+    // node word, neighbour indices, neighbour values), so independent
+    // requests are issued together: the node word with the stop flag, the
+    // neighbour indices and the node's coefficients with the LDS window.
+    const uint32_t e = valid ? ld(a.nodes, i) : 0u;
     if (block_flag(vload(&a.ctrl->mp_stop) != 0)) return;
     if (a.prev_partials && first_block)
         mp_reduce_block<K, BLOCK>(a.prev_partials, nparts, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
-    const uint32_t i = tile * BLOCK + threadIdx.x;
     const uint32_t Np = (uint32_t)a.Np;
     const View<Wide> Pn(a.Pn, K * a.Np);
+    const View<Wide, W> win(a.win, 18 * a.Np), us(a.ustar, 3 * a.Np);
+    const View<Wide> rem(a.rem, a.Np);
+    Neighbours nb;
+    W uw[3] = {W(0), W(0), W(0)};
+    double rm = 0.0;
+    if (valid) {
+        nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
+        load_comps<3>(us, Np, i, uw);
+        rm = rem.ld(i);
+    }
     // The block's nodes and kMpHalo on either side: neighbours along x, and
     // along y where a row holds fewer fluid nodes than that, are read from LDS
     const int64_t w0 = (int64_t)tile * BLOCK - kMpHalo;
@@ -737,19 +763,12 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     double vp[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) vp[k] = 0.0;
-    if (i < a.Nf) {
-        const View<Wide, W> win(a.win, 18 * a.Np), us(a.ustar, 3 * a.Np);
-        const View<Wide> Px(a.Px, K * a.Np), rem(a.rem, a.Np);
-        const uint32_t e = ld(a.nodes, i);
-        Neighbours nb;
-        nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
+    if (valid) {
+        const View<Wide> Px(a.Px, K * a.Np);
         double p[K], acc[K];
         load_comps<K>(pw, WIN, (uint32_t)(kMpHalo + threadIdx.x), p);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] = 0.0;
-        W uw[3];
-        load_comps<3>(us, Np, i, uw);
-        const double rm = rem.ld(i);
         Vec2<W> wp;  // the probabilities of directions L, L+1 (L odd) from one paired load
         static_for<1, kQ>([&](auto I) {
             constexpr int L = I;
