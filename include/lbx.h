@@ -228,6 +228,17 @@ int lbx_kernel_time(lbx_ctx* ctx, int32_t which, double* total_ms, int64_t* laun
  * reject a non-zero window (LBX_ERR_ARG).  No counterpart in the reference. */
 int lbx_set_floor_window(lbx_ctx* ctx, int64_t steps);
 
+/* Element addressing of the step kernels.  The population and moment arrays
+ * are addressed through 32-bit buffer offsets while 19*Np*8 < 4 GiB (Np: the
+ * fluid-node count rounded up to 64), and through 64-bit pointers above that;
+ * the library picks the form from the size.  wide = 1 selects the 64-bit form
+ * on a smaller lattice too, so the two forms can be checked against each other
+ * (they give the same bits); wide = 0 returns to the size's choice.
+ * lbx_addressing reports the form in use (1: 64-bit).  No counterpart in the
+ * reference, whose arrays the Fortran runtime addresses. */
+int lbx_set_addressing(lbx_ctx* ctx, int32_t wide);
+int lbx_addressing(lbx_ctx* ctx);
+
 /* Fluid-node count of the context (the MFLUPS denominator). */
 int64_t lbx_fluid_nodes(lbx_ctx* ctx);
 
@@ -302,6 +313,10 @@ int lbxb_set_block(lbxb* b, int32_t lb_block, int32_t mp_block);
 #define LBXB_LAUNCH_PERSISTENT 2
 int lbxb_set_launch_mode(lbxb* b, int32_t mode);
 int lbxb_launch_mode(lbxb* b);
+/* Element addressing of the batch's step kernels, as lbx_set_addressing
+ * (the capacity decides by itself; wide = 1 forces the 64-bit form). */
+int lbxb_set_addressing(lbxb* b, int32_t wide);
+int lbxb_addressing(lbxb* b);
 /* Autotuner: times 128- and 256-node blocks per phase on the current state
  * without changing it and keeps the faster, then keeps the persistent mode
  * if it beats per-step launches in every phase; times[6] = mean ms per step

@@ -47,6 +47,7 @@ struct lbx_ctx {
     uint32_t N = 0, Nf = 0;
     uint64_t Np = 0;  // packed plane stride
     bool wide = false; // 19*Np*8 >= 2^32: 64-bit addressing of the population buffers
+    bool wide_auto = false;  // what the size alone selects
     int32_t prec = LBX_PREC_FP64;
     double tau = 1.0, A = 0, B = 0, C = 0;
     int64_t nfi = 0;
@@ -501,7 +502,7 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
         delete c;
         return LBX_ERR_ARG;
     }
-    c->wide = kQ * c->Np * 8 >= (1ull << 32);
+    c->wide = c->wide_auto = kQ * c->Np * 8 >= (1ull << 32);
     c->ntiles = nblocks(c->Nf);
     int st = LBX_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = LBX_ERR_DEVICE;
@@ -996,6 +997,14 @@ int lbx_set_floor_window(lbx_ctx* c, int64_t steps) {
     c->floor_window = steps;
     return LBX_OK;
 }
+
+int lbx_set_addressing(lbx_ctx* c, int32_t wide) {
+    if (!c || wide < 0 || wide > 1) return LBX_ERR_ARG;
+    c->wide = c->wide_auto || wide == 1;
+    return LBX_OK;
+}
+
+int lbx_addressing(lbx_ctx* c) { return c ? (c->wide ? 1 : 0) : -1; }
 
 int64_t lbx_fluid_nodes(lbx_ctx* c) { return c ? (int64_t)c->Nf : -1; }
 

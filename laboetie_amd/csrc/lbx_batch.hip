@@ -359,7 +359,8 @@ struct HostInst {
 
 struct lbxb {
     int32_t prec = LBX_PREC_FP64;
-    bool wide = false;
+    bool wide = false;       // 64-bit addressing of the population buffers
+    bool wide_auto = false;  // what the capacity alone selects
     uint64_t Np = 0, cap_box = 0;
     uint32_t cap_inst = 0, ntiles_cap = 0;
     int32_t lb_block = 256, mp_block = 256;
@@ -767,7 +768,7 @@ int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
     b->cap_box = (uint64_t)capacity_box;
     b->cap_inst = (uint32_t)max_instances;
     b->ntiles_cap = (uint32_t)(Np / kTile);
-    b->wide = kQ * Np * 8 >= (1ull << 32);
+    b->wide = b->wide_auto = kQ * Np * 8 >= (1ull << 32);
     {
         int dev = 0;
         (void)hipGetDevice(&dev);
@@ -1154,6 +1155,15 @@ int lbxb_set_launch_mode(lbxb* b, int32_t mode) {
     if (!b->graphs) clear_graphs(b);
     return LBX_OK;
 }
+
+int lbxb_set_addressing(lbxb* b, int32_t wide) {
+    if (!b || wide < 0 || wide > 1) return LBX_ERR_ARG;
+    b->wide = b->wide_auto || wide == 1;
+    clear_graphs(b);  // captured launches name the kernels of the previous form
+    return LBX_OK;
+}
+
+int lbxb_addressing(lbxb* b) { return b ? (b->wide ? 1 : 0) : -1; }
 
 int lbxb_launch_mode(lbxb* b) {
     if (!b) return -1;

@@ -57,6 +57,8 @@ SIGNATURES = {
     "lbx_timing_enable": (ctypes.c_int, [_p, _i32]),
     "lbx_kernel_time": (ctypes.c_int, [_p, _i32, _p, _p]),
     "lbx_set_floor_window": (ctypes.c_int, [_p, _i64]),
+    "lbx_set_addressing": (ctypes.c_int, [_p, _i32]),
+    "lbx_addressing": (ctypes.c_int, [_p]),
     "lbx_fluid_nodes": (_i64, [_p]),
     "lbx_sync": (ctypes.c_int, [_p]),
     "lbxb_footprint_bytes": (_i64, [_i32, _i64, _i64, _i32]),
@@ -70,6 +72,8 @@ SIGNATURES = {
     "lbxb_set_block": (ctypes.c_int, [_p, _i32, _i32]),
     "lbxb_autotune": (ctypes.c_int, [_p, _i32, _p, _p, _p]),
     "lbxb_set_launch_mode": (ctypes.c_int, [_p, _i32]),
+    "lbxb_set_addressing": (ctypes.c_int, [_p, _i32]),
+    "lbxb_addressing": (ctypes.c_int, [_p]),
     "lbxb_launch_mode": (ctypes.c_int, [_p]),
     "lbxb_stats": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
     "lbxb_free_tiles": (_i64, [_p]),
@@ -228,6 +232,16 @@ class Lattice:
         `steps` steps (0: off; default 1000 for FP32-shift contexts)."""
         _check(lib().lbx_set_floor_window(self._h, int(steps)), "set_floor_window")
 
+    def set_addressing(self, wide):
+        """1: the 64-bit addressing form of the step kernels on any size (the
+        library selects it by itself above 4 GiB of populations); 0: the
+        size's choice."""
+        _check(lib().lbx_set_addressing(self._h, int(wide)), "set_addressing")
+
+    @property
+    def addressing(self):
+        return int(lib().lbx_addressing(self._h))
+
     @property
     def fluid_nodes(self):
         return int(lib().lbx_fluid_nodes(self._h))
@@ -260,10 +274,12 @@ class Lattice:
             _check(lib().lbx_l2err_trace(self._h, 1, n.value, _ptr(out), None), "l2err_trace")
         return out
 
-    def get_state(self):
-        n = np.zeros(19 * self.N)
+    def get_state(self, populations=True):
+        """(n, rho, j) on the full box; n is None with populations=False."""
+        n = np.zeros(19 * self.N) if populations else None
         rho, jx, jy, jz = (np.zeros(self.N) for _ in range(4))
-        _check(lib().lbx_get_state(self._h, _ptr(n), _ptr(rho), _ptr(jx), _ptr(jy), _ptr(jz)), "get_state")
+        _check(lib().lbx_get_state(self._h, _ptr(n) if populations else None, _ptr(rho), _ptr(jx), _ptr(jy),
+                                   _ptr(jz)), "get_state")
         return n, rho, np.stack([jx, jy, jz])
 
     def mp_init(self, fext, Db, ka=0.0, kd=0.0, flow=None):
@@ -437,6 +453,14 @@ class Batch:
 
     def set_launch_mode(self, mode):
         _check(lib().lbxb_set_launch_mode(self._h, int(mode)), "lbxb_set_launch_mode")
+
+    def set_addressing(self, wide):
+        """As Lattice.set_addressing, for the batch's step kernels."""
+        _check(lib().lbxb_set_addressing(self._h, int(wide)), "lbxb_set_addressing")
+
+    @property
+    def addressing(self):
+        return int(lib().lbxb_addressing(self._h))
 
     @property
     def launch_mode(self):
