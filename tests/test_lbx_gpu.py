@@ -201,6 +201,52 @@ def test_capped_runs_match_restatement_at_larger_sizes(label, dims, tau, fext):
         assert np.array_equal(P, mp.P)
 
 
+@pytest.mark.parametrize("dims", [(1, 1, 9), (2, 3, 4), (1, 6, 5), (2, 2, 2), (5, 1, 3), (3, 2, 1)])
+def test_degenerate_axes_match_restatement(dims):
+    """Axes of length 1 (a node is its own neighbour) and 2 (x+1 and x-1 are
+    the same node) on seeded random media, against the restatement: LB and
+    MP with adsorption, bit for bit (VACF to 1e-12 of |VACF(0)|)."""
+    say("degenerate axes", dims)
+    rng = np.random.default_rng(sum(dims) * 7 + dims[0])
+    N = int(np.prod(dims))
+    nat = (rng.random(N) < 0.25).astype(np.int8)
+    nat[rng.integers(N)] = 0  # at least one fluid node
+    fext = [1e-5, -2e-5, 3e-5]
+    steps = 60
+    ref = lbref.equilibrate(dims, nat, 0.9, 1.0, fext, 1e-12, max_steps=steps)
+    with lbx.Lattice(dims, nat, 0.9) as L:
+        L.init_rest(1.0)
+        r = L.equilibrate(fext, 1e-12, max_steps=steps)
+        assert (r["steps"], r["exit_reason"]) == (ref["steps"], ref["exit_reason"])
+        assert np.array_equal(r["l2err"], ref["l2err"])
+        n, rho, j = L.get_state()
+        assert np.array_equal(n, ref["n"]) and np.array_equal(rho, ref["rho"]) and np.array_equal(j, ref["j"])
+        itf = lbref.interfacial(*dims, nat)
+        L.mp_init(fext, 0.01, 0.02, 0.01)
+        m = L.mp_run(25)
+        nt = lbref.tracer_population(dims, nat, rho, j, fext)
+        mp = lbref.MP(dims, nat, itf, nt, rho, 0.01, 0.02, 0.01)
+        st, vref = mp.run(25)
+        scale = np.abs(vref[0]).max()
+        assert vacf_close(m["vacf"], vref[1:], scale)
+        P, Pa = L.mp_get()
+        assert np.array_equal(P, mp.P)
+        if mp.ads:
+            assert np.array_equal(Pa, mp.Pa)
+
+
+def test_lattices_the_library_refuses():
+    """No fluid node, or a box of 2^29 nodes or more: LBX_ERR_ARG, before any
+    device memory is taken."""
+    with pytest.raises(lbx.LbxError) as e:
+        lbx.Lattice((3, 4, 5), np.ones(60, np.int8), 1.0)
+    assert e.value.status == lbx.ERR_ARG
+    big = np.zeros(1 << 29, np.int8)
+    with pytest.raises(lbx.LbxError) as e:
+        lbx.Lattice((1 << 10, 1 << 10, 1 << 9), big, 1.0)
+    assert e.value.status == lbx.ERR_ARG
+
+
 def test_full_size_invariants_bcc128():
     """128^3 BCC at full size: mass is conserved to rounding while the force
     acts, stage 1 ends at t = 3, and the tracer probability (free + adsorbed)
