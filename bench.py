@@ -16,10 +16,13 @@ also carries the roofline of the dominant kernel (algorithmic HBM bytes per
 launch / measured mean launch duration, HIP events on the library's stream)
 and the reference program timed on this machine's host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-fp32-shift]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-fp32-shift] [--no-large]
 
 The same job also runs in the FP32-shift precision form; its throughput,
-rooflines and D_eff error against the FP64 run go under "fp32_shift".
+rooflines and D_eff error against the FP64 run go under "fp32_shift".  The
+128^3 job's working set (~0.25 GB) fits the 256 MB MALL, so "roofline_256"
+adds the two kernels' rooflines on the 256^3 random medium (3 GB of FP64
+populations), outside the timed region.
 """
 import argparse
 import json
@@ -113,6 +116,54 @@ def fp32_shift_leg(w, nat, args, nf, nfi):
                             "frac": round(mp_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": round(mp_bytes),
                             "avg_launch_us": round(mp_ms / mp_n * 1e3, 2)},
             "deff": deffs}
+
+
+LARGE = dict(dims=(256, 256, 256), sigma=8.0, porosity=0.6, seed=20261202, tau=0.8, fext=(1e-6, 0.0, 0.0),
+             Db=0.01, kd=0.01, dF=-1.0, lb_steps=300, mp_steps=300)
+
+
+def large_leg():
+    """The kernels on a working set far above the 256 MB MALL: the 256^3
+    Gaussian random medium (correlation length 8, porosity 0.6), tau 0.8,
+    F = 1e-6 along x; LARGE["lb_steps"] LB steps from rest (the force on
+    from t = 4) and LARGE["mp_steps"] MP steps of one tracer (dF = -1 kT),
+    FP64 and FP32-shift, kernel times from HIP events."""
+    g = LARGE
+    nat = lbx.geometry_random(*g["dims"], g["sigma"], g["porosity"], g["seed"])
+    lx, ly, lz = g["dims"]
+    itf = lbx.detect_interfacial(lx, ly, lz, nat)
+    nfi = int(((nat == 0) & (itf == 1)).sum())
+    ka = lbx.ka_kd_from_dF(g["dF"], g["kd"])[0]
+    out = {"workload": f"Gaussian random medium 256^3 (sigma 8, porosity 0.6, seed {g['seed']}), tau 0.8, "
+                       f"F=(1e-6,0,0): {g['lb_steps']} LB steps from rest, {g['mp_steps']} MP steps of one "
+                       "tracer (dF=-1 kT)"}
+    for prec, name, lbb, coef in ((lbx.PREC_FP64, "fp64", LB_BYTES_PER_FLUID, MP_COEF_BYTES_PER_FLUID),
+                                  (lbx.PREC_FP32_SHIFT, "fp32_shift", LB32_BYTES_PER_FLUID,
+                                   MP32_COEF_BYTES_PER_FLUID)):
+        with lbx.Lattice(g["dims"], nat, g["tau"], precision=prec) as L:
+            if prec == lbx.PREC_FP32_SHIFT:
+                L.set_floor_window(0)
+            nf = L.fluid_nodes
+            L.init_rest(1.0)
+            L.timing(True)
+            L.equilibrate(g["fext"], 0.0, max_steps=g["lb_steps"], trace_cap=1)
+            L.mp_init(g["fext"], g["Db"], ka, g["kd"])
+            L.mp_run(g["mp_steps"])
+            L.sync()
+            lb_ms, lb_n = L.kernel_time(lbx.KERNEL_LB)
+            mp_ms, mp_n = L.kernel_time(lbx.KERNEL_MP)
+        lb_s, mp_s = lb_ms / lb_n / 1e3, mp_ms / mp_n / 1e3
+        mp_bytes = (coef + MP_P_BYTES_PER_FLUID) * nf + MP_BYTES_PER_ADS_INTERFACIAL * nfi
+        out[name] = {
+            "fluid_nodes": nf,
+            "roofline_lb": {"achieved": round(lbb * nf / lb_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(lbb * nf / lb_s / 1e9 / HBM_PEAK_GBS, 4), "avg_launch_us": round(lb_s * 1e6, 1)},
+            "roofline_mp": {"achieved": round(mp_bytes / mp_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(mp_bytes / mp_s / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_launch": round(mp_bytes),
+                            "avg_launch_us": round(mp_s * 1e6, 1)},
+            "lb_mlups": round(lx * ly * lz / lb_s / 1e6, 1), "mp_mlups": round(lx * ly * lz / mp_s / 1e6, 1)}
+    print("bench: 256^3 leg done", file=sys.stderr, flush=True)
+    return out
 
 
 def read_traffic():
@@ -250,6 +301,7 @@ def main():
     ap.add_argument("--no-fp32-shift", action="store_true", help="skip the FP32-shift leg")
     ap.add_argument("--mp-one-at-a-time", action="store_true",
                     help="propagate the dF tracers one after the other instead of together")
+    ap.add_argument("--no-large", action="store_true", help="skip the 256^3 roofline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain-copy bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -280,6 +332,7 @@ def main():
         lb_ms, lb_n = L.kernel_time(lbx.KERNEL_LB)
         mp_ms, mp_n = L.kernel_time(lbx.KERNEL_MP)
     shift = None if args.no_fp32_shift else fp32_shift_leg(w, nat, args, nf, nfi)
+    large = None if args.no_large else large_leg()
     copy_gbs = lbx.copy_bandwidth(2 << 30, 20)  # measured ceiling, outside the timed region
     lb_steps, mp_steps, deffs = results[-1]
     updates = N * sum(r[0] + r[1] for r in results)
@@ -313,7 +366,8 @@ def main():
                "per_tracer_step_achieved": round(mp_bytes_tr / mp_avg_s / 1e9, 1)}
     line = {
         "metric": "D3Q19+MP MLUPS & HBM roofline %, FP64 vs FP32-shift; p50/p99 solve latency",
-        "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": round(value, 2), "unit": "MLUPS", "mflups": round(value * nf / N, 2),
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "BCC touching spheres 128^3 (geometryLabel 3), FP64: equilibrate to "
@@ -335,6 +389,8 @@ def main():
         shift["deff_rel_error_vs_fp64"] = float(f"{np.abs(d32 - d64).max() / np.abs(d64).max():.3e}")
         shift["vs_fp64_value"] = round(shift["value"] / value, 3)
         line["fp32_shift"] = shift
+    if large is not None:
+        line["roofline_256"] = large
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, lb_steps, mp_steps)
     print(json.dumps(line), flush=True)
