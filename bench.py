@@ -17,12 +17,17 @@ launch / measured mean launch duration, HIP events on the library's stream)
 and the reference program timed on this machine's host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-fp32-shift] [--no-large]
+                    [--no-batch]
 
 The same job also runs in the FP32-shift precision form; its throughput,
 rooflines and D_eff error against the FP64 run go under "fp32_shift".  The
 128^3 job's working set (~0.25 GB) fits the 256 MB MALL, so "roofline_256"
 adds the two kernels' rooflines on the 256^3 random medium (3 GB of FP64
-populations), outside the timed region.
+populations), outside the timed region.  The other configurations run
+beside it, also outside the timed region: the 4x4x34 slit ("slit_4x4x34"),
+the 1024-instance batch ("batch_1024", FP64 and FP32-shift), and, in
+"cpu_baseline", the reference program on a bounded sample of each of them
+and of two serving requests.
 """
 import argparse
 import json
@@ -166,6 +171,53 @@ def large_leg():
     return out
 
 
+def slit_leg():
+    """The CPU-runnable configuration on the GPU: slit 4x4x34 (walls at z = 1
+    and 34), tau 1, F = 1e-5 along x, equilibrated to max|dj| <= 1e-10 (the
+    step count the reference's own loop takes), then 5000 MP steps of a
+    neutral tracer (Db = 0.01).  A 512-fluid-node lattice is launch-bound:
+    the figure is the whole job's wall rate."""
+    dims = (4, 4, 34)
+    nat = lbx.build_geometry(1, *dims)
+    with lbx.Lattice(dims, nat, 1.0) as L:
+        L.sync()
+        t0 = time.perf_counter()
+        L.init_rest(1.0)
+        r = L.equilibrate((1e-5, 0.0, 0.0), 1e-10, trace_cap=1)
+        L.mp_init((1e-5, 0.0, 0.0), 0.01)
+        m = L.mp_run(5000)
+        L.sync()
+        dt = time.perf_counter() - t0
+        d = L.mp_deff()
+    N = int(np.prod(dims))
+    return {"lattice": list(dims), "lb_steps": r["steps"], "mp_steps": m["steps"],
+            "mlups": round(N * (r["steps"] + m["steps"]) / dt / 1e6, 2), "seconds": round(dt, 3),
+            "deff": d.tolist()}
+
+
+def batch_leg(n=1024):
+    """The batching configuration: n chromatography-style 2-D instances
+    (1x50x50 / 1x100x100, 1-8 disks of radius 5, F = 1e-5 along y, dF
+    uniform in [-3, 0] kT, 2000 MP steps each) admitted at once into the
+    batch engine and run to completion, FP64 and FP32-shift."""
+    from laboetie_amd import serving
+
+    reqs = serving.chromatography_requests(n)
+    out = {"workload": f"{n} chromatography 2-D instances (1x50x50 / 1x100x100, 1-8 disks r=5), tau 1, "
+                       "F=1e-5 along y, target 1e-10 (cap 20000), 2000 MP steps, dF U[-3,0]"}
+    deffs = {}
+    for prec, name in ((lbx.PREC_FP64, "fp64"), (lbx.PREC_FP32_SHIFT, "fp32_shift")):
+        r = serving.run_batch(reqs, prec)
+        deffs[name] = np.array([x["deff"] for x in r["results"]])
+        out[name] = {"mlups": round(r["mlups"], 1), "mflups": round(r["mflups"], 1), "wall_s": round(r["wall_s"], 2),
+                     "lb_kernel_mlups": round(r["lb_kernel_mlups"], 1), "mp_kernel_mlups": round(r["mp_kernel_mlups"], 1)}
+    err = np.abs(deffs["fp32_shift"] - deffs["fp64"]).max(axis=1) / np.abs(deffs["fp64"]).max(axis=1)
+    out["deff_rel_error_fp32_shift"] = {"median": float(f"{np.median(err):.3e}"), "max": float(f"{err.max():.3e}")}
+    out["fp32_shift_vs_fp64"] = round(out["fp32_shift"]["mlups"] / out["fp64"]["mlups"], 3)
+    print("bench: batch leg done", file=sys.stderr, flush=True)
+    return out
+
+
 def read_traffic():
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), as committed in profiles/traffic_bcc128.txt lines
@@ -182,7 +234,7 @@ def read_traffic():
 
 def lb_in_text(w, fext, mp_steps, ka, print_frequency=1):
     lx, ly, lz = w["dims"]
-    return (f"lx = {lx}\nly = {ly}\nlz = {lz}\ngeometryLabel = {w['label']}\n"
+    return (f"lx = {lx}\nly = {ly}\nlz = {lz}\ngeometryLabel = {w.get('label', 0)}\n"
             f"relaxation_time = {w['tau']}\ninitialSolventDensity = {w['rho0']}\n"
             f"f_ext = {fext[0]} {fext[1]} {fext[2]}\ntarget_error = {w['target']}\n"
             f"print_frequency = {print_frequency}\nmaximum_moment_propagation_steps = {mp_steps}\n"
@@ -256,7 +308,84 @@ def _mp_steps_per_second(exe, d, threads, nsteps):
     return nsteps / (t_end - t_init)
 
 
-def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0, mp_sample=30):
+def geom_in_text(dims, nat):
+    """geom.in for geometryLabel 0: one 1-based `i j k` line per solid node
+    (module_geometry.f90:380-426)."""
+    lx, ly, lz = dims
+    z, y, x = np.nonzero(np.asarray(nat).reshape(lz, ly, lx) != 0)
+    ijk = np.stack([x + 1, y + 1, z + 1], axis=1)
+    return "\n".join(f"{a} {b} {c}" for a, b, c in ijk.tolist()) + "\n"
+
+
+def _ref_rates(exe, threads, w, nat, lb_window, mp_sample):
+    """The reference's LB steps/s (per-step stdout lines over lb_window
+    seconds) and MP steps/s (mp_sample steps on the flow at rest) on one
+    lattice; geometryLabel 0 lattices get their geom.in from nat."""
+    geom = geom_in_text(w["dims"], nat) if w.get("label", 0) == 0 else None
+
+    def setup(d, text):
+        open(os.path.join(d, "lb.in"), "w").write(text)
+        if geom is not None:
+            open(os.path.join(d, "geom.in"), "w").write(geom)
+
+    with tempfile.TemporaryDirectory() as d:
+        setup(d, lb_in_text(w, w["fext"], 0, 0.0))
+        lb_rate = _lb_steps_per_second(exe, d, threads, lb_window)
+    with tempfile.TemporaryDirectory() as d:
+        setup(d, lb_in_text(w, (0.0, 0.0, 0.0), mp_sample, w.get("ka", 0.0)))
+        mp_rate = _mp_steps_per_second(exe, d, threads, mp_sample)
+    return lb_rate, mp_rate
+
+
+def cpu_configs(exe, threads):
+    """The reference on a bounded sample of each other configuration, on the
+    same host cores: node-update rates of its LB and MP phases (lattice nodes
+    x steps / s), which the GPU figures of those configurations are read
+    beside (the slit and 256^3 legs of this bench, the batch and serving
+    tools).  Returns {config: {...}}."""
+    from laboetie_amd import serving
+
+    out = {}
+    # slit 4x4x34, tau 1, F = 1e-5 along x, neutral tracer
+    w1 = dict(label=1, dims=(4, 4, 34), tau=1.0, rho0=1.0, fext=(1e-5, 0.0, 0.0), target=1e-10, Db=0.01,
+              kd=0.0, ka=0.0)
+    jobs = [("slit_4x4x34", w1, None, 3.0, 2000)]
+    # the 256^3 Gaussian random medium (tau 0.8, F = 1e-6 along x, dF = -1 kT), sampled by its
+    # 128^3 corner: the reference spends minutes reading a 256^3 geom.in and setting it up,
+    # and its per-node step cost does not depend on the lattice size
+    g = LARGE
+    lx3, ly3, lz3 = g["dims"]
+    nat3 = lbx.geometry_random(*g["dims"], g["sigma"], g["porosity"], g["seed"]).reshape(lz3, ly3, lx3)
+    nat3 = np.ascontiguousarray(nat3[:lz3 // 2, :ly3 // 2, :lx3 // 2]).ravel()
+    w3 = dict(label=0, dims=(lx3 // 2, ly3 // 2, lz3 // 2), tau=g["tau"], rho0=1.0, fext=g["fext"], target=1e-10,
+              Db=g["Db"], kd=g["kd"], ka=lbx.ka_kd_from_dF(g["dF"], g["kd"])[0])
+    jobs.append(("grf_256_corner_128", w3, nat3, 8.0, 10))
+    # chromatography instances: one 1x50x50 and one 1x100x100 of the batch workload
+    for r in serving.chromatography_requests(2):
+        wr = dict(label=0, dims=r["dims"], tau=r["tau"], rho0=r["rho0"], fext=r["fext"], target=r["target"],
+                  Db=r["Db"], kd=r["kd"], ka=r["ka"])
+        jobs.append((f"chromatography_1x{r['dims'][1]}x{r['dims'][2]}", wr, r["nature"], 2.0, 500))
+    # two requests of the serving stream (its first draws)
+    rng = np.random.default_rng(20261202)
+    for k in range(2):
+        r = serving.serving_request(rng)
+        wr = dict(label=0, dims=r["dims"], tau=r["tau"], rho0=r["rho0"], fext=r["fext"], target=r["target"],
+                  Db=r["Db"], kd=r["kd"], ka=r["ka"])
+        jobs.append((f"serving_{r['kind']}_{r['dims'][0]}", wr, r["nature"], 5.0, 20))
+    for name, w, nat, window, mp_sample in jobs:
+        print(f"cpu_baseline: reference on {name}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        lb_rate, mp_rate = _ref_rates(exe, threads, w, nat, window, mp_sample)
+        N = int(np.prod(w["dims"]))
+        out[name] = {"lattice": list(w["dims"]),
+                     "lb_mlups": round(N * lb_rate / 1e6, 4) if lb_rate else None,
+                     "mp_mlups": round(N * mp_rate / 1e6, 4) if mp_rate else None,
+                     "sample": f"LB steps over {window:.0f} s, {mp_sample} MP steps",
+                     "wall_s": round(time.perf_counter() - t0, 1)}
+    return out
+
+
+def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0, mp_sample=30, other_configs=True):
     """The reference program (oracle/_ref/laboetie_fast: flang -O3 -fopenmp
     -funroll-loops, the reference Makefile's flags) on this machine's host
     cores, timed on a bounded sample of the same workload: the LB step rate
@@ -283,13 +412,20 @@ def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0, mp_sample=30):
     if not lb_rate or not mp_rate:
         return None
     t_job = lb_steps / lb_rate + mp_steps_total / mp_rate
+    print("cpu_baseline: the reference's LB phase on one core", file=sys.stderr, flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, w["fext"], 0, 0.0))
+        lb_rate1 = _lb_steps_per_second(exe, d, 1, window)
+    configs = cpu_configs(exe, threads) if other_configs else None
     return {"value": round(N * (lb_steps + mp_steps_total) / t_job / 1e6, 4), "unit": "MLUPS",
             "cores": threads, "kind": "reference",
             "sample": (f"reference laboetie (flang -O3 -fopenmp -funroll-loops), OMP_NUM_THREADS={threads}: "
                        f"LB {lb_rate:.3f} steps/s over {window:.0f} s of the 128^3 BCC run, MP "
                        f"{mp_rate:.3f} steps/s over {mp_sample} steps (dF=-1 tracer), combined over this "
                        f"job's {lb_steps} LB + {mp_steps_total} MP steps"),
-            "lb_mlups": round(N * lb_rate / 1e6, 4), "mp_mlups": round(N * mp_rate / 1e6, 4)}
+            "lb_mlups": round(N * lb_rate / 1e6, 4), "mp_mlups": round(N * mp_rate / 1e6, 4),
+            "lb_mlups_1_core": round(N * lb_rate1 / 1e6, 4) if lb_rate1 else None,
+            "other_configs": configs}
 
 
 def main():
@@ -302,6 +438,7 @@ def main():
     ap.add_argument("--mp-one-at-a-time", action="store_true",
                     help="propagate the dF tracers one after the other instead of together")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 roofline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 1024-instance batch leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain-copy bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -331,8 +468,14 @@ def main():
         dt = time.perf_counter() - t0
         lb_ms, lb_n = L.kernel_time(lbx.KERNEL_LB)
         mp_ms, mp_n = L.kernel_time(lbx.KERNEL_MP)
+        # Darcy permeability of the converged flow, k = nu <j_z>_box / F_z (lattice units)
+        _, _, jf = L.get_state(populations=False)
+        permeability = (w["tau"] - 0.5) / 3.0 * float(jf[2][nat == 0].sum()) / N / w["fext"][2]
+        del jf
     shift = None if args.no_fp32_shift else fp32_shift_leg(w, nat, args, nf, nfi)
     large = None if args.no_large else large_leg()
+    slit = slit_leg()
+    batch = None if args.no_batch else batch_leg()
     copy_gbs = lbx.copy_bandwidth(2 << 30, 20)  # measured ceiling, outside the timed region
     lb_steps, mp_steps, deffs = results[-1]
     updates = N * sum(r[0] + r[1] for r in results)
@@ -382,6 +525,7 @@ def main():
         "mp_mlups": round(N * mp_steps * args.steps / (mp_ms / 1e3) / 1e6, 1),
         "mp_tracers_together": per_launch,
         "deff": deffs,
+        "permeability": permeability,
     }
     if shift is not None:
         d64 = np.array(deffs)
@@ -389,6 +533,9 @@ def main():
         shift["deff_rel_error_vs_fp64"] = float(f"{np.abs(d32 - d64).max() / np.abs(d64).max():.3e}")
         shift["vs_fp64_value"] = round(shift["value"] / value, 3)
         line["fp32_shift"] = shift
+    line["slit_4x4x34"] = slit
+    if batch is not None:
+        line["batch_1024"] = batch
     if large is not None:
         line["roofline_256"] = large
     if not args.no_cpu_baseline:
