@@ -446,7 +446,7 @@ def main():
         print(json.dumps({"copy_gbs": round(lbx.copy_bandwidth(1 << 30, 20), 1), "bytes_each_way": 1 << 30}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the bench runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
     w = WORKLOAD

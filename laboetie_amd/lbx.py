@@ -172,8 +172,8 @@ KD_DEFAULT = 0.01
 def ka_kd_from_dF(dF, kd=KD_DEFAULT):
     """Adsorption/desorption probabilities per step for an adsorption free
     energy dF (in kT).  The reference takes ka and kd directly
-    (module_moment_propagation.f90:41-56, K = ka/kd); this build fixes
-    kd = 0.01 and sets K = exp(-dF), so ka = kd exp(-dF) (dF in [-3, 0]
+    (module_moment_propagation.f90:41-56, K = ka/kd); here kd is fixed
+    (default 0.01) and K = exp(-dF), so ka = kd exp(-dF) (dF in [-3, 0]
     gives ka in [0.01, 0.201])."""
     return kd * float(np.exp(-dF)), kd
 

@@ -20,7 +20,8 @@ Two workloads:
   latency is arrival -> result.
 
     python -m laboetie_amd.serving batch [--instances 1024] [--precision fp64|fp32|both]
-    python -m laboetie_amd.serving serve [--requests 48] [--rates 0.5,1,2]
+    python -m laboetie_amd.serving serve [--requests 48] [--rates 0.5,1,2] [--hold 5]
+                                         [--capacity-nodes N] [--capacity-box N]
 """
 import argparse
 import json
@@ -210,6 +211,11 @@ def main(argv=None):
     ap.add_argument("--rates", default="0.5,1,2")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--single-sample", type=int, default=16, help="requests timed on the plain FP64 path")
+    ap.add_argument("--hold", type=float, default=5.0,
+                    help="seconds a request that does not fit may be overtaken by later ones (0: first come, "
+                         "first served)")
+    ap.add_argument("--capacity-nodes", type=int, default=48 << 20, help="serve: fluid-node budget of the batch")
+    ap.add_argument("--capacity-box", type=int, default=64 << 20, help="serve: lattice-node budget of the batch")
     args = ap.parse_args(argv)
     if args.mode == "batch":
         reqs = chromatography_requests(args.instances, args.seed)
@@ -242,23 +248,24 @@ def main(argv=None):
     rng = np.random.default_rng(args.seed)
     reqs = [serving_request(rng) for _ in range(args.requests)]
     prec = _prec("fp32_shift" if args.precision == "both" else args.precision)
-    # HBM budget: 48 M fluid nodes (~20 GB in the FP32-shift form), room for
-    # several of the largest requests at once
-    cap_nodes, cap_box = 48 << 20, 64 << 20
+    # HBM budget (default 48 M fluid nodes, ~20 GB in the FP32-shift form: room
+    # for several of the largest requests at once)
+    cap_nodes, cap_box = args.capacity_nodes, args.capacity_box
     # saturation throughput: a closed batch of the first requests, all admitted at t = 0
     sat_reqs = reqs[:12]
-    t = serve(sat_reqs, np.zeros(len(sat_reqs)), prec, cap_nodes, cap_box)
+    t = serve(sat_reqs, np.zeros(len(sat_reqs)), prec, cap_nodes, cap_box, hold=args.hold)
     sat_rate = len(sat_reqs) / t["wall_s"]
     out = {"workload": f"{args.requests} requests per rate: N^3, N log-uniform in [32,192], slit/cylinder/BCC/"
                        "random(phi U[0.4,0.8]), f_ext log-uniform [1e-7,1e-5], target 1e-9 (cap 10000 LB steps), "
                        "2000 MP steps, dF U[-3,0]",
            "precision": "fp32_shift" if prec == lbx.PREC_FP32_SHIFT else "fp64",
-           "capacity_nodes": cap_nodes, "saturation_req_per_s": round(sat_rate, 3),
+           "capacity_nodes": cap_nodes, "capacity_box": cap_box, "hold_s": args.hold,
+           "saturation_req_per_s": round(sat_rate, 3),
            "saturation_mlups": round(t["mlups"], 1)}
     for f in [float(x) for x in args.rates.split(",")]:
         rate = f * sat_rate
         arr = np.cumsum(np.random.default_rng(args.seed + int(f * 1000)).exponential(1.0 / rate, len(reqs)))
-        s = serve(reqs, arr, prec, cap_nodes, cap_box)
+        s = serve(reqs, arr, prec, cap_nodes, cap_box, hold=args.hold)
         out[f"rate_{f:g}x"] = dict(_percentiles(s["latency"]), req_per_s=round(rate, 3), wall_s=round(s["wall_s"], 1),
                                    mlups=round(s["mlups"], 1), failed=int((s["status"] != 0).sum()))
         print(f"serving: rate {f:g}x done", file=sys.stderr, flush=True)

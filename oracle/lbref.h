@@ -2,8 +2,8 @@
  *
  * lbref: scalar C restatement of laboetie's lattice-Boltzmann + moment-
  * propagation path (maxlevesque/laboetie, Fortran).  It is the checker the
- * tests compare the HIP library against on the GPU box, where the reference
- * itself is not available; its own correctness is pinned against fixtures
+ * GPU tests compare the HIP library against, on machines where the reference
+ * sources are not available; its own correctness is pinned against fixtures
  * written by the reference program (tests/golden) and against closed forms.
  *
  * Layouts follow the reference exactly (Fortran column-major, x fastest):

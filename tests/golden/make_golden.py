@@ -8,7 +8,7 @@ container has the reference sources needed to build oracle/_ref; the fixtures
 are committed so that tests run anywhere.
 
     python tests/golden/make_golden.py          # all cases
-    python tests/golden/make_golden.py cfg1_slit
+    python tests/golden/make_golden.py slit_4x4x34
 """
 import os
 import subprocess
@@ -30,8 +30,8 @@ def lb_in(d):
 
 # Flow + tracer cases run through `ref_dump full`.  Keys are lb.in tags.
 FULL = {
-    # 4x4x34 slit, tau 1, F = 1e-5 along x, neutral tracer
-    "cfg1_slit": dict(lb=dict(lx=4, ly=4, lz=34, geometryLabel=1, relaxation_time=1.0,
+    # the CPU-runnable case: 4x4x34 slit, tau 1, F = 1e-5 along x, neutral tracer
+    "slit_4x4x34": dict(lb=dict(lx=4, ly=4, lz=34, geometryLabel=1, relaxation_time=1.0,
                               initialSolventDensity=1.0, f_ext="1e-5 0.0 0.0", target_error=1e-10,
                               maximum_moment_propagation_steps=5000, tracer_Db=0.01)),
     # BCC of touching spheres (label 3), flow along z, adsorbing tracer (dF = -1 kT)

@@ -6,7 +6,7 @@ import numpy as np
 from laboetie_amd.lbin import LbIn, read_geom_in
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-FULL_CASES = ["cfg1_slit", "bcc12_ads", "cyl9_tau08", "chromat_2disks", "random_3d", "shipped_geom"]
+FULL_CASES = ["slit_4x4x34", "bcc12_ads", "cyl9_tau08", "chromat_2disks", "random_3d", "shipped_geom"]
 
 
 def load(name):

@@ -17,7 +17,7 @@ from laboetie_amd import lbx
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["bcc12_ads", "cyl9_tau08", "random_3d", "chromat_2disks", "cfg1_slit"]
+CASES = ["bcc12_ads", "cyl9_tau08", "random_3d", "chromat_2disks", "slit_4x4x34"]
 
 
 def say(*a):
@@ -151,7 +151,7 @@ def test_served_requests_equal_alone():
 def test_batch_persistent_launches_same_bits(prec):
     """A round's steps as one cooperative launch per phase (grid-wide barrier
     between steps) give the same bits as per-step launches."""
-    cases = [Case(n) for n in ["bcc12_ads", "random_3d", "cyl9_tau08", "cfg1_slit"]]
+    cases = [Case(n) for n in ["bcc12_ads", "random_3d", "cyl9_tau08", "slit_4x4x34"]]
     mp_steps = 90
     ref, _ = run_batch(cases, prec, mp_steps)
     cap = sum(((int((c.nature == 0).sum()) + 255) // 256) * 256 for c in cases) + 4096

@@ -38,7 +38,7 @@ def run_case(exe, lb_in, geom_in, d):
 
 @pytest.mark.skipif(not (os.path.exists(REF_BIN) and os.path.exists(GPU_BIN)),
                     reason="drop-in or reference binary not built (needs the reference sources at build time)")
-@pytest.mark.parametrize("name,mp_steps", [("cfg1_slit", 600), ("bcc12_ads", 400), ("chromat_2disks", 200)])
+@pytest.mark.parametrize("name,mp_steps", [("slit_4x4x34", 600), ("bcc12_ads", 400), ("chromat_2disks", 200)])
 def test_dropin_matches_reference_program(name, mp_steps):
     g = load(name)
     lb_in = str(g["lb_in"]).replace(

@@ -132,10 +132,10 @@ def test_rest_state_stage_one_exits_at_t3_for_any_geometry():
         assert r["l2err"][3] > 0.0  # the force acts from t = 4
 
 
-def test_poiseuille_second_difference_cfg1():
+def test_poiseuille_second_difference_slit():
     """Steady plane Poiseuille: u(z+1) - 2u(z) + u(z-1) = -F/(rho nu),
-    nu = (tau - 1/2)/3, at interior nodes (cfg 1: -6e-5)."""
-    c = Case("cfg1_slit")
+    nu = (tau - 1/2)/3, at interior nodes (the 4x4x34 slit, F = 1e-5, tau 1: -6e-5)."""
+    c = Case("slit_4x4x34")
     lx, ly, lz = c.dims
     ux = (c.g["jx"] / np.where(c.g["rho"] > 0, c.g["rho"], 1)).reshape(lz, ly, lx)[:, 0, 0]
     d2 = ux[3:lz - 3] - 2 * ux[2:lz - 4] + ux[1:lz - 5]
@@ -158,9 +158,9 @@ def test_bulk_tracer_diffuses_at_Db():
 
 
 def test_slit_transverse_vacf0_closed_form():
-    """cfg 1 (no adsorption): D_y(0) = Db (N_bulk + 5/6 N_int)/N_f, so
+    """The 4x4x34 slit (no adsorption): D_y(0) = Db (N_bulk + 5/6 N_int)/N_f, so
     VACF_y(0) = 2 Db (480 + 32*5/6)/512."""
-    c = Case("cfg1_slit")
+    c = Case("slit_4x4x34")
     assert np.isclose(c.g["vacf"][0, 1], 2 * 0.01 * (480 + 32 * 5 / 6) / 512, rtol=1e-13)
 
 

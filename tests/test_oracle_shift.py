@@ -15,7 +15,7 @@ import pytest
 import lbref
 from golden_cases import Case
 
-CASES = ["cfg1_slit", "bcc12_ads", "cyl9_tau08", "random_3d"]
+CASES = ["slit_4x4x34", "bcc12_ads", "cyl9_tau08", "random_3d"]
 
 
 @pytest.fixture(scope="module")
@@ -96,6 +96,6 @@ def test_shift_bulk_deff_equals_db():
 def test_shift_floor_exit_on_strong_flow(runs):
     """The strong slit flow reaches the FP32 rounding floor (l2err cycling
     above 1e-10): the floor test ends it instead of running forever."""
-    c, a, b = runs["cfg1_slit"]
+    c, a, b = runs["slit_4x4x34"]
     assert b["exit_reason"] == lbref.EXIT_FLOOR
     assert b["l2err"][-1] > c.target

@@ -27,7 +27,7 @@ SHIFT = lbx.PREC_FP32_SHIFT
 
 
 @pytest.mark.parametrize("name,cap", [("bcc12_ads", 0), ("random_3d", 1600), ("cyl9_tau08", 1400),
-                                      ("cfg1_slit", 12000)])
+                                      ("slit_4x4x34", 12000)])
 def test_shift_matches_its_restatement(name, cap):
     c = Case(name)
     say("shift", name, c.dims, cap)
