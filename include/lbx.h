@@ -45,11 +45,14 @@ extern "C" {
 #define LBX_PREC_FP64        0  /* populations and arithmetic in FP64 (parity form) */
 #define LBX_PREC_FP32_SHIFT  1  /* populations stored as FP32 f_l - w_l*rho0, FP32 collision,
                                    FP64 moments/reductions/VACF/D_eff */
+#define LBX_PREC_FP32_STORE  2  /* the same FP32 storage, collision and j_old in FP64 (half the
+                                   FP64 form's population bytes at a fraction of the FP32-shift
+                                   form's error) */
 
 /* lbx_equilibrate exit reasons */
 #define LBX_EXIT_CONVERGED 1    /* second convergence, equilibration.f90:373-374 */
 #define LBX_EXIT_CAP       2    /* max_steps reached (extension: the reference has no cap) */
-#define LBX_EXIT_FLOOR     3    /* FP32-shift only: l2err stopped improving above target_error
+#define LBX_EXIT_FLOOR     3    /* FP32-storage forms only: l2err stopped improving above target_error
                                    (the form's rounding floor, lbx_set_floor_window) */
 
 typedef struct lbx_ctx lbx_ctx;
@@ -119,8 +122,8 @@ int lbx_init_rest(lbx_ctx* ctx, double rho0);
 
 /* Arbitrary state: n (post-streaming populations, (lx,ly,lz,19)), and the
  * density and flux that the next collision uses (collide's arguments).  In
- * the FP32-shift form the shift rho0 becomes the density of the first fluid
- * node (lbx_init_rest sets it to its rho0). */
+ * the FP32-storage forms the rest density rho0 of the deviations becomes the
+ * density of the first fluid node (lbx_init_rest sets it to its rho0). */
 int lbx_set_state(lbx_ctx* ctx, const double* n, const double* rho,
                   const double* jx, const double* jy, const double* jz);
 
@@ -219,13 +222,14 @@ int lbx_collide(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double
 int lbx_timing_enable(lbx_ctx* ctx, int32_t on);
 int lbx_kernel_time(lbx_ctx* ctx, int32_t which, double* total_ms, int64_t* launches);
 
-/* FP32-shift form: a stage of lbx_equilibrate also ends when l2err has not
- * set a new minimum for `steps` consecutive steps after falling at least
+/* FP32-storage forms: a stage of lbx_equilibrate also ends when l2err has
+ * not set a new minimum for `steps` consecutive steps after falling at least
  * 100-fold below the stage's largest l2err (its FP32 rounding floor;
  * a stage-1 floor switches the force on like a convergence, a stage-2 floor
  * exits with LBX_EXIT_FLOOR).  0 disables the test.  Default 1000 for
- * FP32-shift contexts; FP64 contexts keep the reference's semantics and
- * reject a non-zero window (LBX_ERR_ARG).  No counterpart in the reference. */
+ * FP32-shift and FP32-store contexts; FP64 contexts keep the reference's
+ * semantics and reject a non-zero window (LBX_ERR_ARG).  No counterpart in
+ * the reference. */
 int lbx_set_floor_window(lbx_ctx* ctx, int64_t steps);
 
 /* Element addressing of the step kernels.  The population and moment arrays

@@ -51,13 +51,13 @@ struct lbx_ctx {
     int32_t prec = LBX_PREC_FP64;
     double tau = 1.0, A = 0, B = 0, C = 0;
     int64_t nfi = 0;
-    int64_t floor_window = 0;    // FP32-shift l2err floor test (0: off)
+    int64_t floor_window = 0;    // FP32-storage forms: l2err floor test (0: off)
     int device = 0;
     hipStream_t stream = nullptr;
     double* f[2] = {nullptr, nullptr};  // FP64 populations (ping-pong)
-    float* g[2] = {nullptr, nullptr};   // FP32-shift: deviations g = n - a0 rho0 (ping-pong)
+    float* g[2] = {nullptr, nullptr};   // FP32 storage: deviations g = n - a0 rho0 (ping-pong)
     float* j32 = nullptr;               // FP32-shift: the flux of the last step (j_old)
-    double rho0 = 1.0;                  // FP32-shift: the rest density of the shift
+    double rho0 = 1.0;                  // FP32 storage: the rest density of the deviations
     double* j = nullptr;                // FP64 flux (FP32-shift: of the final state only)
     double* rho = nullptr;
     uint32_t* nodes = nullptr;
@@ -73,7 +73,7 @@ struct lbx_ctx {
     bool mp_ready = false;
     double* win = nullptr;       // FP64 form
     double* ustar = nullptr;
-    float* win32 = nullptr;      // FP32-shift form
+    float* win32 = nullptr;      // FP32-storage forms
     float* ustar32 = nullptr;
     double* rem = nullptr;
     double* P[2] = {nullptr, nullptr};
@@ -94,6 +94,11 @@ struct lbx_ctx {
     std::vector<double> l2host;
     std::vector<double> l2trace;  // every l2err of the last lbx_equilibrate call
 };
+
+// the forms that store populations as FP32 deviations (FP32-shift: FP32
+// collision; FP32-store: FP64 collision), whose moment propagation keeps its
+// probabilities in FP32
+inline bool f32_storage(int32_t prec) { return prec == LBX_PREC_FP32_SHIFT || prec == LBX_PREC_FP32_STORE; }
 
 namespace {
 
@@ -224,13 +229,13 @@ E* pop_buf(lbx_ctx* c, int b) {
     if constexpr (std::is_same<E, float>::value) return c->g[b]; else return c->f[b];
 }
 
-template <typename E>
-StepArgs<E> step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3], int64_t check_after,
-                      double target, int write_l2) {
-    StepArgs<E> a;
+template <typename E, typename R = E>
+StepArgs<E, R> step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3], int64_t check_after,
+                         double target, int write_l2) {
+    StepArgs<E, R> a;
     a.fin = pop_buf<E>(c, (t - 1) & 1);
     a.fout = pop_buf<E>(c, t & 1);
-    if constexpr (std::is_same<E, float>::value) a.j = c->j32; else a.j = c->j;
+    if constexpr (std::is_same<R, float>::value) a.j = c->j32; else a.j = c->j;
     a.nodes = c->nodes;
     a.map = c->map;
     a.slots = c->slots;
@@ -241,16 +246,16 @@ StepArgs<E> step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc
     a.lz = c->lz;
     a.Nf = c->Nf;
     a.Np = c->Np;
-    a.A = (E)c->A;
-    a.B = (E)c->B;
-    a.C = (E)c->C;
-    a.fmx = (E)(fm[0] / 2.0);
-    a.fmy = (E)(fm[1] / 2.0);
-    a.fmz = (E)(fm[2] / 2.0);
-    a.fcx = (E)fc[0];
-    a.fcy = (E)fc[1];
-    a.fcz = (E)fc[2];
-    a.rho0 = (E)c->rho0;
+    a.A = (R)c->A;
+    a.B = (R)c->B;
+    a.C = (R)c->C;
+    a.fmx = (R)(fm[0] / 2.0);
+    a.fmy = (R)(fm[1] / 2.0);
+    a.fmz = (R)(fm[2] / 2.0);
+    a.fcx = (R)fc[0];
+    a.fcy = (R)fc[1];
+    a.fcz = (R)fc[2];
+    a.rho0 = (R)c->rho0;
     a.t = t;
     a.check_after = check_after;
     a.floor_window = c->floor_window;
@@ -268,6 +273,11 @@ void launch_lb_step(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3
         StepArgs<float> a = step_args<float>(c, t, fm, fc, check_after, target, write_l2);
         a.have_prev = have_prev;
         auto kern = c->wide ? k_lb_step<true, float> : k_lb_step<false, float>;
+        hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
+    } else if (c->prec == LBX_PREC_FP32_STORE) {
+        StepArgs<float, double> a = step_args<float, double>(c, t, fm, fc, check_after, target, write_l2);
+        a.have_prev = have_prev;
+        auto kern = c->wide ? k_lb_step<true, float, double> : k_lb_step<false, float, double>;
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
     } else {
         StepArgs<double> a = step_args<double>(c, t, fm, fc, check_after, target, write_l2);
@@ -447,9 +457,8 @@ int64_t lbx_footprint_bytes(int32_t lx, int32_t ly, int32_t lz, int32_t precisio
     if (lx <= 0 || ly <= 0 || lz <= 0) return -1;
     // upper bound: every node fluid
     const uint64_t N = (uint64_t)lx * ly * lz, Np = (N + 63) / 64 * 64;
-    const bool shift = precision == LBX_PREC_FP32_SHIFT;
-    const uint64_t pop = shift ? 4 : 8;  // populations; also win and u* of moment propagation
-    uint64_t b = 2 * kQ * Np * pop + (shift ? 3 * Np * 4 : 0) + 4 * Np * 8 + N * 8 +
+    const uint64_t pop = f32_storage(precision) ? 4 : 8;  // populations; also win and u* of moment propagation
+    uint64_t b = 2 * kQ * Np * pop + (precision == LBX_PREC_FP32_SHIFT ? 3 * Np * 4 : 0) + 4 * Np * 8 + N * 8 +
                  nblocks(N) * 6 * kMaxTracers * 8 + sizeof(Ctrl) + kRing * 8;
     if (with_mp > 0) {  // with_mp tracers, each with adsorption (the upper bound)
         const uint64_t T = (uint64_t)std::min(with_mp, kMaxTracers);
@@ -465,7 +474,7 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     if (lx <= 0 || ly <= 0 || lz <= 0 || !nature) return LBX_ERR_ARG;
     // packed indices times 8 bytes must fit in 32 bits (the 32-bit offset loads)
     if ((uint64_t)lx * ly * lz >= (1ull << 29)) return LBX_ERR_ARG;
-    if (precision != LBX_PREC_FP64 && precision != LBX_PREC_FP32_SHIFT) return LBX_ERR_ARG;
+    if (precision != LBX_PREC_FP64 && !f32_storage(precision)) return LBX_ERR_ARG;
     if (!(tau >= 0.5)) return LBX_ERR_TAU;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LBX_ERR_DEVICE;
@@ -478,7 +487,7 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     c->dlx.set((uint32_t)lx);
     c->dly.set((uint32_t)ly);
     c->prec = precision;
-    c->floor_window = precision == LBX_PREC_FP32_SHIFT ? 1000 : 0;
+    c->floor_window = f32_storage(precision) ? 1000 : 0;
     c->tau = tau;
     relaxation_coeffs(tau, &c->A, &c->B, &c->C);
     (void)hipGetDevice(&c->device);
@@ -506,12 +515,12 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     c->ntiles = nblocks(c->Nf);
     int st = LBX_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = LBX_ERR_DEVICE;
-    const bool shift = precision == LBX_PREC_FP32_SHIFT;
+    const bool shift = f32_storage(precision);
     if (!st && !shift) st = dalloc(&c->f[0], kQ * c->Np);
     if (!st && !shift) st = dalloc(&c->f[1], kQ * c->Np);
     if (!st && shift) st = dalloc(&c->g[0], kQ * c->Np);
     if (!st && shift) st = dalloc(&c->g[1], kQ * c->Np);
-    if (!st && shift) st = dalloc(&c->j32, 3 * c->Np);
+    if (!st && precision == LBX_PREC_FP32_SHIFT) st = dalloc(&c->j32, 3 * c->Np);
     if (!st) st = dalloc(&c->j, 3 * c->Np);
     if (!st) st = dalloc(&c->rho, c->Np);
     if (!st) st = dalloc(&c->nodes, (uint64_t)c->Nf);
@@ -530,7 +539,7 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
         if (!st && !shift) st = hip_status(hipMemsetAsync(c->f[b], 0, kQ * c->Np * sizeof(double), c->stream));
         if (!st && shift) st = hip_status(hipMemsetAsync(c->g[b], 0, kQ * c->Np * sizeof(float), c->stream));
     }
-    if (!st && shift) st = hip_status(hipMemsetAsync(c->j32, 0, 3 * c->Np * sizeof(float), c->stream));
+    if (!st && c->j32) st = hip_status(hipMemsetAsync(c->j32, 0, 3 * c->Np * sizeof(float), c->stream));
     if (!st) st = hip_status(hipMemsetAsync(c->j, 0, 3 * c->Np * sizeof(double), c->stream));
     if (!st) st = hip_status(hipMemsetAsync(c->rho, 0, c->Np * sizeof(double), c->stream));
     if (!st) st = reset_ctrl(c);
@@ -555,7 +564,7 @@ int lbx_destroy(lbx_ctx* c) {
 
 int lbx_init_rest(lbx_ctx* c, double rho0) {
     if (!c) return LBX_ERR_ARG;
-    if (c->prec == LBX_PREC_FP32_SHIFT) c->rho0 = rho0;  // the shift follows the rest state
+    if (f32_storage(c->prec)) c->rho0 = rho0;  // the deviations follow the rest state
     hipLaunchKernelGGL(k_init_rest, dim3(c->ntiles), dim3(kBlock), 0, c->stream, c->f[0], c->g[0], c->j32, c->rho,
                        c->j, 0u, c->Nf, c->Np, rho0);
     c->pre = 0;
@@ -566,9 +575,9 @@ int lbx_init_rest(lbx_ctx* c, double rho0) {
 int lbx_set_state(lbx_ctx* c, const double* n, const double* rho, const double* jx, const double* jy,
                   const double* jz) {
     if (!c || !n || !rho || !jx || !jy || !jz) return LBX_ERR_ARG;
-    if (c->prec == LBX_PREC_FP32_SHIFT) {
-        // the shift: the density of the first fluid node (the uniform rest
-        // density when the state is a rest state)
+    if (f32_storage(c->prec)) {
+        // the rest density of the deviations: the density of the first fluid
+        // node (the uniform rest density when the state is a rest state)
         c->rho0 = rho[c->nodes_h[0] & kIndexMask];
         LBX_TRY(upload_shift(c, n, c->g[0]));
     } else {
@@ -597,8 +606,8 @@ int lbx_l2err_trace(lbx_ctx* c, int64_t first_step, int64_t count, double* out, 
 
 int lbx_get_state(lbx_ctx* c, double* n, double* rho, double* jx, double* jy, double* jz) {
     if (!c) return LBX_ERR_ARG;
-    if (n && c->prec == LBX_PREC_FP32_SHIFT) LBX_TRY(download_shift(c, c->g[c->pre], n));
-    if (n && c->prec != LBX_PREC_FP32_SHIFT) LBX_TRY(download_full(c, c->f[c->pre], kQ, n));
+    if (n && f32_storage(c->prec)) LBX_TRY(download_shift(c, c->g[c->pre], n));
+    if (n && !f32_storage(c->prec)) LBX_TRY(download_full(c, c->f[c->pre], kQ, n));
     if (rho) LBX_TRY(download_full(c, c->rho, 1, rho));
     double* jp[3] = {jx, jy, jz};
     for (int a = 0; a < 3; ++a)
@@ -614,7 +623,7 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
     const double zero[3] = {0.0, 0.0, 0.0};
     const double F[3] = {fext[0], fext[1], fext[2]};
     const uint32_t gt = c->ntiles;
-    const bool shift = c->prec == LBX_PREC_FP32_SHIFT;
+    const bool shift = f32_storage(c->prec);
     // opening collision of step 1 (stage 1: no force), into buffer 0
     if (c->pre != 0 && !shift)
         LBX_TRY(hipMemcpyAsync(c->f[0], c->f[c->pre], kQ * c->Np * sizeof(double), hipMemcpyDeviceToDevice,
@@ -622,10 +631,13 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
     if (c->pre != 0 && shift)
         LBX_TRY(hipMemcpyAsync(c->g[0], c->g[c->pre], kQ * c->Np * sizeof(float), hipMemcpyDeviceToDevice,
                                c->stream));
-    if (shift) {
-        hipLaunchKernelGGL(k_collide_packed_shift, dim3(gt), dim3(kBlock), 0, c->stream, c->g[0], c->rho, c->j,
-                           c->j32, 0.0f, 0.0f, 0.0f, 0u, c->Nf, c->Np, (float)c->A, (float)c->B, (float)c->C,
+    if (c->prec == LBX_PREC_FP32_SHIFT) {
+        hipLaunchKernelGGL(k_collide_packed_shift<float>, dim3(gt), dim3(kBlock), 0, c->stream, c->g[0], c->rho,
+                           c->j, c->j32, 0.0f, 0.0f, 0.0f, 0u, c->Nf, c->Np, (float)c->A, (float)c->B, (float)c->C,
                            c->rho0);
+    } else if (c->prec == LBX_PREC_FP32_STORE) {  // j_old is j itself
+        hipLaunchKernelGGL(k_collide_packed_shift<double>, dim3(gt), dim3(kBlock), 0, c->stream, c->g[0], c->rho,
+                           c->j, c->j, 0.0, 0.0, 0.0, 0u, c->Nf, c->Np, c->A, c->B, c->C, c->rho0);
     } else {
         hipLaunchKernelGGL(k_collide_packed, dim3(gt), dim3(kBlock), 0, c->stream, c->f[0], c->rho, c->j, 0.0, 0.0,
                            0.0, 0u, c->Nf, c->Np, c->A, c->B, c->C);
@@ -751,7 +763,7 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, int32_t ntr, co
     }
     const uint64_t np3 = 3 * (uint64_t)ntr * c->Np;
     int st = LBX_OK;
-    const bool shift = c->prec == LBX_PREC_FP32_SHIFT;
+    const bool shift = f32_storage(c->prec);
     if (!st && !shift) st = dalloc(&c->win, 18 * c->Np);
     if (!st && !shift) st = dalloc(&c->ustar, 3 * c->Np);
     if (!st && shift) st = dalloc(&c->win32, 18 * c->Np);
@@ -834,7 +846,7 @@ int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* 
         for (int64_t k = 0; k < S; ++k) {
             const int now = c->mp_now ^ (int)(k & 1);
             const int64_t it = c->mp_it + k + 1;
-            if (c->prec == LBX_PREC_FP32_SHIFT)
+            if (f32_storage(c->prec))
                 launch_mp_step<float>(c, now, it, k, stop_on_convergence, lim1, lim2);
             else
                 launch_mp_step<double>(c, now, it, k, stop_on_convergence, lim1, lim2);
@@ -993,7 +1005,7 @@ int lbx_kernel_time(lbx_ctx* c, int32_t which, double* total_ms, int64_t* launch
 
 int lbx_set_floor_window(lbx_ctx* c, int64_t steps) {
     if (!c || steps < 0) return LBX_ERR_ARG;
-    if (steps > 0 && c->prec != LBX_PREC_FP32_SHIFT) return LBX_ERR_ARG;
+    if (steps > 0 && !f32_storage(c->prec)) return LBX_ERR_ARG;
     c->floor_window = steps;
     return LBX_OK;
 }

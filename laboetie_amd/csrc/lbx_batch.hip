@@ -68,13 +68,13 @@ struct BInst {
     TracerParams tp;
 };
 
-template <typename E>
+template <typename E, typename R = E>
 struct BatchView {
     const BInst* insts;
     const uint32_t* tile_inst;
     E* f0;
     E* f1;
-    E* j;                         // FP64: the flux; FP32-shift: j32
+    R* j;                         // the step's flux (FP32-shift: j32)
     const uint32_t* nodes;
     const uint32_t* maps;
     unsigned long long* slots;    // [inst][3][kSlots]
@@ -84,14 +84,14 @@ struct BatchView {
 };
 
 // LB launch k of a round on one BLOCK-node tile (a redo repeats step lb_t0)
-template <int BLOCK, bool Wide, typename E>
-__device__ __forceinline__ void kb_lb_tile(const BatchView<E>& v, uint32_t tile, int64_t k) {
+template <int BLOCK, bool Wide, typename E, typename R = E>
+__device__ __forceinline__ void kb_lb_tile(const BatchView<E, R>& v, uint32_t tile, int64_t k) {
     const uint32_t id = v.tile_inst[tile / (kTile / BLOCK)];
     if (id == kNoInst) return;
     const BInst& in = v.insts[id];
     if (!in.lb_on || k >= in.lb_n) return;
     const int64_t t = in.lb_t0 + (in.lb_redo ? 0 : k);
-    StepArgs<E> a;
+    StepArgs<E, R> a;
     a.fin = (t - 1) & 1 ? v.f1 : v.f0;
     a.fout = t & 1 ? v.f1 : v.f0;
     a.j = v.j;
@@ -105,16 +105,16 @@ __device__ __forceinline__ void kb_lb_tile(const BatchView<E>& v, uint32_t tile,
     a.lz = in.lz;
     a.Nf = in.node_end;
     a.Np = v.Np;
-    a.A = (E)in.A;
-    a.B = (E)in.B;
-    a.C = (E)in.C;
-    a.fmx = (E)in.fm[0];
-    a.fmy = (E)in.fm[1];
-    a.fmz = (E)in.fm[2];
-    a.fcx = (E)in.fc[0];
-    a.fcy = (E)in.fc[1];
-    a.fcz = (E)in.fc[2];
-    a.rho0 = (E)in.rho0;
+    a.A = (R)in.A;
+    a.B = (R)in.B;
+    a.C = (R)in.C;
+    a.fmx = (R)in.fm[0];
+    a.fmy = (R)in.fm[1];
+    a.fmz = (R)in.fm[2];
+    a.fcx = (R)in.fc[0];
+    a.fcy = (R)in.fc[1];
+    a.fcz = (R)in.fc[2];
+    a.rho0 = (R)in.rho0;
     a.t = t;
     a.check_after = in.check_after;
     a.floor_window = in.floor_window;
@@ -122,12 +122,12 @@ __device__ __forceinline__ void kb_lb_tile(const BatchView<E>& v, uint32_t tile,
     a.write_l2 = in.lb_redo ? 0 : 1;
     a.have_prev = (!in.lb_redo && t > 1) ? 1 : 0;
     const uint32_t first = in.tile_first * (kTile / BLOCK);
-    lb_step_tile<BLOCK, Wide, E>(a, tile, tile - first, tile == first);
+    lb_step_tile<BLOCK, Wide, E, R>(a, tile, tile - first, tile == first);
 }
 
-template <int BLOCK, bool Wide, typename E>
-__global__ __launch_bounds__(BLOCK) void kb_lb_step(BatchView<E> v, int64_t k) {
-    kb_lb_tile<BLOCK, Wide, E>(v, xcd_tile(), k);
+template <int BLOCK, bool Wide, typename E, typename R = E>
+__global__ __launch_bounds__(BLOCK) void kb_lb_step(BatchView<E, R> v, int64_t k) {
+    kb_lb_tile<BLOCK, Wide, E, R>(v, xcd_tile(), k);
 }
 
 // l2err of each LB instance's last step of the round (one wave per instance slot)
@@ -237,8 +237,8 @@ __global__ __launch_bounds__(kBlock) void kb_mp_reduce_last(const BInst* insts, 
 // launches, so the results are the same bits.  Every block runs every
 // barrier: the loops have grid-uniform bounds and the per-tile functions
 // only return from the tile, never from the kernel.
-template <int BLOCK, bool Wide, typename E, typename W>
-__global__ __launch_bounds__(BLOCK) void kb_round(BatchView<E> lv, BatchMpView<W> mv, int64_t S_lb, int64_t S_mp,
+template <int BLOCK, bool Wide, typename E, typename R, typename W>
+__global__ __launch_bounds__(BLOCK) void kb_round(BatchView<E, R> lv, BatchMpView<W> mv, int64_t S_lb, int64_t S_mp,
                                                   uint32_t ntiles, uint32_t ninst, int32_t lb_redo_only) {
     cooperative_groups::grid_group grid = cooperative_groups::this_grid();
     // the tiles of XCD x (blocks b = x mod 8) are one contiguous range
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(BLOCK) void kb_round(BatchView<E> lv, BatchMpView<W
     const uint32_t per = (ntiles + 7u) / 8u;
     const uint32_t t0 = x * per, t1 = min(ntiles, t0 + per);
     for (int64_t k = 0; k < S_lb; ++k) {
-        for (uint32_t tile = t0 + local; tile < t1; tile += nblk) kb_lb_tile<BLOCK, Wide, E>(lv, tile, k);
+        for (uint32_t tile = t0 + local; tile < t1; tile += nblk) kb_lb_tile<BLOCK, Wide, E, R>(lv, tile, k);
         grid.sync();
     }
     if (S_lb > 0 && !lb_redo_only) {
@@ -447,7 +447,9 @@ void free_all(lbxb* b) {
     if (b->stream) (void)hipStreamDestroy(b->stream), b->stream = nullptr;
 }
 
-bool shift(const lbxb* b) { return b->prec == LBX_PREC_FP32_SHIFT; }
+// populations stored as FP32 deviations (FP32-shift and FP32-store forms);
+// their moment propagation keeps its probabilities in FP32
+bool shift(const lbxb* b) { return b->prec == LBX_PREC_FP32_SHIFT || b->prec == LBX_PREC_FP32_STORE; }
 
 int reset_inst_ctrl(lbxb* b, uint32_t id, int64_t check_after) {
     Ctrl h{};
@@ -461,15 +463,15 @@ int reset_inst_ctrl(lbxb* b, uint32_t id, int64_t check_after) {
 
 uint32_t grid_of(const lbxb* b, int block) { return b->hw_tiles * (kTile / block); }
 
-template <typename E>
-BatchView<E> lb_view(lbxb* b) {
-    BatchView<E> v;
+template <typename E, typename R = E>
+BatchView<E, R> lb_view(lbxb* b) {
+    BatchView<E, R> v;
     v.insts = b->insts;
     v.tile_inst = b->tile_inst;
     if constexpr (std::is_same<E, float>::value) {
         v.f0 = b->g[0];
         v.f1 = b->g[1];
-        v.j = b->j32;
+        if constexpr (std::is_same<R, float>::value) v.j = b->j32; else v.j = b->j;
     } else {
         v.f0 = b->f[0];
         v.f1 = b->f[1];
@@ -516,10 +518,16 @@ BatchMpView<W> mp_view(lbxb* b) {
 void launch_lb(lbxb* b, int64_t k, int block) {
     const uint32_t grid = grid_of(b, block);
     if (!grid) return;
-    if (shift(b)) {
+    if (b->prec == LBX_PREC_FP32_SHIFT) {
         auto v = lb_view<float>(b);
         auto kern = block == 128 ? (b->wide ? kb_lb_step<128, true, float> : kb_lb_step<128, false, float>)
                                  : (b->wide ? kb_lb_step<256, true, float> : kb_lb_step<256, false, float>);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, v, k);
+    } else if (b->prec == LBX_PREC_FP32_STORE) {
+        auto v = lb_view<float, double>(b);
+        auto kern = block == 128
+                        ? (b->wide ? kb_lb_step<128, true, float, double> : kb_lb_step<128, false, float, double>)
+                        : (b->wide ? kb_lb_step<256, true, float, double> : kb_lb_step<256, false, float, double>);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, v, k);
     } else {
         auto v = lb_view<double>(b);
@@ -608,17 +616,18 @@ int run_phase_persistent(lbxb* b, int phase, int64_t S, int32_t redo_only) {
     const int block = b->lb_block;
     const uint32_t ntiles = grid_of(b, block);
     if (!ntiles) return LBX_OK;
-    auto go = [&](auto etag, auto wtag, auto btag, auto wide) -> int {
-        using E = decltype(etag);
-        using W = decltype(wtag);
+    auto go = [&](auto etag, auto rtag, auto btag, auto wide) -> int {
+        using E = decltype(etag);  // population storage
+        using R = decltype(rtag);  // LB arithmetic
+        using W = E;               // MP probabilities are stored like the populations
         constexpr int BL = decltype(btag)::value;
         constexpr bool WD = decltype(wide)::value;
-        auto kern = kb_round<BL, WD, E, W>;
+        auto kern = kb_round<BL, WD, E, R, W>;
         int per_cu = 0;
         LBX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BL, 0));
         if (per_cu <= 0) return LBX_ERR_DEVICE;
         const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)per_cu * (uint32_t)b->num_cu);
-        BatchView<E> lv = lb_view<E>(b);
+        BatchView<E, R> lv = lb_view<E, R>(b);
         BatchMpView<W> mv = mp_view<W>(b);
         int64_t S_lb = phase == 0 ? S : 0, S_mp = phase == 1 ? S : 0;
         uint32_t nt = ntiles, ni = b->cap_inst;
@@ -630,9 +639,13 @@ int run_phase_persistent(lbxb* b, int phase, int64_t S, int32_t redo_only) {
     using I256 = std::integral_constant<int, 256>;
     using T_ = std::true_type;
     using F_ = std::false_type;
-    if (shift(b)) {
+    if (b->prec == LBX_PREC_FP32_SHIFT) {
         if (block == 128) return b->wide ? go(float{}, float{}, I128{}, T_{}) : go(float{}, float{}, I128{}, F_{});
         return b->wide ? go(float{}, float{}, I256{}, T_{}) : go(float{}, float{}, I256{}, F_{});
+    }
+    if (b->prec == LBX_PREC_FP32_STORE) {
+        if (block == 128) return b->wide ? go(float{}, double{}, I128{}, T_{}) : go(float{}, double{}, I128{}, F_{});
+        return b->wide ? go(float{}, double{}, I256{}, T_{}) : go(float{}, double{}, I256{}, F_{});
     }
     if (block == 128) return b->wide ? go(double{}, double{}, I128{}, T_{}) : go(double{}, double{}, I128{}, F_{});
     return b->wide ? go(double{}, double{}, I256{}, T_{}) : go(double{}, double{}, I256{}, F_{});
@@ -744,7 +757,7 @@ extern "C" {
 int64_t lbxb_footprint_bytes(int32_t precision, int64_t capacity_nodes, int64_t capacity_box, int32_t max_instances) {
     if (capacity_nodes <= 0 || capacity_box <= 0 || max_instances <= 0) return -1;
     const uint64_t Np = ((uint64_t)capacity_nodes + kTile - 1) / kTile * kTile;
-    const uint64_t pop = precision == LBX_PREC_FP32_SHIFT ? 4 : 8;
+    const uint64_t pop = (precision == LBX_PREC_FP32_SHIFT || precision == LBX_PREC_FP32_STORE) ? 4 : 8;
     uint64_t by = 2 * kQ * Np * pop + (precision == LBX_PREC_FP32_SHIFT ? 3 * Np * 4 : 0) + 4 * Np * 8 + Np * 4 +
                   (uint64_t)capacity_box * 4 + Np / kTile * 4;
     by += (uint64_t)max_instances * (sizeof(BInst) + 3 * kSlots * 8 + kRing * 8 + sizeof(Ctrl));
@@ -757,7 +770,8 @@ int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
                 lbxb** out) {
     if (!out) return LBX_ERR_ARG;
     *out = nullptr;
-    if (precision != LBX_PREC_FP64 && precision != LBX_PREC_FP32_SHIFT) return LBX_ERR_ARG;
+    if (precision != LBX_PREC_FP64 && precision != LBX_PREC_FP32_SHIFT && precision != LBX_PREC_FP32_STORE)
+        return LBX_ERR_ARG;
     if (capacity_nodes <= 0 || capacity_box <= 0 || max_instances <= 0) return LBX_ERR_ARG;
     const uint64_t Np = ((uint64_t)capacity_nodes + kTile - 1) / kTile * kTile;
     if (kQ * Np >= (1ull << 32) || (uint64_t)capacity_box >= (1ull << 32)) return LBX_ERR_ARG;
@@ -774,7 +788,7 @@ int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&b->num_cu, hipDeviceAttributeMultiprocessorCount, dev);
     }
-    const bool sh = precision == LBX_PREC_FP32_SHIFT;
+    const bool sh = precision == LBX_PREC_FP32_SHIFT || precision == LBX_PREC_FP32_STORE;
     int st = LBX_OK;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = LBX_ERR_DEVICE;
     for (int k = 0; k < 2; ++k) {
@@ -784,7 +798,7 @@ int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
         if (!st) st = dalloc(&b->Pa[k], 3 * Np);
         if (!st) st = dalloc(&b->part[k], (Np / 128) * 3);
     }
-    if (!st && sh) st = dalloc(&b->j32, 3 * Np);
+    if (!st && precision == LBX_PREC_FP32_SHIFT) st = dalloc(&b->j32, 3 * Np);
     if (!st) st = dalloc(&b->j, 3 * Np);
     if (!st) st = dalloc(&b->rho, Np);
     if (!st) st = dalloc(&b->nodes, Np);
@@ -903,9 +917,14 @@ int lbxb_add(lbxb* b, int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, 
         if (shift(b)) {
             hipLaunchKernelGGL(k_init_rest, dim3(ntiles), dim3(kBlock), 0, b->stream, nullptr, b->g[0], b->j32,
                                b->rho, b->j, in.node_off, in.node_end, b->Np, rho0);
-            hipLaunchKernelGGL(k_collide_packed_shift, dim3(ntiles), dim3(kBlock), 0, b->stream, b->g[0], b->rho,
-                               b->j, b->j32, 0.0f, 0.0f, 0.0f, in.node_off, in.node_end, b->Np, (float)in.A,
-                               (float)in.B, (float)in.C, rho0);
+            if (b->prec == LBX_PREC_FP32_SHIFT)
+                hipLaunchKernelGGL(k_collide_packed_shift<float>, dim3(ntiles), dim3(kBlock), 0, b->stream, b->g[0],
+                                   b->rho, b->j, b->j32, 0.0f, 0.0f, 0.0f, in.node_off, in.node_end, b->Np,
+                                   (float)in.A, (float)in.B, (float)in.C, rho0);
+            else  // FP32-store: j_old is j itself
+                hipLaunchKernelGGL(k_collide_packed_shift<double>, dim3(ntiles), dim3(kBlock), 0, b->stream, b->g[0],
+                                   b->rho, b->j, b->j, 0.0, 0.0, 0.0, in.node_off, in.node_end, b->Np, in.A, in.B,
+                                   in.C, rho0);
         } else {
             hipLaunchKernelGGL(k_init_rest, dim3(ntiles), dim3(kBlock), 0, b->stream, b->f[0], nullptr, nullptr,
                                b->rho, b->j, in.node_off, in.node_end, b->Np, rho0);

@@ -93,18 +93,19 @@ __device__ __forceinline__ double collide_pop(double f, double rho, double jx, d
 //   g' = n' - a0 rho0 = (1-1/tau) g + (1/tau) (neq - a0 rho0) + (1-1/(2 tau)) G
 // (because (1-1/tau) + 1/tau = 1), with neq - a0 rho0 = a0 drho + a1 (c.j) +
 // a2 sum_ab j_a u_b (c_a c_b - cs^2 delta_ab) and drho = rho - rho0 = sum g.
-// All of it in FP32: the deviations stay small, so their rounding error is
-// relative to the deviation, not to rho0.
-template <int L>
-__device__ __forceinline__ float collide_shift(float g, float drho, float jx, float jy, float jz, float ux,
-                                               float uy, float uz, float fx, float fy, float fz, float A, float B,
-                                               float C) {
+// V = float (the FP32-shift form) runs all of it in FP32: the deviations
+// stay small, so their rounding error is relative to the deviation, not to
+// rho0.  V = double (the FP32-store form) runs the same operations in FP64 on
+// the FP32-stored deviations.
+template <int L, typename V = float>
+__device__ __forceinline__ V collide_shift(V g, V drho, V jx, V jy, V jz, V ux, V uy, V uz, V fx, V fy, V fz, V A,
+                                           V B, V C) {
     constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
-    constexpr float w0 = (float)a0(L), w1 = (float)a1(L), w2 = (float)a2(L);
-    constexpr float Kx = (float)((double)(cx * cx) - kCsq);
-    constexpr float Ky = (float)((double)(cy * cy) - kCsq);
-    constexpr float Kz = (float)((double)(cz * cz) - kCsq);
-    float q = (jx * ux) * Kx;
+    constexpr V w0 = (V)a0(L), w1 = (V)a1(L), w2 = (V)a2(L);
+    constexpr V Kx = (V)((double)(cx * cx) - kCsq);
+    constexpr V Ky = (V)((double)(cy * cy) - kCsq);
+    constexpr V Kz = (V)((double)(cz * cz) - kCsq);
+    V q = (jx * ux) * Kx;
     if constexpr (cx * cy != 0) q = q + sgn<cx * cy>(jx * uy);
     if constexpr (cx * cz != 0) q = q + sgn<cx * cz>(jx * uz);
     if constexpr (cy * cx != 0) q = q + sgn<cy * cx>(jy * ux);
@@ -113,11 +114,11 @@ __device__ __forceinline__ float collide_shift(float g, float drho, float jx, fl
     if constexpr (cz * cx != 0) q = q + sgn<cz * cx>(jz * ux);
     if constexpr (cz * cy != 0) q = q + sgn<cz * cy>(jz * uy);
     q = q + (jz * uz) * Kz;
-    float dneq = w0 * drho;
+    V dneq = w0 * drho;
     if constexpr (L != 0) dneq = dneq + w1 * dot3<cx, cy, cz>(jx, jy, jz);
     dneq = dneq + w2 * q;
-    float G = w1 * ((((float)cx - ux) * fx + ((float)cy - uy) * fy) + ((float)cz - uz) * fz);
-    if constexpr (L != 0) G = G + ((2.0f * w2) * dot3<cx, cy, cz>(ux, uy, uz)) * dot3<cx, cy, cz>(fx, fy, fz);
+    V G = w1 * ((((V)cx - ux) * fx + ((V)cy - uy) * fy) + ((V)cz - uz) * fz);
+    if constexpr (L != 0) G = G + (((V)2 * w2) * dot3<cx, cy, cz>(ux, uy, uz)) * dot3<cx, cy, cz>(fx, fy, fz);
     return (A * g + B * dneq) + C * G;
 }
 

@@ -81,14 +81,15 @@ struct Neighbours {
 
 // ---------------------------------------------------------------- LB kernels
 
-// Populations, flux and step constants are of type E: double for the FP64
-// form, float for the FP32-shift form (populations stored as deviations
-// g = n - a0 rho0 from the rest state).
-template <typename E>
+// Populations are stored as E, the step's arithmetic, its flux and step
+// constants are R: E = R = double for the FP64 form; E = float (deviations
+// g = n - a0 rho0 from the rest state) with R = float for the FP32-shift form
+// and R = double for the FP32-store form.
+template <typename E, typename R = E>
 struct StepArgs {
     const E* __restrict__ fin;
     E* __restrict__ fout;
-    E* __restrict__ j;
+    R* __restrict__ j;
     const uint32_t* __restrict__ nodes;
     const uint32_t* __restrict__ map;
     unsigned long long* slots;            // [3][kSlots] striped per-step max|dj| bit patterns
@@ -97,10 +98,10 @@ struct StepArgs {
     FastDiv dlx, dly;
     uint32_t lz, Nf;
     uint64_t Np;
-    E A, B, C;
-    E fmx, fmy, fmz;   // F(t)/2, the force of step t in j(t) (equilibration.f90:293-295)
-    E fcx, fcy, fcz;   // F(t+1), the force of the collision that opens step t+1
-    E rho0;            // FP32-shift: the rest density of the shift
+    R A, B, C;
+    R fmx, fmy, fmz;   // F(t)/2, the force of step t in j(t) (equilibration.f90:293-295)
+    R fcx, fcy, fcz;   // F(t+1), the force of the collision that opens step t+1
+    R rho0;            // FP32 storage: the rest density of the deviations
     int64_t t;
     int64_t check_after;    // stop check on l2err(t-1) only when t-1 > check_after
     int64_t floor_window;   // > 0: also stop when l2err set no new minimum for this many steps
@@ -161,9 +162,10 @@ __device__ __forceinline__ E pull(const View<Wide, E>& fin, uint32_t Np, uint32_
 // consecutive packed nodes, i = tile*BLOCK + lane, valid below a.Nf), `stripe`
 // picks its l2err slot, and `first_block` is the one block of the lattice that
 // writes the step's control fields and clears the next step's slots.
-template <int BLOCK, bool Wide, typename E>
-__device__ __forceinline__ void lb_step_tile(const StepArgs<E>& a, uint32_t tile, uint32_t stripe, bool first_block) {
-    constexpr bool kShift = std::is_same<E, float>::value;
+template <int BLOCK, bool Wide, typename E, typename R = E>
+__device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t tile, uint32_t stripe,
+                                             bool first_block) {
+    constexpr bool kShift = std::is_same<E, float>::value;  // deviations stored in FP32
     const uint32_t i = tile * BLOCK + threadIdx.x;
     // Each wave's work is a chain of dependent memory round trips, so
     // independent requests go out together: the node word, the stop flags
@@ -229,24 +231,25 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E>& a, uint32_t tile
     bool neg = false;
     if (i < a.Nf) {
         const uint32_t Np = (uint32_t)a.Np;
-        const View<Wide, E> fin(a.fin, kQ * a.Np), fout(a.fout, kQ * a.Np), jv(a.j, 3 * a.Np);
+        const View<Wide, E> fin(a.fin, kQ * a.Np), fout(a.fout, kQ * a.Np);
+        const View<Wide, R> jv(a.j, 3 * a.Np);
         Neighbours nb;
         nb.load(a.map, node & kIndexMask, a.dlx, a.dly, a.lz);
-        E jo[3] = {E(0), E(0), E(0)};
+        R jo[3] = {R(0), R(0), R(0)};
         if (a.write_l2) {  // j_old, issued before the populations so its latency overlaps theirs
             jo[0] = jv.ld(i);
             jo[1] = jv.ld(Np + i);
             jo[2] = jv.ld(2 * Np + i);
         }
-        E f[kQ];
-        static_for<0, kQ>([&](auto I) { f[I] = pull<I, Wide, E>(fin, Np, i, nb); });
+        R f[kQ];
+        static_for<0, kQ>([&](auto I) { f[I] = (R)pull<I, Wide, E>(fin, Np, i, nb); });
         if constexpr (kShift)  // n < 0  <=>  g < -a0 rho0
-            static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < -((float)a0(I) * a.rho0)); });
+            static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < -((R)a0(I) * a.rho0)); });
         else
             static_for<0, kQ>([&](auto I) { neg = neg || (f[I] < 0.0); });
-        E rho = f[0];  // FP32-shift: drho = rho - rho0
+        R rho = f[0];  // FP32 storage: drho = rho - rho0
         static_for<1, kQ>([&](auto I) { rho = rho + f[I]; });
-        E jx = a.fmx, jy = a.fmy, jz = a.fmz;
+        R jx = a.fmx, jy = a.fmy, jz = a.fmz;
         static_for<0, kQ>([&](auto I) {
             constexpr int L = I;
             if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(f[L]);
@@ -254,19 +257,19 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E>& a, uint32_t tile
             if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(f[L]);
         });
         if (a.write_l2) {
-            const E dx = fabs(jx - jo[0]), dy = fabs(jy - jo[1]), dz = fabs(jz - jo[2]);
+            const R dx = fabs(jx - jo[0]), dy = fabs(jy - jo[1]), dz = fabs(jz - jo[2]);
             l2 = (double)fmax(fmax(dx, dy), dz);
         }
         jv.st(i, jx);
         jv.st(Np + i, jy);
         jv.st(2 * Np + i, jz);
         if constexpr (kShift) {
-            const float drho = rho, full = a.rho0 + drho;
-            const float ux = jx / full, uy = jy / full, uz = jz / full;
+            const R drho = rho, full = a.rho0 + drho;
+            const R ux = jx / full, uy = jy / full, uz = jz / full;
             static_for<0, kQ>([&](auto I) {
                 constexpr int L = I;
-                fout.st((uint32_t)L * Np + i, collide_shift<L>(f[L], drho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy,
-                                                               a.fcz, a.A, a.B, a.C));
+                fout.st((uint32_t)L * Np + i, (E)collide_shift<L, R>(f[L], drho, jx, jy, jz, ux, uy, uz, a.fcx,
+                                                                     a.fcy, a.fcz, a.A, a.B, a.C));
             });
         } else {
             const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
@@ -297,9 +300,9 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E>& a, uint32_t tile
     }
 }
 
-template <bool Wide, typename E>
-__global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs<E> a) {
-    lb_step_tile<kBlock, Wide, E>(a, xcd_tile(), blockIdx.x, blockIdx.x == 0);
+template <bool Wide, typename E, typename R = E>
+__global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs<E, R> a) {
+    lb_step_tile<kBlock, Wide, E, R>(a, xcd_tile(), blockIdx.x, blockIdx.x == 0);
 }
 
 // l2err of the last step of a launch run, which no later step kernel reads
@@ -389,26 +392,27 @@ __global__ __launch_bounds__(kBlock) void k_collide_packed(double* f, const doub
     });
 }
 
-// The same for the FP32-shift form: the FP64 density and flux of the state
-// become the FP32 drho = rho - rho0 and j of the collision (and j_old of the
-// first step).
+// The same for the FP32 storage forms: the FP64 density and flux of the state
+// become the step's drho = rho - rho0 and j (type R: float for FP32-shift,
+// double for FP32-store) of the collision and j_old of the first step
+// (jold may be j itself in the FP32-store form).
+template <typename R>
 __global__ __launch_bounds__(kBlock) void k_collide_packed_shift(float* g, const double* __restrict__ rho,
-                                                                 const double* __restrict__ j,
-                                                                 float* __restrict__ j32, float fx, float fy,
-                                                                 float fz, uint32_t i0, uint32_t Nf, uint64_t Np,
-                                                                 float A, float B, float C, double rho0) {
+                                                                 const double* j, R* jold, R fx, R fy, R fz,
+                                                                 uint32_t i0, uint32_t Nf, uint64_t Np, R A, R B,
+                                                                 R C, double rho0) {
     const uint32_t i = i0 + blockIdx.x * kBlock + threadIdx.x;
     if (i >= Nf) return;
-    const float drho = (float)(rho[i] - rho0), full = (float)rho0 + drho;
-    const float jx = (float)j[i], jy = (float)j[Np + i], jz = (float)j[2 * Np + i];
-    j32[i] = jx;  // j_old of the first step
-    j32[Np + i] = jy;
-    j32[2 * Np + i] = jz;
-    const float ux = jx / full, uy = jy / full, uz = jz / full;
+    const R drho = (R)(rho[i] - rho0), full = (R)rho0 + drho;
+    const R jx = (R)j[i], jy = (R)j[Np + i], jz = (R)j[2 * Np + i];
+    jold[i] = jx;  // j_old of the first step
+    jold[Np + i] = jy;
+    jold[2 * Np + i] = jz;
+    const R ux = jx / full, uy = jy / full, uz = jz / full;
     static_for<0, kQ>([&](auto I) {
         constexpr int L = I;
-        const float v = g[(uint64_t)L * Np + i];
-        g[(uint64_t)L * Np + i] = collide_shift<L>(v, drho, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
+        const R v = g[(uint64_t)L * Np + i];
+        g[(uint64_t)L * Np + i] = (float)collide_shift<L, R>(v, drho, jx, jy, jz, ux, uy, uz, fx, fy, fz, A, B, C);
     });
 }
 

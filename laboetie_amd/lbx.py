@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblbx.so")
 
 OK, ERR_NEG_POP, ERR_RESTPART, ERR_TAU, ERR_NOMEM, ERR_ARG, ERR_DEVICE, ERR_STATE = range(8)
-PREC_FP64, PREC_FP32_SHIFT = 0, 1
+PREC_FP64, PREC_FP32_SHIFT, PREC_FP32_STORE = 0, 1, 2
 EXIT_CONVERGED, EXIT_CAP, EXIT_FLOOR = 1, 2, 3
 KERNEL_LB, KERNEL_MP = 0, 1
 

@@ -76,6 +76,13 @@ int lbshift_equilibrate(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature
                         double* n, double* rho, double* jx, double* jy, double* jz, double* l2err_trace,
                         int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason);
 
+/* The FP32-store form: the same FP32 deviation storage, collision and
+ * j_old in FP64 (the library's LBX_PREC_FP32_STORE). */
+int lbstore_equilibrate(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double rho0,
+                        const double fext[3], double target_error, int64_t max_steps, int64_t floor_window,
+                        double* n, double* rho, double* jx, double* jy, double* jz, double* l2err_trace,
+                        int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason);
+
 int lbshift_mp(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, const int8_t* interfacial,
                const double* rho, const double* jx, const double* jy, const double* jz, const double fext[3],
                double Db, double ka, double kd, int64_t nsteps, double* P, double* Pa, double* vacf);

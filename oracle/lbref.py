@@ -41,6 +41,7 @@ def lib():
         L.lbref_mp_converged.argtypes = [_i64, _p, _i32, _i32, _i32, _f64]
         L.lbref_deff.argtypes = [_p, _i64, _p]
         L.lbshift_equilibrate.argtypes = [_i32] * 3 + [_p, _f64, _f64, _p, _f64, _i64, _i64] + [_p] * 6 + [_i64, _p, _p]
+        L.lbstore_equilibrate.argtypes = L.lbshift_equilibrate.argtypes
         L.lbshift_mp.argtypes = [_i32] * 3 + [_p] * 7 + [_f64] * 3 + [_i64] + [_p] * 3
         _lib = L
     return _lib
@@ -172,9 +173,11 @@ def deff(vacf):
 
 # ---------------------------------------------------------------- FP32-shift
 def equilibrate_shift(dims, nature, tau, rho0, fext, target_error, max_steps=0, floor_window=1000,
-                      trace_cap=1 << 20):
-    """lbshift_equilibrate: the FP32-shift form of equilibrate (same outputs,
-    FP64 n / rho / j built from the FP32 deviations)."""
+                      trace_cap=1 << 20, arith="fp32"):
+    """lbshift_equilibrate (arith="fp32": the FP32-shift form) or
+    lbstore_equilibrate (arith="fp64": the FP32-store form, collision in
+    FP64): equilibrate with FP32 deviation storage (same outputs, FP64 n /
+    rho / j built from the FP32 deviations)."""
     lx, ly, lz = dims
     N = lx * ly * lz
     n = np.zeros(19 * N)
@@ -182,9 +185,10 @@ def equilibrate_shift(dims, nature, tau, rho0, fext, target_error, max_steps=0, 
     tr = np.zeros(trace_cap)
     sd, er = _i64(), _i32()
     fe = np.ascontiguousarray(fext, np.float64)
-    st = lib().lbshift_equilibrate(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)), tau, rho0, _ptr(fe),
-                                   target_error, max_steps, floor_window, _ptr(n), _ptr(rho), _ptr(jx), _ptr(jy),
-                                   _ptr(jz), _ptr(tr), trace_cap, ctypes.byref(sd), ctypes.byref(er))
+    fn = lib().lbshift_equilibrate if arith == "fp32" else lib().lbstore_equilibrate
+    st = fn(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)), tau, rho0, _ptr(fe), target_error, max_steps,
+            floor_window, _ptr(n), _ptr(rho), _ptr(jx), _ptr(jy), _ptr(jz), _ptr(tr), trace_cap, ctypes.byref(sd),
+            ctypes.byref(er))
     steps = sd.value
     nl2 = steps - 1 if st == NEG_POP else steps
     return dict(status=st, steps=steps, exit_reason=er.value, n=n, rho=rho, j=np.stack([jx, jy, jz]),

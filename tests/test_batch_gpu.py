@@ -59,7 +59,7 @@ def run_batch(cases, prec, mp_steps, block=256, graphs=True, lb_round=64, mp_rou
         return out, B.stats()
 
 
-@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT])
+@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT, lbx.PREC_FP32_STORE])
 def test_batch_matches_single_lattice_bitwise(prec):
     cases = [Case(n) for n in CASES]
     mp_steps = 150
@@ -147,7 +147,7 @@ def test_served_requests_equal_alone():
             assert np.array_equal(L.mp_deff(), s["deff"][k]), k
 
 
-@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT])
+@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT, lbx.PREC_FP32_STORE])
 def test_batch_persistent_launches_same_bits(prec):
     """A round's steps as one cooperative launch per phase (grid-wide barrier
     between steps) give the same bits as per-step launches."""

@@ -1,13 +1,15 @@
-"""Pins of the FP32-shift restatement (oracle/lbshift.c), a precision form the
-reference does not have: its results against the FP64 restatement (pinned to
-the reference's own outputs) within the form's error bounds, exact
-invariants (rest state, conservation of sum_r (P + Pa)) and the bulk closed
-form D_eff = Db.
+"""Pins of the FP32-storage restatements (oracle/lbshift.c), precision forms
+the reference does not have: FP32-shift (FP32 deviations, FP32 collision)
+and FP32-store (the same storage, collision and j_old in FP64).  Their
+results against the FP64 restatement (pinned to the reference's own outputs)
+within each form's error bounds, exact invariants (rest state, conservation
+of sum_r (P + Pa)) and the bulk closed form D_eff = Db.
 
-The bounds below are the measured accuracy of the FP32-shift form on these
-inputs with margin; the weakest case is the strong Poiseuille flow of the
-4x4x34 slit (Peclet ~25), where the flux error of the FP32 steady state
-(~2.5e-5 relative) carries into D_eff through Taylor dispersion.
+The bounds below are the measured accuracy of the forms on these inputs
+with margin; the weakest case is the strong Poiseuille flow of the 4x4x34
+slit (Peclet ~25), where the flux error of the FP32-shift steady state
+(~2.5e-5 relative; FP32-store ~3.9e-6) carries into D_eff through Taylor
+dispersion.
 """
 import numpy as np
 import pytest
@@ -25,13 +27,15 @@ def runs():
         c = Case(name)
         a = lbref.equilibrate(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target)
         b = lbref.equilibrate_shift(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, max_steps=4 * a["steps"])
-        out[name] = (c, a, b)
+        s = lbref.equilibrate_shift(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, max_steps=4 * a["steps"],
+                                    arith="fp64")
+        out[name] = (c, a, b, s)
     return out
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_shift_flow_within_fp32_bounds_of_fp64(runs, name):
-    c, a, b = runs[name]
+    c, a, b, _ = runs[name]
     assert b["status"] == lbref.OK
     assert b["exit_reason"] in (lbref.EXIT_CONVERGED, lbref.EXIT_FLOOR, lbref.EXIT_CAP)
     jm = np.abs(a["j"]).max()
@@ -46,7 +50,7 @@ def test_shift_flow_within_fp32_bounds_of_fp64(runs, name):
 
 @pytest.mark.parametrize("name", CASES)
 def test_shift_mp_within_bounds_and_conserving(runs, name):
-    c, a, b = runs[name]
+    c, a, b, _ = runs[name]
     itf = lbref.interfacial(*c.dims, c.nature)
     nt = lbref.tracer_population(c.dims, c.nature, a["rho"], a["j"], c.fext)
     mp = lbref.MP(c.dims, c.nature, itf, nt, a["rho"], c.Db, c.ka, c.kd)
@@ -96,6 +100,49 @@ def test_shift_bulk_deff_equals_db():
 def test_shift_floor_exit_on_strong_flow(runs):
     """The strong slit flow reaches the FP32 rounding floor (l2err cycling
     above 1e-10): the floor test ends it instead of running forever."""
-    c, a, b = runs["slit_4x4x34"]
+    c, a, b, _ = runs["slit_4x4x34"]
     assert b["exit_reason"] == lbref.EXIT_FLOOR
     assert b["l2err"][-1] > c.target
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_store_flow_closer_to_fp64_than_shift(runs, name):
+    """FP32-store: the same FP32 storage with the collision in FP64 puts the
+    steady flux within 1e-5 of max|j| on every case (3.9e-6 on the slit) and
+    nearer the FP64 flux than the FP32-shift form."""
+    c, a, b, s = runs[name]
+    assert s["status"] == lbref.OK
+    assert s["exit_reason"] in (lbref.EXIT_CONVERGED, lbref.EXIT_FLOOR, lbref.EXIT_CAP)
+    jm = np.abs(a["j"]).max()
+    err_store, err_shift = np.abs(a["j"] - s["j"]).max(), np.abs(a["j"] - b["j"]).max()
+    assert err_store <= 1e-5 * jm
+    assert err_store < err_shift
+    assert np.abs(a["rho"] - s["rho"]).max() <= 1e-8
+    assert np.abs(a["n"] - s["n"]).max() <= 1e-8
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_store_deff_within_1e5(runs, name):
+    """D_eff of the FP32-store flow (FP32 probabilities, FP64 P) within 1e-5
+    relative of the FP64 path on every case, the slit included."""
+    c, a, _, s = runs[name]
+    itf = lbref.interfacial(*c.dims, c.nature)
+    nt = lbref.tracer_population(c.dims, c.nature, a["rho"], a["j"], c.fext)
+    mp = lbref.MP(c.dims, c.nature, itf, nt, a["rho"], c.Db, c.ka, c.kd)
+    st, v = mp.run(c.mp_steps)
+    st2, v2, _, _ = lbref.mp_shift(c.dims, c.nature, itf, s["rho"], s["j"], c.fext, c.Db, c.ka, c.kd, c.mp_steps)
+    assert st == st2 == lbref.OK
+    d, d2 = lbref.deff(v), lbref.deff(v2)
+    assert np.abs(d - d2).max() <= 1e-5 * np.abs(d).max()
+
+
+def test_store_rest_state_is_a_fixed_point():
+    _, nat = lbref.geometry(3, 7, 7, 7)
+    dims = (7, 7, 7)
+    s = lbref.equilibrate_shift(dims, nat, 0.8, 1.3, [0, 0, 0], 1e-12, max_steps=20, arith="fp64")
+    a0 = lbref.tables()[1]
+    fluid = nat == 0
+    n = s["n"].reshape(19, -1)
+    for l in range(19):
+        assert np.all(n[l][fluid] == a0[l] * 1.3)
+    assert np.all(s["j"] == 0.0) and np.all(s["rho"][fluid] == 1.3)

@@ -6,8 +6,8 @@ buffer offsets while 19*Np*8 < 4 GiB and through 64-bit pointers above that.
 Two checks:
 - on small lattices, the 64-bit form forced on (lbx_set_addressing,
   lbxb_set_addressing) against the default form, single lattice and batch,
-  FP64 and FP32-shift: bit-identical populations, density, flux, step count,
-  VACF(t), P and D_eff;
+  FP64, FP32-shift and FP32-store: bit-identical populations, density, flux,
+  step count, VACF(t), P and D_eff;
 - a lattice large enough to select the 64-bit form by itself: the
   reference's random_3d case (7x6x5, periodic) repeated 50 x 60 x 60 times
   (350x360x300, 28.4 M fluid nodes, 19*Np*8 = 4.32e9 bytes).  Every copy
@@ -56,7 +56,7 @@ def solve(c, prec, wide, mp_steps=80):
                     deffb=L.mp_deff())
 
 
-@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT])
+@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT, lbx.PREC_FP32_STORE])
 @pytest.mark.parametrize("name", CASES)
 def test_forced_64bit_addressing_same_bits(name, prec):
     c = Case(name)
