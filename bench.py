@@ -19,13 +19,14 @@ and the reference program timed on this machine's host cores.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-fp32-shift] [--no-large]
                     [--no-batch]
 
-The same job also runs in the FP32-shift precision form; its throughput,
-rooflines and D_eff error against the FP64 run go under "fp32_shift".  The
+The same job also runs in the two FP32-storage precision forms (FP32-shift:
+FP32 collision; FP32-store: FP64 collision); their throughput, rooflines and
+D_eff error against the FP64 run go under "fp32_shift" and "fp32_store".  The
 128^3 job's working set (~0.25 GB) fits the 256 MB MALL, so "roofline_256"
 adds the two kernels' rooflines on the 256^3 random medium (3 GB of FP64
 populations), outside the timed region.  The other configurations run
 beside it, also outside the timed region: the 4x4x34 slit ("slit_4x4x34"),
-the 1024-instance batch ("batch_1024", FP64 and FP32-shift), and, in
+the 1024-instance batch ("batch_1024", FP64, FP32-shift and FP32-store), and, in
 "cpu_baseline", the reference program on a bounded sample of each of them
 and of two serving requests.
 """
@@ -88,11 +89,12 @@ def solve(L, w, batch=True):
     return r["steps"], mp_steps, deffs
 
 
-def fp32_shift_leg(w, nat, args, nf, nfi):
-    """The same job in the FP32-shift precision form (populations as FP32
-    deviations from a0*rho0, FP32 collision, FP32 MP probabilities; FP64
-    moments, P, VACF and D_eff), timed the same way."""
-    with lbx.Lattice(w["dims"], nat, w["tau"], precision=lbx.PREC_FP32_SHIFT) as L:
+def fp32_leg(w, nat, args, nf, nfi, prec=lbx.PREC_FP32_SHIFT):
+    """The same job in an FP32-storage precision form (populations as FP32
+    deviations from a0*rho0, FP32 MP probabilities; FP64 moments, P, VACF and
+    D_eff), timed the same way: FP32-shift collides in FP32, FP32-store in
+    FP64."""
+    with lbx.Lattice(w["dims"], nat, w["tau"], precision=prec) as L:
         for _ in range(args.warmup):
             solve(L, w, not args.mp_one_at_a_time)
         L.timing(True)
@@ -111,7 +113,7 @@ def fp32_shift_leg(w, nat, args, nf, nfi):
     mp_bytes = (MP32_COEF_BYTES_PER_FLUID * nf + MP_P_BYTES_PER_FLUID * nf * T
                 + MP_BYTES_PER_ADS_INTERFACIAL * nfi * T)
     mp_gbs = mp_bytes / (mp_ms / mp_n / 1e3) / 1e9
-    print("bench: fp32-shift leg done", file=sys.stderr, flush=True)
+    print(f"bench: fp32 leg (precision {prec}) done", file=sys.stderr, flush=True)
     return {"value": round(N * sum(r[0] + r[1] for r in results) / dt / 1e6, 2), "unit": "MLUPS",
             "ms_per_step": round(dt / args.steps * 1e3, 3), "lb_steps": lb_steps, "mp_steps": mp_steps,
             "roofline_lb": {"achieved": round(lb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -132,7 +134,7 @@ def large_leg():
     Gaussian random medium (correlation length 8, porosity 0.6), tau 0.8,
     F = 1e-6 along x; LARGE["lb_steps"] LB steps from rest (the force on
     from t = 4) and LARGE["mp_steps"] MP steps of one tracer (dF = -1 kT),
-    FP64 and FP32-shift, kernel times from HIP events."""
+    FP64, FP32-shift and FP32-store, kernel times from HIP events."""
     g = LARGE
     nat = lbx.geometry_random(*g["dims"], g["sigma"], g["porosity"], g["seed"])
     lx, ly, lz = g["dims"]
@@ -144,9 +146,11 @@ def large_leg():
                        "tracer (dF=-1 kT)"}
     for prec, name, lbb, coef in ((lbx.PREC_FP64, "fp64", LB_BYTES_PER_FLUID, MP_COEF_BYTES_PER_FLUID),
                                   (lbx.PREC_FP32_SHIFT, "fp32_shift", LB32_BYTES_PER_FLUID,
+                                   MP32_COEF_BYTES_PER_FLUID),
+                                  (lbx.PREC_FP32_STORE, "fp32_store", LB32_BYTES_PER_FLUID,
                                    MP32_COEF_BYTES_PER_FLUID)):
         with lbx.Lattice(g["dims"], nat, g["tau"], precision=prec) as L:
-            if prec == lbx.PREC_FP32_SHIFT:
+            if prec != lbx.PREC_FP64:
                 L.set_floor_window(0)
             nf = L.fluid_nodes
             L.init_rest(1.0)
@@ -206,14 +210,24 @@ def batch_leg(n=1024):
     out = {"workload": f"{n} chromatography 2-D instances (1x50x50 / 1x100x100, 1-8 disks r=5), tau 1, "
                        "F=1e-5 along y, target 1e-10 (cap 20000), 2000 MP steps, dF U[-3,0]"}
     deffs = {}
-    for prec, name in ((lbx.PREC_FP64, "fp64"), (lbx.PREC_FP32_SHIFT, "fp32_shift")):
+    conv = None
+    for prec, name in ((lbx.PREC_FP64, "fp64"), (lbx.PREC_FP32_SHIFT, "fp32_shift"),
+                       (lbx.PREC_FP32_STORE, "fp32_store")):
         r = serving.run_batch(reqs, prec)
         deffs[name] = np.array([x["deff"] for x in r["results"]])
+        if conv is None:  # instances whose FP64 run converged (the others stop at the step cap)
+            conv = np.array([x["exit_reason"] == lbx.EXIT_CONVERGED for x in r["results"]])
         out[name] = {"mlups": round(r["mlups"], 1), "mflups": round(r["mflups"], 1), "wall_s": round(r["wall_s"], 2),
                      "lb_kernel_mlups": round(r["lb_kernel_mlups"], 1), "mp_kernel_mlups": round(r["mp_kernel_mlups"], 1)}
-    err = np.abs(deffs["fp32_shift"] - deffs["fp64"]).max(axis=1) / np.abs(deffs["fp64"]).max(axis=1)
-    out["deff_rel_error_fp32_shift"] = {"median": float(f"{np.median(err):.3e}"), "max": float(f"{err.max():.3e}")}
-    out["fp32_shift_vs_fp64"] = round(out["fp32_shift"]["mlups"] / out["fp64"]["mlups"], 3)
+    for name in ("fp32_shift", "fp32_store"):
+        err = np.abs(deffs[name] - deffs["fp64"]).max(axis=1) / np.abs(deffs["fp64"]).max(axis=1)
+        out[f"deff_rel_error_{name}"] = {"median": float(f"{np.median(err):.3e}"), "max": float(f"{err.max():.3e}"),
+                                         "count_above_1e-5": int((err > 1e-5).sum())}
+        ec = err[conv]
+        out[f"deff_rel_error_{name}_where_fp64_converged"] = {
+            "instances": int(conv.sum()), "median": float(f"{np.median(ec):.3e}"), "max": float(f"{ec.max():.3e}"),
+            "count_above_1e-5": int((ec > 1e-5).sum())}
+        out[f"{name}_vs_fp64"] = round(out[name]["mlups"] / out["fp64"]["mlups"], 3)
     print("bench: batch leg done", file=sys.stderr, flush=True)
     return out
 
@@ -434,7 +448,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3, help="timed full solves")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp32-shift", action="store_true", help="skip the FP32-shift leg")
+    ap.add_argument("--no-fp32-shift", action="store_true", help="skip the FP32-shift and FP32-store legs")
     ap.add_argument("--mp-one-at-a-time", action="store_true",
                     help="propagate the dF tracers one after the other instead of together")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 roofline leg")
@@ -472,7 +486,8 @@ def main():
         _, _, jf = L.get_state(populations=False)
         permeability = (w["tau"] - 0.5) / 3.0 * float(jf[2][nat == 0].sum()) / N / w["fext"][2]
         del jf
-    shift = None if args.no_fp32_shift else fp32_shift_leg(w, nat, args, nf, nfi)
+    shift = None if args.no_fp32_shift else fp32_leg(w, nat, args, nf, nfi, lbx.PREC_FP32_SHIFT)
+    store = None if args.no_fp32_shift else fp32_leg(w, nat, args, nf, nfi, lbx.PREC_FP32_STORE)
     large = None if args.no_large else large_leg()
     slit = slit_leg()
     batch = None if args.no_batch else batch_leg()
@@ -527,12 +542,13 @@ def main():
         "deff": deffs,
         "permeability": permeability,
     }
-    if shift is not None:
-        d64 = np.array(deffs)
-        d32 = np.array(shift.pop("deff"))
-        shift["deff_rel_error_vs_fp64"] = float(f"{np.abs(d32 - d64).max() / np.abs(d64).max():.3e}")
-        shift["vs_fp64_value"] = round(shift["value"] / value, 3)
-        line["fp32_shift"] = shift
+    for name, leg in (("fp32_shift", shift), ("fp32_store", store)):
+        if leg is not None:
+            d64 = np.array(deffs)
+            d32 = np.array(leg.pop("deff"))
+            leg["deff_rel_error_vs_fp64"] = float(f"{np.abs(d32 - d64).max() / np.abs(d64).max():.3e}")
+            leg["vs_fp64_value"] = round(leg["value"] / value, 3)
+            line[name] = leg
     line["slit_4x4x34"] = slit
     if batch is not None:
         line["batch_1024"] = batch

@@ -8,7 +8,7 @@ module lbx_bind
 
     integer(c_int), parameter :: LBX_OK = 0, LBX_ERR_NEG_POP = 1, LBX_ERR_RESTPART = 2, LBX_ERR_TAU = 3, &
                                  LBX_ERR_NOMEM = 4, LBX_ERR_ARG = 5, LBX_ERR_DEVICE = 6, LBX_ERR_STATE = 7
-    integer(c_int32_t), parameter :: LBX_PREC_FP64 = 0, LBX_PREC_FP32_SHIFT = 1
+    integer(c_int32_t), parameter :: LBX_PREC_FP64 = 0, LBX_PREC_FP32_SHIFT = 1, LBX_PREC_FP32_STORE = 2
     integer(c_int32_t), parameter :: LBX_EXIT_CONVERGED = 1, LBX_EXIT_CAP = 2
 
     type(c_ptr), save :: lbx_lattice = c_null_ptr
@@ -145,7 +145,7 @@ contains
     ! the process-wide lattice context, created on first use
     ! The lattice, created on first use.  One lb.in key beyond the
     ! reference's selects the precision form: gpu_precision = fp64 (default,
-    ! the reference's arithmetic) or fp32_shift.
+    ! the reference's arithmetic), fp32_shift or fp32_store.
     subroutine lbx_ensure_lattice(tau)
         use system, only: supercell
         use module_input, only: getinput
@@ -164,8 +164,10 @@ contains
             prec = LBX_PREC_FP64
         case ("fp32_shift")
             prec = LBX_PREC_FP32_SHIFT
+        case ("fp32_store")
+            prec = LBX_PREC_FP32_STORE
         case default
-            print *, "gpu_precision must be fp64 or fp32_shift, found: ", pname
+            print *, "gpu_precision must be fp64, fp32_shift or fp32_store, found: ", pname
             stop
         end select
         call lbx_nature(nature)

@@ -1,20 +1,21 @@
-"""FP64 vs FP32-shift on one workload: per-output error table and throughput.
+"""FP64 vs the FP32-storage forms on one workload: per-output error table and
+throughput.  FP32-shift stores populations as FP32 deviations g = n - a0 rho0
+and collides in FP32; FP32-store keeps the same storage and collides in FP64.
 
 Default workload: a 256^3 Gaussian random porous medium (correlation length
 8 nodes, seed 20261202, thresholded to porosity 0.6, every solid cluster
 kept), tau = 0.8, f_ext = 1e-6 along x, 10000 flow steps from rest (the
 force switched on after the stage-1 convergence at t = 3), then 5000
 moment-propagation steps of a tracer with Db = 0.01 adsorbing at
-dF = -1 kT (kd = 0.01, ka = kd e).  Both precision forms run the same steps
-on the same medium; the FP64 form is the reference's arithmetic (bit-identical
-to it at every size the tests cover), so the differences are the FP32-shift
-form's error.
+dF = -1 kT (kd = 0.01, ka = kd e).  All forms run the same steps on the same
+medium; the FP64 form is the reference's arithmetic (bit-identical to it at
+every size the tests cover), so the differences are the FP32 forms' errors.
 
     python -m laboetie_amd.precision [--size 256] [--flow-steps 10000] [--mp-steps 5000]
 
 Prints one JSON object: the error of every output (populations n, density,
-flux j, permeability, VACF(t), D_eff) relative to its FP64 scale, and the
-MLUPS / HBM roofline of both forms.
+flux j, permeability, VACF(t), D_eff) of each FP32 form relative to its FP64
+scale, and the MLUPS / HBM roofline of every form.
 """
 import argparse
 import json
@@ -26,13 +27,14 @@ import numpy as np
 from . import lbx
 
 HBM_PEAK_GBS = 8000.0
-LB_BYTES = {lbx.PREC_FP64: 19 * 8 * 2, lbx.PREC_FP32_SHIFT: 19 * 4 * 2}  # per fluid node per step
+LB_BYTES = {lbx.PREC_FP64: 19 * 8 * 2, lbx.PREC_FP32_SHIFT: 19 * 4 * 2,
+            lbx.PREC_FP32_STORE: 19 * 4 * 2}  # populations read + written per fluid node per step
 
 
 def run(prec, dims, nat, tau, fext, flow_steps, mp_steps, Db, ka, kd):
     N = int(np.prod(dims))
     with lbx.Lattice(dims, nat, tau, precision=prec) as L:
-        if prec == lbx.PREC_FP32_SHIFT:
+        if prec != lbx.PREC_FP64:
             L.set_floor_window(0)  # a fixed step count, like the FP64 run
         L.init_rest(1.0)
         L.timing(True)
@@ -83,10 +85,11 @@ def main(argv=None):
     nat = lbx.geometry_random(*dims, 8.0, 0.6, args.seed)
     print(f"precision: medium {dims} built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     res = {}
-    for prec, name in ((lbx.PREC_FP64, "fp64"), (lbx.PREC_FP32_SHIFT, "fp32_shift")):
+    forms = (("fp64", lbx.PREC_FP64), ("fp32_shift", lbx.PREC_FP32_SHIFT), ("fp32_store", lbx.PREC_FP32_STORE))
+    for name, prec in forms:
         res[name] = run(prec, dims, nat, tau, fext, args.flow_steps, args.mp_steps, Db, ka, kd)
         print(f"precision: {name} done", file=sys.stderr, flush=True)
-    a, b = res["fp64"], res["fp32_shift"]
+    a = res["fp64"]
     fluid = nat == 0
     nu = (tau - 0.5) / 3.0
     N = fluid.size
@@ -94,29 +97,34 @@ def main(argv=None):
     def perm(r):  # Darcy permeability from the superficial velocity, k = nu <j_x>_box / F_x
         return nu * float(r["j"][0][fluid].sum()) / N / fext[0]
 
-    errors = {
-        "n": rel(b["n"], a["n"]),
-        "rho": rel(b["rho"][fluid], a["rho"][fluid]),
-        "j": rel(b["j"][:, fluid], a["j"][:, fluid]),
-        "permeability": abs(perm(b) - perm(a)) / abs(perm(a)),
-        "vacf": rel(b["vacf"], a["vacf"]),
-        "deff": float(np.abs(b["deff"] - a["deff"]).max() / np.abs(a["deff"]).max()),
-    }
+    def errors(b):
+        return {
+            "n": rel(b["n"], a["n"]),
+            "rho": rel(b["rho"][fluid], a["rho"][fluid]),
+            "j": rel(b["j"][:, fluid], a["j"][:, fluid]),
+            "permeability": abs(perm(b) - perm(a)) / abs(perm(a)),
+            "vacf": rel(b["vacf"], a["vacf"]),
+            "deff": float(np.abs(b["deff"] - a["deff"]).max() / np.abs(a["deff"]).max()),
+        }
+
     out = {
         "workload": f"Gaussian random medium {L_}^3 (sigma 8, porosity 0.6, seed {args.seed}), tau 0.8, "
                     f"F=(1e-6,0,0), {args.flow_steps} flow + {args.mp_steps} MP steps, dF=-1 kT",
         "fluid_nodes": a["fluid"],
-        "errors_vs_fp64": {k: float(f"{v:.3e}") for k, v in errors.items()},
         "error_scale": "max|x32 - x64| / max|x64| per output (D_eff: per component, relative to max |D|)",
-        "deff_fp64": a["deff"].tolist(), "deff_fp32_shift": b["deff"].tolist(),
-        "permeability_fp64": perm(a), "permeability_fp32_shift": perm(b),
+        "deff_fp64": a["deff"].tolist(), "permeability_fp64": perm(a),
     }
-    for name in ("fp64", "fp32_shift"):
+    for name in ("fp32_shift", "fp32_store"):
+        out[f"errors_vs_fp64_{name}"] = {k: float(f"{v:.3e}") for k, v in errors(res[name]).items()}
+        out[f"deff_{name}"] = res[name]["deff"].tolist()
+        out[f"permeability_{name}"] = perm(res[name])
+    for name, _ in forms:
         r = res[name]
         out[name] = {k: (round(v, 2) if isinstance(v, float) else v)
                      for k, v in r.items() if k in ("lb_mlups", "mp_mlups", "lb_kernel_us", "mp_kernel_us",
                                                     "lb_roofline")}
-    out["lb_speedup_fp32_shift"] = round(b["lb_mlups"] / a["lb_mlups"], 3)
+    for name in ("fp32_shift", "fp32_store"):
+        out[f"lb_speedup_{name}"] = round(res[name]["lb_mlups"] / a["lb_mlups"], 3)
     print(json.dumps(out), flush=True)
 
 

@@ -19,7 +19,7 @@ Two workloads:
   others run are admitted as soon as their nodes fit (continuous batching);
   latency is arrival -> result.
 
-    python -m laboetie_amd.serving batch [--instances 1024] [--precision fp64|fp32|both]
+    python -m laboetie_amd.serving batch [--instances 1024] [--precision fp64|fp32_shift|fp32_store|both|all]
     python -m laboetie_amd.serving serve [--requests 48] [--rates 0.5,1,2] [--hold 5]
                                          [--capacity-nodes N] [--capacity-box N]
 """
@@ -37,7 +37,8 @@ KD = lbx.KD_DEFAULT
 
 
 def _prec(name):
-    return {"fp64": lbx.PREC_FP64, "fp32": lbx.PREC_FP32_SHIFT, "fp32_shift": lbx.PREC_FP32_SHIFT}[name]
+    return {"fp64": lbx.PREC_FP64, "fp32": lbx.PREC_FP32_SHIFT, "fp32_shift": lbx.PREC_FP32_SHIFT,
+            "fp32_store": lbx.PREC_FP32_STORE}[name]
 
 
 # ------------------------------------------------------------------ workloads
@@ -206,7 +207,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=["batch", "serve"])
     ap.add_argument("--instances", type=int, default=1024)
-    ap.add_argument("--precision", default="both")
+    ap.add_argument("--precision", default="all",
+                    help="fp64, fp32_shift, fp32_store, both (fp64 + fp32_shift) or all; serve uses fp32_shift "
+                         "for both/all")
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--rates", default="0.5,1,2")
     ap.add_argument("--seed", type=int, default=20261202)
@@ -221,7 +224,8 @@ def main(argv=None):
         reqs = chromatography_requests(args.instances, args.seed)
         out = {"workload": f"{args.instances} chromatography 2D instances (1x50x50 / 1x100x100, 1-8 disks r=5), "
                            "tau 1, f_ext 1e-5 along y, target 1e-10 (cap 20000), 2000 MP steps, dF U[-3,0]"}
-        precs = ["fp64", "fp32_shift"] if args.precision == "both" else [args.precision]
+        precs = {"both": ["fp64", "fp32_shift"], "all": ["fp64", "fp32_shift", "fp32_store"]}.get(
+            args.precision, [args.precision])
         res = {}
         for p in precs:
             r = run_batch(reqs, _prec(p))
@@ -229,25 +233,33 @@ def main(argv=None):
             out[p] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in r.items()
                       if k in ("wall_s", "rounds", "tuned", "mlups", "mflups", "lb_kernel_mlups", "mp_kernel_mlups")}
             print(f"serving: batch {p} done in {r['wall_s']:.1f} s", file=sys.stderr, flush=True)
-        if len(precs) == 2:
-            d64 = np.array([x["deff"] for x in res["fp64"]["results"]])
-            d32 = np.array([x["deff"] for x in res["fp32_shift"]["results"]])
-            rel = np.abs(d32 - d64).max(axis=1) / np.abs(d64).max(axis=1)
-            out["deff_rel_error_fp32_shift"] = {"max": float(f"{rel.max():.3e}"),
-                                                "median": float(f"{np.median(rel):.3e}"),
-                                                "count_above_1e-5": int((rel > 1e-5).sum())}
-            out["speedup_fp32_shift_batch_vs_fp64_batch"] = round(res["fp32_shift"]["mlups"] / res["fp64"]["mlups"], 3)
+        for p in ("fp32_shift", "fp32_store"):
+            if p in res and "fp64" in res:
+                d64 = np.array([x["deff"] for x in res["fp64"]["results"]])
+                d32 = np.array([x["deff"] for x in res[p]["results"]])
+                rel = np.abs(d32 - d64).max(axis=1) / np.abs(d64).max(axis=1)
+                out[f"deff_rel_error_{p}"] = {"max": float(f"{rel.max():.3e}"), "median": float(f"{np.median(rel):.3e}"),
+                                              "count_above_1e-5": int((rel > 1e-5).sum())}
+                # the instances whose FP64 run converged (the others stop at the step cap,
+                # where the FP32 forms' floor exit can take a different step)
+                conv = np.array([x["exit_reason"] == lbx.EXIT_CONVERGED for x in res["fp64"]["results"]])
+                rc = rel[conv]
+                out[f"deff_rel_error_{p}_where_fp64_converged"] = {
+                    "instances": int(conv.sum()), "max": float(f"{rc.max():.3e}"),
+                    "median": float(f"{np.median(rc):.3e}"), "count_above_1e-5": int((rc > 1e-5).sum())}
+                out[f"speedup_{p}_batch_vs_fp64_batch"] = round(res[p]["mlups"] / res["fp64"]["mlups"], 3)
         if args.single_sample > 0:
             one = run_one_at_a_time(reqs[:args.single_sample], lbx.PREC_FP64)
             out["plain_fp64_one_at_a_time"] = {k: round(v, 2) if isinstance(v, float) else v for k, v in one.items()}
-            if "fp32_shift" in res:
-                out["speedup_fp32_shift_batch_vs_plain_fp64"] = round(res["fp32_shift"]["mlups"] / one["mlups"], 2)
+            for p in ("fp32_shift", "fp32_store"):
+                if p in res:
+                    out[f"speedup_{p}_batch_vs_plain_fp64"] = round(res[p]["mlups"] / one["mlups"], 2)
         print(json.dumps(out), flush=True)
         return
     # serve
     rng = np.random.default_rng(args.seed)
     reqs = [serving_request(rng) for _ in range(args.requests)]
-    prec = _prec("fp32_shift" if args.precision == "both" else args.precision)
+    prec = _prec("fp32_shift" if args.precision in ("both", "all") else args.precision)
     # HBM budget (default 48 M fluid nodes, ~20 GB in the FP32-shift form: room
     # for several of the largest requests at once)
     cap_nodes, cap_box = args.capacity_nodes, args.capacity_box
@@ -258,7 +270,8 @@ def main(argv=None):
     out = {"workload": f"{args.requests} requests per rate: N^3, N log-uniform in [32,192], slit/cylinder/BCC/"
                        "random(phi U[0.4,0.8]), f_ext log-uniform [1e-7,1e-5], target 1e-9 (cap 10000 LB steps), "
                        "2000 MP steps, dF U[-3,0]",
-           "precision": "fp32_shift" if prec == lbx.PREC_FP32_SHIFT else "fp64",
+           "precision": {lbx.PREC_FP64: "fp64", lbx.PREC_FP32_SHIFT: "fp32_shift",
+                         lbx.PREC_FP32_STORE: "fp32_store"}[prec],
            "capacity_nodes": cap_nodes, "capacity_box": cap_box, "hold_s": args.hold,
            "saturation_req_per_s": round(sat_rate, 3),
            "saturation_mlups": round(t["mlups"], 1)}

@@ -63,19 +63,21 @@ def test_dropin_matches_reference_program(name, mp_steps):
 
 @pytest.mark.skipif(not (os.path.exists(REF_BIN) and os.path.exists(GPU_BIN)),
                     reason="drop-in or reference binary not built (needs the reference sources at build time)")
-def test_dropin_fp32_shift_key():
-    """`gpu_precision = fp32_shift` in lb.in runs the same program in the
-    FP32-shift form: VACF within 1e-6 of VACF(0) of the reference program's
-    (FP64) and an unknown value stops with a message."""
+@pytest.mark.parametrize("key", ["fp32_shift", "fp32_store"])
+def test_dropin_fp32_precision_keys(key):
+    """`gpu_precision = fp32_shift` / `fp32_store` in lb.in runs the same
+    program in that FP32-storage form: VACF within 1e-6 of VACF(0) of the
+    reference program's (FP64), and an unknown value stops with a message."""
     g = load("bcc12_ads")
     mp_steps = 300
     lb_in = str(g["lb_in"]).replace(
         [ln for ln in str(g["lb_in"]).splitlines() if ln.startswith("maximum_moment_propagation_steps")][0],
         f"maximum_moment_propagation_steps = {mp_steps}")
+    print("drop-in key", key, flush=True)
     with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg, \
             tempfile.TemporaryDirectory() as db:
         rc_r, out_r = run_case(REF_BIN, lb_in, str(g["geom_in"]), dr)
-        rc_g, out_g = run_case(GPU_BIN, lb_in + "\ngpu_precision = fp32_shift\n", str(g["geom_in"]), dg)
+        rc_g, out_g = run_case(GPU_BIN, lb_in + f"\ngpu_precision = {key}\n", str(g["geom_in"]), dg)
         assert rc_r == 0 and rc_g == 0, out_g
         vr = np.loadtxt(os.path.join(dr, "output", "vacf.dat"), comments="#")
         vg = np.loadtxt(os.path.join(dg, "output", "vacf.dat"), comments="#")
@@ -83,4 +85,4 @@ def test_dropin_fp32_shift_key():
         scale = np.abs(vr[0, 1:]).max()
         assert np.all(np.abs(vg[:, 1:] - vr[:, 1:]) <= 1e-6 * scale)
         rc_b, out_b = run_case(GPU_BIN, lb_in + "\ngpu_precision = fp16\n", str(g["geom_in"]), db)
-        assert "gpu_precision must be fp64 or fp32_shift" in out_b
+        assert "gpu_precision must be fp64, fp32_shift or fp32_store" in out_b
