@@ -48,6 +48,7 @@ namespace {
 constexpr int kTile = 256;                 // allocation unit (nodes) of an instance's range
 constexpr uint32_t kNoInst = 0xFFFFFFFFu;
 constexpr int kRoundMax = 1024;            // launches per phase per round (vtrace rows)
+constexpr size_t kGraphCacheMax = 64;      // captured round graphs kept at once
 
 struct BInst {
     // geometry (set when the instance is added)
@@ -593,6 +594,10 @@ int run_phase(lbxb* b, int phase /*0 LB, 1 MP*/, int64_t S) {
         it = b->graph_cache.end();
     }
     if (it == b->graph_cache.end()) {
+        // one graph per (phase, round length): bounded by the callers' round
+        // sizes, but callers that vary them freely would grow the cache; start
+        // over past a few dozen entries
+        if (b->graph_cache.size() >= kGraphCacheMax) clear_graphs(b);
         hipGraph_t graph = nullptr;
         LBX_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         emit();

@@ -87,6 +87,29 @@ def test_batch_block128_no_graphs_and_mixed_rounds():
         assert np.all(np.abs(g["vacf"] - r["vacf"]) <= 1e-12 * scale)
 
 
+def test_batch_rounds_of_many_lengths_same_bits():
+    """Graph-replayed rounds of 1, 2, ..., 90 steps (more distinct round
+    lengths than the captured-graph cache keeps, so it starts over) give the
+    same bits as rounds of 64."""
+    cases = [Case(n) for n in ["bcc12_ads", "random_3d"]]
+    mp_steps = 120
+    ref, _ = run_batch(cases, lbx.PREC_FP64, mp_steps)
+    cap = sum(((int((c.nature == 0).sum()) + 255) // 256) * 256 for c in cases) + 4096
+    box = sum(int(np.prod(c.dims)) for c in cases) + 4096
+    with lbx.Batch(cap, box, len(cases) + 1) as B:
+        ids = [B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, mp_steps,
+                     max_lb_steps=20000) for c in cases]
+        k = 0
+        while B.round(1 + k % 90, 1 + k % 90) > 0:
+            k += 1
+            assert k < 5000
+        for c, iid, r in zip(cases, ids, ref):
+            g = B.result(iid, vacf_rows=mp_steps + 1, flow=True)
+            assert B.info(iid)["lb_steps"] == r["lb_steps"]
+            assert np.array_equal(g["rho"], r["rho"]) and np.array_equal(g["j"], r["j"])
+            assert np.array_equal(g["vacf"], r["vacf"]) and np.array_equal(g["deff"], r["deff"])
+
+
 def test_batch_admission_autotune_and_slot_reuse():
     c = Case("bcc12_ads")
     nf = int((c.nature == 0).sum())
