@@ -48,3 +48,36 @@ def test_footprints_grow_with_size_and_capacity():
     nodes, box = serving.capacity_for(reqs)
     assert nodes >= sum(int((r["nature"] == 0).sum()) for r in reqs)
     assert box >= sum(r["nature"].size for r in reqs)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; --gpus 2 exits at once with a message (no
+    device is touched, so this runs on the CPU)."""
+    import os
+    import subprocess
+    import sys
+
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(repo, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr and p.stdout == ""
+
+
+def test_geom_in_writer_round_trips():
+    """bench.py's geom.in writer (the reference's label-0 input) read back by
+    the reference-grammar reader gives the same solid nodes."""
+    import os
+    import sys
+
+    from laboetie_amd.lbin import read_geom_in
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+
+    dims = (5, 7, 3)
+    nat = (np.random.default_rng(1).random(105) < 0.3).astype(np.int8)
+    pts = read_geom_in(bench.geom_in_text(dims, nat))
+    back = np.zeros(105, np.int8)
+    back[(pts[:, 0] - 1) + 5 * ((pts[:, 1] - 1) + 7 * (pts[:, 2] - 1))] = 1
+    assert np.array_equal(back, nat)
