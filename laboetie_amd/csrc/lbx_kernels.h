@@ -50,9 +50,14 @@ struct Ctrl {
     double deff_s[12], deff_c[12]; // Neumaier running sums of the VACF integrals
 };
 
-__device__ __forceinline__ int32_t vload(const int32_t* p) { return *(const volatile int32_t*)p; }
-__device__ __forceinline__ int64_t vload(const int64_t* p) { return *(const volatile int64_t*)p; }
-__device__ __forceinline__ double vload(const double* p) { return *(const volatile double*)p; }
+// A value another block or an earlier launch wrote (control fields, l2err
+// slots, VACF partials): a relaxed device-scope atomic load, which reads past
+// this CU's L1 like a volatile one but, unlike it, does not wait for each load
+// before issuing the next (a volatile load is followed by a full wait).
+template <typename T>
+__device__ __forceinline__ T vload(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // A control flag as thread 0 of the block read it, the same for every wave:
 // another block may write the flag while this block's waves read it, and a
@@ -128,7 +133,7 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 
 // max over the kSlots slots of one step (called by the first wave of a block)
 __device__ __forceinline__ unsigned long long slots_max(const unsigned long long* set) {
-    return wave_max_u64(*(const volatile unsigned long long*)&set[threadIdx.x & (kSlots - 1)]);
+    return wave_max_u64(vload(&set[threadIdx.x & (kSlots - 1)]));
 }
 
 // Pull-streaming with half-way bounce-back (equilibration.f90:204-243): the
@@ -517,7 +522,7 @@ __device__ __forceinline__ void mp_reduce_block(const double* part, uint32_t npa
     for (uint32_t k = threadIdx.x; k < nparts; k += BLOCK)
 #pragma unroll
         for (int a = 0; a < NV; ++a) {
-            const double x = *(const volatile double*)&part[(uint64_t)NV * k + a];
+            const double x = vload(&part[(uint64_t)NV * k + a]);
             s[a] = (k == threadIdx.x) ? x : s[a] + x;
         }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
