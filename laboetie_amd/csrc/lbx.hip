@@ -50,6 +50,8 @@ struct lbx_ctx {
     bool wide_auto = false;  // what the size alone selects
     int32_t prec = LBX_PREC_FP64;
     double tau = 1.0, A = 0, B = 0, C = 0;
+    float A32 = 0, B32 = 0;      // FP32-shift collision: A + B = 1 exactly (fp32_relaxation)
+    int32_t check_every = 1;     // stage-2 steps between l2err measurements (kCheckEvery in FP32 storage)
     int64_t nfi = 0;
     int64_t floor_window = 0;    // FP32-storage forms: l2err floor test (0: off)
     int device = 0;
@@ -231,7 +233,7 @@ E* pop_buf(lbx_ctx* c, int b) {
 
 template <typename E, typename R = E>
 StepArgs<E, R> step_args(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3], int64_t check_after,
-                         double target, int write_l2) {
+                         double target, int write_l2, int32_t check_every, int64_t check_base) {
     StepArgs<E, R> a;
     a.fin = pop_buf<E>(c, (t - 1) & 1);
     a.fout = pop_buf<E>(c, t & 1);
@@ -246,8 +248,13 @@ StepArgs<E, R> step_args(lbx_ctx* c, int64_t t, const double fm[3], const double
     a.lz = c->lz;
     a.Nf = c->Nf;
     a.Np = c->Np;
-    a.A = (R)c->A;
-    a.B = (R)c->B;
+    if constexpr (std::is_same<R, float>::value) {
+        a.A = c->A32;
+        a.B = c->B32;
+    } else {
+        a.A = (R)c->A;
+        a.B = (R)c->B;
+    }
     a.C = (R)c->C;
     a.fmx = (R)(fm[0] / 2.0);
     a.fmy = (R)(fm[1] / 2.0);
@@ -262,25 +269,29 @@ StepArgs<E, R> step_args(lbx_ctx* c, int64_t t, const double fm[3], const double
     a.target = target;
     a.write_l2 = write_l2;
     a.have_prev = 0;
+    a.check_every = check_every;
+    a.check_base = check_base;
+    a.check_inv = (R)(1.0 / check_every);
     return a;
 }
 
 // one LB step kernel of the context's precision and addressing
 void launch_lb_step(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3], int64_t check_after,
-                    double target, int write_l2, int have_prev) {
+                    double target, int write_l2, int have_prev, int32_t check_every = 1, int64_t check_base = 0) {
     const uint32_t gt = c->ntiles;
     if (c->prec == LBX_PREC_FP32_SHIFT) {
-        StepArgs<float> a = step_args<float>(c, t, fm, fc, check_after, target, write_l2);
+        StepArgs<float> a = step_args<float>(c, t, fm, fc, check_after, target, write_l2, check_every, check_base);
         a.have_prev = have_prev;
         auto kern = c->wide ? k_lb_step<true, float> : k_lb_step<false, float>;
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
     } else if (c->prec == LBX_PREC_FP32_STORE) {
-        StepArgs<float, double> a = step_args<float, double>(c, t, fm, fc, check_after, target, write_l2);
+        StepArgs<float, double> a =
+            step_args<float, double>(c, t, fm, fc, check_after, target, write_l2, check_every, check_base);
         a.have_prev = have_prev;
         auto kern = c->wide ? k_lb_step<true, float, double> : k_lb_step<false, float, double>;
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
     } else {
-        StepArgs<double> a = step_args<double>(c, t, fm, fc, check_after, target, write_l2);
+        StepArgs<double> a = step_args<double>(c, t, fm, fc, check_after, target, write_l2, check_every, check_base);
         a.have_prev = have_prev;
         auto kern = c->wide ? k_lb_step<true, double> : k_lb_step<false, double>;
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
@@ -488,8 +499,10 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     c->dly.set((uint32_t)ly);
     c->prec = precision;
     c->floor_window = f32_storage(precision) ? 1000 : 0;
+    c->check_every = f32_storage(precision) ? kCheckEvery : 1;
     c->tau = tau;
     relaxation_coeffs(tau, &c->A, &c->B, &c->C);
+    fp32_relaxation(tau, &c->A32, &c->B32);
     (void)hipGetDevice(&c->device);
     std::vector<int8_t> itf(c->N);
     lbx_detect_interfacial(lx, ly, lz, nature, itf.data());
@@ -633,7 +646,7 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
                                c->stream));
     if (c->prec == LBX_PREC_FP32_SHIFT) {
         hipLaunchKernelGGL(k_collide_packed_shift<float>, dim3(gt), dim3(kBlock), 0, c->stream, c->g[0], c->rho,
-                           c->j, c->j32, 0.0f, 0.0f, 0.0f, 0u, c->Nf, c->Np, (float)c->A, (float)c->B, (float)c->C,
+                           c->j, c->j32, 0.0f, 0.0f, 0.0f, 0u, c->Nf, c->Np, c->A32, c->B32, (float)c->C,
                            c->rho0);
     } else if (c->prec == LBX_PREC_FP32_STORE) {  // j_old is j itself
         hipLaunchKernelGGL(k_collide_packed_shift<double>, dim3(gt), dim3(kBlock), 0, c->stream, c->g[0], c->rho,
@@ -664,10 +677,16 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
             break;
         }
         const double* Fs = stage == 1 ? zero : F;
+        // l2err of every step in stage 1; in stage 2 every check_every steps
+        // from the switch (every step in FP64)
+        const int32_t every = stage == 1 ? 1 : c->check_every;
+        const int64_t base = stage == 1 ? 0 : switch_step;
         timer_begin(c);
-        for (int64_t k = 1; k <= S; ++k) launch_lb_step(c, t + k, Fs, Fs, check_after, target_error, 1, t + k > 1);
+        for (int64_t k = 1; k <= S; ++k)
+            launch_lb_step(c, t + k, Fs, Fs, check_after, target_error, 1, t + k > 1, every, base);
         timer_stop(c);
-        hipLaunchKernelGGL(k_lb_finalize, dim3(1), dim3(kSlots), 0, c->stream, c->slots, c->l2ring, c->ctrl, t + S);
+        hipLaunchKernelGGL(k_lb_finalize, dim3(1), dim3(kSlots), 0, c->stream, c->slots, c->l2ring, c->ctrl, t + S,
+                           base, every);
         LBX_TRY(hipGetLastError());
         Ctrl h;
         LBX_TRY(hipMemcpyAsync(c->l2host.data(), c->l2ring, kRing * sizeof(double), hipMemcpyDeviceToHost,

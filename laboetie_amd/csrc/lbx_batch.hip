@@ -61,6 +61,9 @@ struct BInst {
     int32_t lb_on, lb_redo;       // takes part; the launch is the redo of step lb_t0 (force switch)
     int64_t lb_t0, lb_n;          // launch k runs step lb_t0 + k while k < lb_n
     int64_t check_after, floor_window;
+    int64_t check_base;           // l2err measured at check_base + m check_every (StepArgs)
+    int32_t check_every, pad2_;
+    double check_inv;             // 1 / check_every
     double fm[3], fc[3];          // F(t)/2 and the collision force (the stage's)
     // this round's MP work
     int32_t mp_on, any_ads;
@@ -122,6 +125,9 @@ __device__ __forceinline__ void kb_lb_tile(const BatchView<E, R>& v, uint32_t ti
     a.target = in.target;
     a.write_l2 = in.lb_redo ? 0 : 1;
     a.have_prev = (!in.lb_redo && t > 1) ? 1 : 0;
+    a.check_every = in.check_every;
+    a.check_base = in.check_base;
+    a.check_inv = (R)in.check_inv;
     const uint32_t first = in.tile_first * (kTile / BLOCK);
     lb_step_tile<BLOCK, Wide, E, R>(a, tile, tile - first, tile == first);
 }
@@ -137,7 +143,8 @@ __device__ __forceinline__ void kb_lb_finalize_inst(const BInst* insts, unsigned
                                                     Ctrl* ctrl, uint32_t id) {
     const BInst& in = insts[id];
     if (!in.lb_on || in.lb_redo || in.lb_n <= 0) return;
-    lb_finalize(slots + (uint64_t)id * 3 * kSlots, l2ring + (uint64_t)id * kRing, ctrl + id, in.lb_t0 + in.lb_n - 1);
+    lb_finalize(slots + (uint64_t)id * 3 * kSlots, l2ring + (uint64_t)id * kRing, ctrl + id, in.lb_t0 + in.lb_n - 1,
+                in.check_base, in.check_every);
 }
 
 __global__ __launch_bounds__(64) void kb_lb_finalize(const BInst* insts, unsigned long long* slots, double* l2ring,
@@ -906,6 +913,15 @@ int lbxb_add(lbxb* b, int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, 
     in.A = 1.0 - 1.0 / tau;  // module_collision.f90:99-101
     in.B = 1.0 / tau;
     in.C = 1.0 - 1.0 / (2.0 * tau);
+    if (b->prec == LBX_PREC_FP32_SHIFT) {  // FP32 collision: A + B = 1 exactly
+        float a32, b32;
+        fp32_relaxation(tau, &a32, &b32);
+        in.A = a32;
+        in.B = b32;
+    }
+    in.check_every = 1;
+    in.check_base = 0;
+    in.check_inv = 1.0;
     in.rho0 = rho0;
     in.target = target_error;
     in.floor_window = shift(b) ? 1000 : 0;
@@ -1029,6 +1045,11 @@ int lbxb_round(lbxb* b, int32_t lb_steps, int32_t mp_steps, int32_t* active) {
             in.lb_t0 = h.t + 1;
             in.lb_n = n;
             in.check_after = h.check_after;
+            // l2err of every step in stage 1; in stage 2 of the FP32 storage
+            // forms every kCheckEvery steps from the force switch
+            in.check_every = (h.stage == 2 && shift(b)) ? kCheckEvery : 1;
+            in.check_base = h.stage == 2 ? h.switch_step : 0;
+            in.check_inv = 1.0 / in.check_every;
             for (int a = 0; a < 3; ++a) {
                 const double F = h.stage == 1 ? 0.0 : h.F[a];
                 in.fm[a] = F / 2.0;

@@ -101,7 +101,8 @@ template <int L, typename V = float>
 __device__ __forceinline__ V collide_shift(V g, V drho, V jx, V jy, V jz, V ux, V uy, V uz, V fx, V fy, V fz, V A,
                                            V B, V C) {
     constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
-    constexpr V w0 = (V)a0(L), w1 = (V)a1(L), w2 = (V)a2(L);
+    // FP32: a1 with an exact momentum sum (a1_f32); FP64: a1 as is
+    constexpr V w0 = (V)a0(L), w1 = std::is_same<V, float>::value ? (V)a1_f32(L) : (V)a1(L), w2 = (V)a2(L);
     constexpr V Kx = (V)((double)(cx * cx) - kCsq);
     constexpr V Ky = (V)((double)(cy * cy) - kCsq);
     constexpr V Kz = (V)((double)(cz * cz) - kCsq);

@@ -113,7 +113,43 @@ struct StepArgs {
     double target;
     int32_t write_l2;       // 0 for a redo of a step whose l2err is already known
     int32_t have_prev;      // step t-1 ran in this run: reduce its l2err first
+    // Steps whose l2err is measured (is_check): every step when check_every
+    // is 1 (FP64, and stage 1 of the FP32 storage forms); else the steps
+    // check_base + m check_every (check_every a power of two), where l2err is
+    // max|j(t) - j(t - check_every)| / check_every and j is kept only there
+    int32_t check_every = 1;
+    int32_t pad_ = 0;
+    int64_t check_base = 0;
+    R check_inv = R(1);     // 1 / check_every
 };
+
+// A = 1 - 1/tau and B = 1/tau rounded to FP32 with A + B = 1 exactly, so that
+// the FP32 relaxation keeps the momentum (rounded one by one they can miss 1
+// by an ulp, a spurious force like the one a1_f32 removes).  1 - B is exact
+// for B in [1/2, 2]; for smaller B, B is re-derived from the rounded A, which
+// is exact for A in [1/2, 1].
+inline void fp32_relaxation(double tau, float* A, float* B) {
+    float b = (float)(1.0 / tau);
+    const float a = 1.0f - b;
+    if (!(b >= 0.5f)) b = 1.0f - a;
+    *A = a;
+    *B = b;
+}
+
+// The number of steps between the l2err measurements of the FP32 storage
+// forms' stage 2 (a power of two)
+constexpr int32_t kCheckEvery = 64;
+
+// In the FP32 storage forms, j(t) - j(t-1) at a node is of the order of the
+// rounding of j itself once the flow is nearly steady, so the per-step max|dj|
+// settles on a rounding floor well above what the flow's decay would give and
+// a stage would end early on the floor test.  Over check_every steps the decay
+// adds up while the rounding does not, so the difference divided by
+// check_every follows the decay down by that factor more; it also leaves j out
+// of the other steps' memory traffic.
+__host__ __device__ __forceinline__ bool is_check(int64_t s, int64_t base, int32_t every) {
+    return every <= 1 || (s - base) % every == 0;
+}
 
 // A negative population among a block's nodes is reported as +inf in place of
 // its max|dj| (max over bit patterns of non-negative doubles = max of values).
@@ -180,7 +216,7 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
     __shared__ unsigned long long prev_bits;
     __syncthreads();  // a previous step of this block (persistent launches) has read both
     if (threadIdx.x < 64) {
-        const bool stop = (vload(&a.ctrl->stop) != 0) | (vload(&a.ctrl->negpop) >= 0);
+        const bool stop = ((int)(vload(&a.ctrl->stop) != 0) | (int)(vload(&a.ctrl->negpop) >= 0)) != 0;
         const unsigned long long v = a.have_prev ? slots_max(a.slots + ((a.t - 1) % 3) * kSlots) : 0ull;
         if (threadIdx.x == 0) {
             stop_flag = stop ? 1 : 0;
@@ -197,7 +233,10 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
             if (first) a.ctrl->negpop = a.t - 1;
             return;
         }
-        const double prev = __longlong_as_double((long long)prev_bits);
+        // a step without a measurement (is_check false) records NaN and
+        // takes no stop decision
+        const bool chk = is_check(a.t - 1, a.check_base, a.check_every);
+        const double prev = chk ? __longlong_as_double((long long)prev_bits) : __builtin_nan("");
         if (first) a.l2ring[(a.t - 1) % kRing] = prev;
         bool floor = false;
         if (a.floor_window > 0 && a.t - 1 > a.check_after) {
@@ -206,11 +245,13 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
             const int s0 = (int)((a.t - 2) & 1), s1 = (int)((a.t - 1) & 1);
             double m = vload(&a.ctrl->l2min[s0]), mx = vload(&a.ctrl->l2max[s0]);
             int64_t ms = vload(&a.ctrl->l2min_step[s0]);
-            if (prev < m) {
-                m = prev;
-                ms = a.t - 1;
+            if (chk) {
+                if (prev < m) {
+                    m = prev;
+                    ms = a.t - 1;
+                }
+                mx = fmax(mx, prev);
             }
-            mx = fmax(mx, prev);
             if (first) {
                 a.ctrl->l2min[s1] = m;
                 a.ctrl->l2min_step[s1] = ms;
@@ -219,9 +260,9 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
             // a floor: no new minimum for floor_window steps after l2err has
             // fallen 100-fold below the stage's largest value (a plateau of a
             // still-accelerating flow is not one)
-            floor = (a.t - 1) - ms >= a.floor_window && m <= kFloorDrop * mx;
+            floor = chk && (a.t - 1) - ms >= a.floor_window && m <= kFloorDrop * mx;
         }
-        if (a.t - 1 > a.check_after && (prev <= a.target || floor)) {
+        if (chk && a.t - 1 > a.check_after && (prev <= a.target || floor)) {
             if (first) {
                 a.ctrl->exit_step = a.t - 1;
                 a.ctrl->floor_hit = prev <= a.target ? 0 : 1;
@@ -240,8 +281,11 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
         const View<Wide, R> jv(a.j, 3 * a.Np);
         Neighbours nb;
         nb.load(a.map, node & kIndexMask, a.dlx, a.dly, a.lz);
+        // l2err of this step is measured: always in FP64; in the FP32 storage
+        // forms only at the check steps, the only ones that read and keep j
+        const bool chk = a.write_l2 && is_check(a.t, a.check_base, a.check_every);
         R jo[3] = {R(0), R(0), R(0)};
-        if (a.write_l2) {  // j_old, issued before the populations so its latency overlaps theirs
+        if (chk) {  // j_old, issued before the populations so its latency overlaps theirs
             jo[0] = jv.ld(i);
             jo[1] = jv.ld(Np + i);
             jo[2] = jv.ld(2 * Np + i);
@@ -261,13 +305,20 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
             if constexpr (kCY[L] != 0) jy = jy + sgn<kCY[L]>(f[L]);
             if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(f[L]);
         });
-        if (a.write_l2) {
-            const R dx = fabs(jx - jo[0]), dy = fabs(jy - jo[1]), dz = fabs(jz - jo[2]);
+        if (chk) {
+            R dx = fabs(jx - jo[0]), dy = fabs(jy - jo[1]), dz = fabs(jz - jo[2]);
+            if constexpr (kShift) {  // exact: check_inv is a power of two
+                dx = dx * a.check_inv;
+                dy = dy * a.check_inv;
+                dz = dz * a.check_inv;
+            }
             l2 = (double)fmax(fmax(dx, dy), dz);
         }
-        jv.st(i, jx);
-        jv.st(Np + i, jy);
-        jv.st(2 * Np + i, jz);
+        if (!kShift || chk) {  // FP64: j of every step is the run's output flux
+            jv.st(i, jx);
+            jv.st(Np + i, jy);
+            jv.st(2 * Np + i, jz);
+        }
         if constexpr (kShift) {
             const R drho = rho, full = a.rho0 + drho;
             const R ux = jx / full, uy = jy / full, uz = jz / full;
@@ -312,20 +363,22 @@ __global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs<E, R>
 
 // l2err of the last step of a launch run, which no later step kernel reads
 // (called by one wave).
-__device__ __forceinline__ void lb_finalize(const unsigned long long* slots, double* l2ring, Ctrl* ctrl, int64_t t) {
+__device__ __forceinline__ void lb_finalize(const unsigned long long* slots, double* l2ring, Ctrl* ctrl, int64_t t,
+                                            int64_t check_base, int32_t check_every) {
     if (vload(&ctrl->stop) || vload(&ctrl->negpop) >= 0) return;
     const unsigned long long v = slots_max(slots + (t % 3) * kSlots);
     if (threadIdx.x == 0) {
         if (v == kNegMark)
             ctrl->negpop = t;
         else
-            l2ring[t % kRing] = __longlong_as_double((long long)v);
+            l2ring[t % kRing] =
+                is_check(t, check_base, check_every) ? __longlong_as_double((long long)v) : __builtin_nan("");
     }
 }
 
 __global__ __launch_bounds__(64) void k_lb_finalize(const unsigned long long* slots, double* l2ring, Ctrl* ctrl,
-                                                    int64_t t) {
-    lb_finalize(slots, l2ring, ctrl, t);
+                                                    int64_t t, int64_t check_base, int32_t check_every) {
+    lb_finalize(slots, l2ring, ctrl, t, check_base, check_every);
 }
 
 // Streaming only: the post-streaming state f_s(t) of the step whose input is

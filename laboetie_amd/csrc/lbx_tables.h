@@ -29,6 +29,16 @@ constexpr LBX_HD double a0(int l) { return wclass(l) == 0 ? kW0 : (wclass(l) == 
 constexpr LBX_HD double a1(int l) { return a0(l) / kCsq; }
 constexpr LBX_HD double a2(int l) { return a0(l) / (2 * (kCsq * kCsq)); }
 
+// a1 for a collision in FP32 arithmetic.  Rounded one by one, fl(1/6) and
+// fl(1/12) give sum_l a1_l c_lx^2 = 2 fl(1/6) + 8 fl(1/12) = 1 + 3.0e-8, so
+// every step adds 3e-8 j of momentum: a spurious force of 3e-8 j per step,
+// which relative to a body force F moves the steady flux by about
+// 3e-8 j / F (1e-3 at j / F = 3e4).  Here the edges take fl(1/12) and the
+// faces 1/2 - 4 fl(1/12), which FP32 holds exactly, so the sum is exactly 1.
+constexpr LBX_HD float a1_f32(int l) {
+    return wclass(l) == 2 ? (float)(kW2 / kCsq) : (wclass(l) == 1 ? 0.5f - 4.0f * (float)(kW2 / kCsq) : (float)a1(l));
+}
+
 // kBT = 1/3 (module_system.f90:94); lambda = 4 D / kBT (module_moment_propagation.f90:336)
 constexpr double kKbt = 1.0 / 3.0;
 

@@ -6,6 +6,7 @@
 subroutine equilibration
 
     use iso_c_binding, only: c_double, c_int64_t, c_int32_t, c_null_ptr
+    use, intrinsic :: ieee_arithmetic, only: ieee_is_nan
     use precision_kinds, only: dp
     use system, only: node, n
     use module_input, only: getinput
@@ -90,6 +91,9 @@ subroutine equilibration
         if (modulo(t, int(print_frequency, c_int64_t)) == 0) &
             print *, int(t), real(l2prev), "(target", real(target_error, 4), ")"
         if (t > held) exit
+        ! the FP32 storage forms measure l2err every 64 steps in stage 2: the
+        ! other steps (NaN in the trace) have no line
+        if (ieee_is_nan(trace(t))) cycle
         write (u13, *) int(t), trace(t)
         l2prev = trace(t)
     end do

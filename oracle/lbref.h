@@ -70,17 +70,20 @@ int lbref_mp_converged(int64_t it, const double vacf[3], int32_t lx, int32_t ly,
 void lbref_deff(const double* vacf, int64_t nsteps, double deff[3]);
 
 /* lbshift.c: the FP32-shift precision form (not in the reference), in the
- * per-node operation order of the library's FP32-shift kernels. */
+ * per-node operation order of the library's FP32-shift kernels.  check_every:
+ * stage 2 measures l2err = max|j(t) - j(t - check_every)| / check_every at the
+ * steps switch + m check_every only (a power of two; 1: every step); the
+ * trace holds NaN at the other steps. */
 int lbshift_equilibrate(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double rho0,
                         const double fext[3], double target_error, int64_t max_steps, int64_t floor_window,
-                        double* n, double* rho, double* jx, double* jy, double* jz, double* l2err_trace,
+                        int64_t check_every, double* n, double* rho, double* jx, double* jy, double* jz, double* l2err_trace,
                         int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason);
 
 /* The FP32-store form: the same FP32 deviation storage, collision and
  * j_old in FP64 (the library's LBX_PREC_FP32_STORE). */
 int lbstore_equilibrate(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double rho0,
                         const double fext[3], double target_error, int64_t max_steps, int64_t floor_window,
-                        double* n, double* rho, double* jx, double* jy, double* jz, double* l2err_trace,
+                        int64_t check_every, double* n, double* rho, double* jx, double* jy, double* jz, double* l2err_trace,
                         int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason);
 
 int lbshift_mp(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, const int8_t* interfacial,

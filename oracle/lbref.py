@@ -40,7 +40,7 @@ def lib():
         L.lbref_mp_step.argtypes = [_i32] * 3 + [_p] * 4 + [_f64] * 3 + [_i32] + [_p] * 5
         L.lbref_mp_converged.argtypes = [_i64, _p, _i32, _i32, _i32, _f64]
         L.lbref_deff.argtypes = [_p, _i64, _p]
-        L.lbshift_equilibrate.argtypes = [_i32] * 3 + [_p, _f64, _f64, _p, _f64, _i64, _i64] + [_p] * 6 + [_i64, _p, _p]
+        L.lbshift_equilibrate.argtypes = [_i32] * 3 + [_p, _f64, _f64, _p, _f64, _i64, _i64, _i64] + [_p] * 6 + [_i64, _p, _p]
         L.lbstore_equilibrate.argtypes = L.lbshift_equilibrate.argtypes
         L.lbshift_mp.argtypes = [_i32] * 3 + [_p] * 7 + [_f64] * 3 + [_i64] + [_p] * 3
         _lib = L
@@ -173,11 +173,13 @@ def deff(vacf):
 
 # ---------------------------------------------------------------- FP32-shift
 def equilibrate_shift(dims, nature, tau, rho0, fext, target_error, max_steps=0, floor_window=1000,
-                      trace_cap=1 << 20, arith="fp32"):
+                      trace_cap=1 << 20, arith="fp32", check_every=64):
     """lbshift_equilibrate (arith="fp32": the FP32-shift form) or
     lbstore_equilibrate (arith="fp64": the FP32-store form, collision in
     FP64): equilibrate with FP32 deviation storage (same outputs, FP64 n /
-    rho / j built from the FP32 deviations)."""
+    rho / j built from the FP32 deviations).  Stage 2 measures l2err every
+    check_every steps (the library's kCheckEvery = 64); the l2err trace is NaN
+    at the other steps."""
     lx, ly, lz = dims
     N = lx * ly * lz
     n = np.zeros(19 * N)
@@ -187,7 +189,7 @@ def equilibrate_shift(dims, nature, tau, rho0, fext, target_error, max_steps=0, 
     fe = np.ascontiguousarray(fext, np.float64)
     fn = lib().lbshift_equilibrate if arith == "fp32" else lib().lbstore_equilibrate
     st = fn(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)), tau, rho0, _ptr(fe), target_error, max_steps,
-            floor_window, _ptr(n), _ptr(rho), _ptr(jx), _ptr(jy), _ptr(jz), _ptr(tr), trace_cap, ctypes.byref(sd),
+            floor_window, check_every, _ptr(n), _ptr(rho), _ptr(jx), _ptr(jy), _ptr(jz), _ptr(tr), trace_cap, ctypes.byref(sd),
             ctypes.byref(er))
     steps = sd.value
     nl2 = steps - 1 if st == NEG_POP else steps

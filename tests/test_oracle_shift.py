@@ -3,14 +3,17 @@ the reference does not have: FP32-shift (FP32 deviations, FP32 collision)
 and FP32-store (the same storage, collision and j_old in FP64).  Their
 results against the FP64 restatement (pinned to the reference's own outputs)
 within each form's error bounds, exact invariants (rest state, conservation
-of sum_r (P + Pa)) and the bulk closed form D_eff = Db.
+of sum_r (P + Pa)), and closed forms: D_eff = Db in a bulk lattice at rest,
+and the mean flux of a periodic box under a constant force.
 
 The bounds below are the measured accuracy of the forms on these inputs
-with margin; the weakest case is the strong Poiseuille flow of the 4x4x34
-slit (Peclet ~25), where the flux error of the FP32-shift steady state
-(~2.5e-5 relative; FP32-store ~3.9e-6) carries into D_eff through Taylor
-dispersion.
+with margin.  The weakest case is the 4x4x34 slit (Peclet ~25, only 32
+distinct node states, so the rounding of the FP32 state does not average
+out): flux error 5.7e-5 of max|j| in FP32-shift (4.1e-6 in FP32-store),
+carried into D_eff through Taylor dispersion (1.0e-4; 7.5e-6).
 """
+from fractions import Fraction
+
 import numpy as np
 import pytest
 
@@ -18,6 +21,7 @@ import lbref
 from golden_cases import Case
 
 CASES = ["slit_4x4x34", "bcc12_ads", "cyl9_tau08", "random_3d"]
+CHECK = 64  # stage-2 steps between l2err measurements
 
 
 @pytest.fixture(scope="module")
@@ -35,17 +39,40 @@ def runs():
 
 @pytest.mark.parametrize("name", CASES)
 def test_shift_flow_within_fp32_bounds_of_fp64(runs, name):
+    """Flux within 1e-4 of max|j| plus 2e-10: the stop tolerance of 1e-10 on
+    max|dj| leaves a state difference of that order (the weak flow of
+    bcc12_ads, max|j| 4e-6, stops 66 steps before the FP64 run)."""
     c, a, b, _ = runs[name]
     assert b["status"] == lbref.OK
     assert b["exit_reason"] in (lbref.EXIT_CONVERGED, lbref.EXIT_FLOOR, lbref.EXIT_CAP)
     jm = np.abs(a["j"]).max()
-    assert np.abs(a["j"] - b["j"]).max() <= 5e-5 * jm
-    assert np.abs(a["rho"] - b["rho"]).max() <= 1e-7
+    assert np.abs(a["j"] - b["j"]).max() <= 1e-4 * jm + 2e-10
+    assert np.abs(a["rho"] - b["rho"]).max() <= 1e-8
     assert np.abs(a["n"] - b["n"]).max() <= 1e-7
-    # the l2err trace follows the FP64 one until the FP32 floor
-    k = min(len(a["l2err"]), len(b["l2err"]), 100)
-    big = a["l2err"][:k] > 1e-6
-    assert np.all(np.abs(a["l2err"][:k][big] - b["l2err"][:k][big]) <= 1e-3 * a["l2err"][:k][big])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_check_steps_of_the_l2err_trace(runs, name):
+    """Stage 1 measures every step (0 at t = 1..3: the rest state is a fixed
+    point, so the force switch comes at t = 3); stage 2 measures only at
+    t = 3 + 64 m, the other entries are NaN.  The first measurement is
+    max|j(67) - j(3)| / 64 = max|j(67)| / 64, and j(67) is within FP32
+    rounding of the FP64 flux of step 67."""
+    c, _, b, s = runs[name]
+    for r in (b, s):
+        tr = r["l2err"]
+        assert list(tr[:3]) == [0.0, 0.0, 0.0]
+        t = np.arange(1, len(tr) + 1)
+        measured = (t <= 3) | ((t - 3) % CHECK == 0)
+        assert np.all(np.isnan(tr[~measured]))
+        assert not np.any(np.isnan(tr[measured]))
+        if r["exit_reason"] == lbref.EXIT_CONVERGED:
+            assert r["steps"] % CHECK == 3 and tr[-1] <= c.target
+    a67 = lbref.equilibrate(c.dims, c.nature, c.tau, c.rho0, c.fext, 1e-30, max_steps=3 + CHECK)
+    assert a67["steps"] == 3 + CHECK
+    want = np.abs(a67["j"]).max() / CHECK
+    for r in (b, s):
+        assert abs(r["l2err"][2 + CHECK] - want) <= 1e-5 * want
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -59,7 +86,7 @@ def test_shift_mp_within_bounds_and_conserving(runs, name):
     st2, v2, P, Pa = lbref.mp_shift(c.dims, c.nature, itf, b["rho"], b["j"], c.fext, c.Db, c.ka, c.kd, c.mp_steps)
     assert st2 == lbref.OK
     d, d2 = lbref.deff(v), lbref.deff(v2)
-    assert np.abs(d - d2).max() <= 1e-4 * np.abs(d).max()
+    assert np.abs(d - d2).max() <= 2e-4 * np.abs(d).max()
     # the FP32-rounded propagator conserves sum_r (P + Pa) up to FP64 rounding
     _, _, P1, Pa1 = lbref.mp_shift(c.dims, c.nature, itf, b["rho"], b["j"], c.fext, c.Db, c.ka, c.kd, 0)
     for a_ in range(3):
@@ -97,26 +124,83 @@ def test_shift_bulk_deff_equals_db():
         assert np.all(np.abs(d - Db) <= 1e-7 * Db)
 
 
-def test_shift_floor_exit_on_strong_flow(runs):
-    """The strong slit flow reaches the FP32 rounding floor (l2err cycling
-    above 1e-10): the floor test ends it instead of running forever."""
-    c, a, b, _ = runs["slit_4x4x34"]
-    assert b["exit_reason"] == lbref.EXIT_FLOOR
-    assert b["l2err"][-1] > c.target
+def _disk(n):
+    y, z = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    return (((y - n / 2) ** 2 + (z - n / 2) ** 2) <= (n / 5) ** 2).astype(np.int8).ravel()
+
+
+def test_floor_exit_on_an_unreachable_target():
+    """A 1x20x20 disk at F = 1e-4 (max|j| 0.018) with target 1e-13: the FP64
+    run converges (2594 steps); FP32 storage cannot resolve 1e-13 at that
+    flux, so both forms end on the floor test, with the flux still within
+    1e-5 of max|j| of the FP64 one."""
+    dims, nat, F = (1, 20, 20), _disk(20), [0.0, 1e-4, 0.0]
+    a = lbref.equilibrate(dims, nat, 1.0, 1.0, F, 1e-13, max_steps=40000)
+    assert a["exit_reason"] == lbref.EXIT_CONVERGED
+    jm = np.abs(a["j"]).max()
+    for arith in ("fp32", "fp64"):
+        b = lbref.equilibrate_shift(dims, nat, 1.0, 1.0, F, 1e-13, max_steps=40000, arith=arith)
+        assert b["exit_reason"] == lbref.EXIT_FLOOR
+        assert np.nanmin(b["l2err"][3:]) > 1e-13
+        assert np.abs(a["j"] - b["j"]).max() <= 1e-5 * jm
+
+
+@pytest.mark.parametrize("tau", [1.0, 0.8, 0.7, 3.0])
+def test_periodic_box_mean_flux_closed_form(tau):
+    """Fully periodic 4x4x4 box, F = 1e-5 along x from step 4 (stage 1 ends
+    at t = 3): every step adds F of momentum except the first, which adds
+    C F (C = 1 - 1/(2 tau), its collision's j has no F/2), so the mean flux
+    of step t is (t - 3.5 + C) F exactly.  The FP64 restatement meets it to
+    1e-12.  In FP32 the collision keeps the momentum only if its rounded
+    constants do (a1 with sum a1 c_x^2 = 1, A + B = 1): with fl(1/6) and
+    fl(1/12) the excess 3e-8 j per step drifts the flux by 3.5e-5 to 8.4e-5
+    relative over these 2000 steps; with them, what is left is the rounding
+    of the stored deviations (< 7e-6; FP32-store < 1.3e-6)."""
+    dims, nat, F = (4, 4, 4), np.zeros(64, np.int8), [1e-5, 0.0, 0.0]
+    C = 1.0 - 1.0 / (2.0 * tau)
+    for form, bound in (("fp64_ref", 1e-12), ("fp32", 1e-5), ("fp64", 2e-6)):
+        if form == "fp64_ref":
+            r = lbref.equilibrate(dims, nat, tau, 1.0, F, 1e-30, max_steps=2000)
+        else:
+            r = lbref.equilibrate_shift(dims, nat, tau, 1.0, F, 1e-30, max_steps=2000, arith=form)
+        assert r["steps"] == 2000 and r["exit_reason"] == lbref.EXIT_CAP
+        want = (r["steps"] - 3.5 + C) * F[0]
+        assert abs(r["j"][0].mean() / want - 1.0) <= bound, form
+        assert np.abs(r["j"][1:]).max() <= 1e-12
+
+
+def test_fp32_collision_constants_conserve_momentum():
+    """The FP32 a1 (fl(1/12) on the edges, 1/2 - 4 fl(1/12) on the faces) has
+    sum_l a1 c_x^2 = 2 a1_face + 8 a1_edge = 1 exactly; fl(1/6), fl(1/12)
+    rounded one by one give 1 + 2.98e-8.  And the FP32 relaxation pair
+    (B = fl(1/tau), A = 1 - B, B re-derived from A below 1/2) sums to 1
+    exactly for every tau tried."""
+    edge = np.float32(1.0 / 12.0)
+    face = np.float32(0.5) - np.float32(4.0) * edge
+    assert 2 * Fraction(float(face)) + 8 * Fraction(float(edge)) == 1
+    naive = 2 * Fraction(float(np.float32(1.0 / 6.0))) + 8 * Fraction(float(edge))
+    assert abs(float(naive - 1) - 2.98e-8) < 1e-10
+    for tau in np.concatenate([np.linspace(0.5, 5.0, 451), [0.7, 0.51, 1.0 / 0.7, 7.3, 100.0]]):
+        b = np.float32(1.0 / tau)
+        a = np.float32(1.0) - b
+        if not b >= np.float32(0.5):
+            b = np.float32(1.0) - a
+        assert Fraction(float(a)) + Fraction(float(b)) == 1, tau
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_store_flow_closer_to_fp64_than_shift(runs, name):
     """FP32-store: the same FP32 storage with the collision in FP64 puts the
-    steady flux within 1e-5 of max|j| on every case (3.9e-6 on the slit) and
-    nearer the FP64 flux than the FP32-shift form."""
+    steady flux within 1e-5 of max|j| (plus the 2e-10 of the stop tolerance)
+    on every case (4.1e-6 on the slit) and no farther from the FP64 flux than
+    the FP32-shift form."""
     c, a, b, s = runs[name]
     assert s["status"] == lbref.OK
     assert s["exit_reason"] in (lbref.EXIT_CONVERGED, lbref.EXIT_FLOOR, lbref.EXIT_CAP)
     jm = np.abs(a["j"]).max()
     err_store, err_shift = np.abs(a["j"] - s["j"]).max(), np.abs(a["j"] - b["j"]).max()
-    assert err_store <= 1e-5 * jm
-    assert err_store < err_shift
+    assert err_store <= 1e-5 * jm + 2e-10
+    assert err_store <= err_shift
     assert np.abs(a["rho"] - s["rho"]).max() <= 1e-8
     assert np.abs(a["n"] - s["n"]).max() <= 1e-8
 

@@ -7,6 +7,8 @@ arithmetic in the same order): the l2err trace, exit step and reason, and
 the FP64 n, rho, j built from the FP32 deviations; P and Pa of the moment
 propagation; VACF to 1e-12 of |VACF(0)| (summation order differs).  Against
 the FP64 path (the reference's results) each form's own error bounds hold.
+In stage 2 both forms measure l2err every 64 steps; the trace holds NaN at
+the other steps, in the library and the restatement alike.
 """
 import sys
 
@@ -42,7 +44,7 @@ def test_shift_matches_its_restatement(name, cap, prec):
         L.init_rest(c.rho0)
         r = L.equilibrate(c.fext, c.target, max_steps=cap)
         assert (r["steps"], r["exit_reason"]) == (ref["steps"], ref["exit_reason"])
-        assert np.array_equal(r["l2err"], ref["l2err"])
+        assert np.array_equal(r["l2err"], ref["l2err"], equal_nan=True)
         n, rho, j = L.get_state()
         assert np.array_equal(n, ref["n"]) and np.array_equal(rho, ref["rho"]) and np.array_equal(j, ref["j"])
         steps = min(c.mp_steps, 300)
@@ -62,8 +64,9 @@ def test_shift_matches_its_restatement(name, cap, prec):
 @pytest.mark.parametrize("name", ["bcc12_ads", "random_3d", "cyl9_tau08"])
 def test_shift_error_against_fp64(name, prec):
     """FP32-shift and FP32-store vs FP64 on the GPU: flux within 1e-5 of
-    max|j| (FP32-store: 2e-6), D_eff within 1e-6 relative on these weak
-    flows."""
+    max|j| (FP32-store: 2e-6) plus 2e-10, the state difference the stop
+    tolerance of 1e-10 on max|dj| leaves (the weak flow of bcc12_ads, max|j|
+    4e-6, stops 66 steps apart), D_eff within 1e-6 relative."""
     c = Case(name)
     out = []
     for prec_ in (lbx.PREC_FP64, prec):
@@ -76,9 +79,40 @@ def test_shift_error_against_fp64(name, prec):
             out.append((r, rho, j, L.mp_deff()))
     (r64, rho64, j64, d64), (r32, rho32, j32, d32) = out
     say(name, "steps", r64["steps"], r32["steps"], "exit", r32["exit_reason"])
-    assert np.abs(j32 - j64).max() <= (1e-5 if prec == SHIFT else 2e-6) * np.abs(j64).max()
+    assert np.abs(j32 - j64).max() <= (1e-5 if prec == SHIFT else 2e-6) * np.abs(j64).max() + 2e-10
     assert np.abs(rho32 - rho64).max() <= 1e-8
     assert np.all(np.abs(d32 - d64) <= 1e-6 * np.abs(d64).max())
+
+
+def _disk(n):
+    """1 x n x n, periodic, one solid disk of radius n/5 in the middle"""
+    y, z = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    return (((y - n / 2) ** 2 + (z - n / 2) ** 2) <= (n / 5) ** 2).astype(np.int8).ravel()
+
+
+@pytest.mark.parametrize("prec", [SHIFT, STORE])
+@pytest.mark.parametrize("case", ["floor", "box_tau1", "box_tau07", "box_tau3"])
+def test_fp32_forms_floor_and_periodic_box_match_restatement(case, prec):
+    """Bit-identical to the restatement on two more kinds of run: a floor
+    exit (the 1x20x20 disk at F = 1e-4 with target 1e-13, which FP32 storage
+    cannot reach), and a periodic box that accelerates without end for 2000
+    steps at tau = 1, 0.7 (1/tau inexact in FP32) and 3 (1/tau < 1/2): the
+    FP32 relaxation coefficients with A + B = 1 are the restatement's."""
+    if case == "floor":
+        dims, nat, tau, F, target, cap = (1, 20, 20), _disk(20), 1.0, [0.0, 1e-4, 0.0], 1e-13, 40000
+    else:
+        tau = {"box_tau1": 1.0, "box_tau07": 0.7, "box_tau3": 3.0}[case]
+        dims, nat, F, target, cap = (4, 4, 4), np.zeros(64, np.int8), [1e-5, 0.0, 0.0], 1e-30, 2000
+    say("fp32 storage", prec, case)
+    ref = lbref.equilibrate_shift(dims, nat, tau, 1.0, F, target, max_steps=cap, arith=ARITH[prec])
+    with lbx.Lattice(dims, nat, tau, precision=prec) as L:
+        L.init_rest(1.0)
+        r = L.equilibrate(F, target, max_steps=cap)
+        assert (r["steps"], r["exit_reason"]) == (ref["steps"], ref["exit_reason"])
+        assert r["exit_reason"] == (lbx.EXIT_FLOOR if case == "floor" else lbx.EXIT_CAP)
+        assert np.array_equal(r["l2err"], ref["l2err"], equal_nan=True)
+        n, rho, j = L.get_state()
+        assert np.array_equal(n, ref["n"]) and np.array_equal(rho, ref["rho"]) and np.array_equal(j, ref["j"])
 
 
 def test_shift_batched_tracers_and_floor_window():

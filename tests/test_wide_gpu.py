@@ -63,8 +63,8 @@ def test_forced_64bit_addressing_same_bits(name, prec):
     say("64-bit addressing vs buffer offsets", name, prec)
     a = solve(c, prec, False)
     b = solve(c, prec, True)
-    for k in a:
-        assert np.array_equal(a[k], b[k]), k
+    for k in a:  # the FP32 forms' l2err trace is NaN between check steps
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
 
 
 def test_batch_forced_64bit_addressing_same_bits():
