@@ -43,11 +43,12 @@ extern "C" {
 
 /* precision forms */
 #define LBX_PREC_FP64        0  /* populations and arithmetic in FP64 (parity form) */
-#define LBX_PREC_FP32_SHIFT  1  /* populations stored as FP32 f_l - w_l*rho0, FP32 collision,
-                                   FP64 moments/reductions/VACF/D_eff */
-#define LBX_PREC_FP32_STORE  2  /* the same FP32 storage, collision and j_old in FP64 (half the
-                                   FP64 form's population bytes at a fraction of the FP32-shift
-                                   form's error) */
+#define LBX_PREC_FP32_SHIFT  1  /* populations stored as FP32 f_l - w_l*rho0, FP32 collision
+                                   (constants rounded to conserve momentum), FP64
+                                   moments/reductions/VACF/D_eff */
+#define LBX_PREC_FP32_STORE  2  /* the same FP32 storage, collision and the stop test's flux in
+                                   FP64 (half the FP64 form's population bytes at a fraction of
+                                   the FP32-shift form's error) */
 
 /* lbx_equilibrate exit reasons */
 #define LBX_EXIT_CONVERGED 1    /* second convergence, equilibration.f90:373-374 */
@@ -134,6 +135,10 @@ int lbx_set_state(lbx_ctx* ctx, const double* n, const double* rho,
  * fluid node; the run exits at the next convergence.  max_steps > 0 caps the
  * number of steps (extension).  l2err_trace (may be NULL) receives the first
  * trace_cap values of the per-step l2err (what l2err.dat holds).
+ * FP32-storage forms (extension): stage 2 measures l2err only at every 64th
+ * step after the force switch, as max|j(t) - j(t-64)| / 64 (the per-step
+ * difference sits on the FP32 rounding of j); the stop tests apply there,
+ * and the trace holds NaN at the other steps.
  * Returns LBX_ERR_NEG_POP with *steps_done = the failing step. */
 int lbx_equilibrate(lbx_ctx* ctx, double target_error, const double fext[3], int64_t max_steps,
                     double* l2err_trace, int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason);

@@ -96,7 +96,9 @@ __device__ __forceinline__ double collide_pop(double f, double rho, double jx, d
 // V = float (the FP32-shift form) runs all of it in FP32: the deviations
 // stay small, so their rounding error is relative to the deviation, not to
 // rho0.  V = double (the FP32-store form) runs the same operations in FP64 on
-// the FP32-stored deviations.
+// the FP32-stored deviations.  In FP32 the constants are rounded so that the
+// collision still conserves momentum: a1 from a1_f32, and A, B from
+// fp32_relaxation (A + B = 1).
 template <int L, typename V = float>
 __device__ __forceinline__ V collide_shift(V g, V drho, V jx, V jy, V jz, V ux, V uy, V uz, V fx, V fy, V fz, V A,
                                            V B, V C) {
