@@ -84,5 +84,11 @@ def test_dropin_fp32_precision_keys(key):
         assert vr.shape == vg.shape
         scale = np.abs(vr[0, 1:]).max()
         assert np.all(np.abs(vg[:, 1:] - vr[:, 1:]) <= 1e-6 * scale)
+        # l2err.dat: every stage-1 step, then only the measured steps (every 64th
+        # after the force switch at t = 3), all finite
+        l2 = np.loadtxt(os.path.join(dg, "output", "l2err.dat"))
+        steps = l2[:, 0].astype(int)
+        assert list(steps[:3]) == [1, 2, 3] and np.all(np.isfinite(l2[:, 1]))
+        assert np.all((steps[3:] - 3) % 64 == 0) and np.all(np.diff(steps) > 0)
         rc_b, out_b = run_case(GPU_BIN, lb_in + "\ngpu_precision = fp16\n", str(g["geom_in"]), db)
         assert "gpu_precision must be fp64, fp32_shift or fp32_store" in out_b
