@@ -102,6 +102,32 @@ int lbx_geometry_random(int32_t lx, int32_t ly, int32_t lz, double sigma, double
 int lbx_geometry_disks(int32_t lx, int32_t ly, int32_t lz, int32_t ndisks, double radius, uint64_t seed,
                        int8_t* nature, double* centres);
 
+/* ---------------------------------------------------------- input files
+ * lb.in: the reference's parser (module_input.f90:561-633, the tag lookup of
+ * input_dp and siblings :62-75): lines left-adjusted, cut to 100 characters,
+ * '#' comments and blank lines dropped; a tag matches the first line that
+ * starts with it followed by a blank, and its value is read list-directed
+ * (blanks or commas, '/' ends; d exponents) from columns len(tag)+4 ..
+ * len(tag)+50.  Each getter sets *found = 0 and leaves out untouched when no
+ * line holds the tag (the caller's default applies), and returns LBX_ERR_ARG
+ * on too few or malformed values.  lbx_lbin_dp / lbx_lbin_int read count
+ * values (getinput%dp, %dp3, %int); lbx_lbin_log reads a logical (T/F,
+ * .true./.false.), lbx_lbin_char the first token without quotes.
+ * lbx_geom_in_parse: geom.in of the custom geometry (construct_custom,
+ * module_geometry.f90:380-426): 1-based `i j k` solid nodes, one per line;
+ * nature (lx,ly,lz, x fastest) becomes 1 there and 0 elsewhere; an index
+ * out of range is LBX_ERR_ARG (the reference stops); *nsolid (may be NULL)
+ * receives the number of distinct solid nodes. */
+typedef struct lbx_lbin lbx_lbin;
+int lbx_lbin_parse(const char* text, int64_t len, lbx_lbin** out);
+int lbx_lbin_free(lbx_lbin* h);
+int lbx_lbin_dp(const lbx_lbin* h, const char* tag, int32_t count, double* out, int32_t* found);
+int lbx_lbin_int(const lbx_lbin* h, const char* tag, int32_t count, int64_t* out, int32_t* found);
+int lbx_lbin_log(const lbx_lbin* h, const char* tag, int32_t* out, int32_t* found);
+int lbx_lbin_char(const lbx_lbin* h, const char* tag, char* out, int64_t cap, int32_t* found);
+int lbx_geom_in_parse(const char* text, int64_t len, int32_t lx, int32_t ly, int32_t lz, int8_t* nature,
+                      int64_t* nsolid);
+
 /* --------------------------------------------------------------- context */
 
 /* HBM bytes a context of this shape needs (admission control for batching

@@ -19,14 +19,17 @@ class LbInError(ValueError):
 
 
 def _fortran_lines(text):
+    # read '(a)' into character(len=100) keeps the first 100 characters;
+    # adjustl / trim shift blanks only (a tab stays a tab, so a tag followed
+    # by a tab is not found, as in the reference)
     lines = []
     for raw in text.splitlines():
-        line = raw.replace("\t", " ").lstrip()[:LINE_LEN].rstrip()
+        line = raw[:LINE_LEN].lstrip(" ").rstrip(" ")
         k = line.find("#")
         if k >= 0:
             line = line[:k]
-        line = line.strip()
-        if line and line[0] != " ":
+        line = line.strip(" ")
+        if line:
             lines.append(line.ljust(LINE_LEN))
     return lines
 

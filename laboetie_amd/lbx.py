@@ -35,6 +35,13 @@ SIGNATURES = {
     "lbx_gaussian_field": (ctypes.c_int, [_i32] * 3 + [_f64, ctypes.c_uint64, _p]),
     "lbx_geometry_random": (ctypes.c_int, [_i32] * 3 + [_f64, _f64, ctypes.c_uint64, _p]),
     "lbx_geometry_disks": (ctypes.c_int, [_i32] * 4 + [_f64, ctypes.c_uint64, _p, _p]),
+    "lbx_lbin_parse": (ctypes.c_int, [ctypes.c_char_p, _i64, _p]),
+    "lbx_lbin_free": (ctypes.c_int, [_p]),
+    "lbx_lbin_dp": (ctypes.c_int, [_p, ctypes.c_char_p, _i32, _p, _p]),
+    "lbx_lbin_int": (ctypes.c_int, [_p, ctypes.c_char_p, _i32, _p, _p]),
+    "lbx_lbin_log": (ctypes.c_int, [_p, ctypes.c_char_p, _p, _p]),
+    "lbx_lbin_char": (ctypes.c_int, [_p, ctypes.c_char_p, _p, _i64, _p]),
+    "lbx_geom_in_parse": (ctypes.c_int, [ctypes.c_char_p, _i64, _i32, _i32, _i32, _p, _p]),
     "lbx_footprint_bytes": (_i64, [_i32] * 5),
     "lbx_create": (ctypes.c_int, [_i32] * 3 + [_p, _f64, _i32, _p]),
     "lbx_destroy": (ctypes.c_int, [_p]),
@@ -159,6 +166,72 @@ def geometry_disks(lx, ly, lz, ndisks, radius=5.0, seed=1):
     _check(lib().lbx_geometry_disks(lx, ly, lz, int(ndisks), float(radius), int(seed), _ptr(nat), _ptr(c)),
            "geometry_disks")
     return nat, c[:3 * ndisks].reshape(-1, 3)
+
+
+class LbInNative:
+    """lb.in through the library's C++ reader (lbx_lbin_*), the reference's
+    grammar; getters return the default when the tag is absent."""
+
+    def __init__(self, text):
+        raw = text.encode() if isinstance(text, str) else bytes(text)
+        h = _p()
+        _check(lib().lbx_lbin_parse(raw, len(raw), ctypes.byref(h)), "lbin_parse")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().lbx_lbin_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _get(self, fn, tag, count, ctype, default):
+        out = (ctype * count)()
+        found = _i32()
+        _check(fn(self._h, tag.encode(), count, out, ctypes.byref(found)), f"lbin {tag}")
+        if not found.value:
+            return default
+        return out[0] if count == 1 else list(out)
+
+    def dp(self, tag, default=None):
+        return self._get(lib().lbx_lbin_dp, tag, 1, _f64, default)
+
+    def dp3(self, tag, default=None):
+        return self._get(lib().lbx_lbin_dp, tag, 3, _f64, default)
+
+    def int(self, tag, default=None):
+        return self._get(lib().lbx_lbin_int, tag, 1, _i64, default)
+
+    def int3(self, tag, default=None):
+        return self._get(lib().lbx_lbin_int, tag, 3, _i64, default)
+
+    def log(self, tag, default=None):
+        out, found = _i32(), _i32()
+        _check(lib().lbx_lbin_log(self._h, tag.encode(), ctypes.byref(out), ctypes.byref(found)), f"lbin {tag}")
+        return bool(out.value) if found.value else default
+
+    def char(self, tag, default=None):
+        buf = ctypes.create_string_buffer(128)
+        found = _i32()
+        _check(lib().lbx_lbin_char(self._h, tag.encode(), buf, 128, ctypes.byref(found)), f"lbin {tag}")
+        return buf.value.decode() if found.value else default
+
+    def has(self, tag):
+        return self.char(tag) is not None
+
+
+def geom_in_nature(text, lx, ly, lz):
+    """geom.in -> nature (lx,ly,lz, x fastest) through lbx_geom_in_parse;
+    returns (nature, number of distinct solid nodes)."""
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    nat = np.zeros(lx * ly * lz, np.int8)
+    ns = _i64()
+    _check(lib().lbx_geom_in_parse(raw, len(raw), lx, ly, lz, _ptr(nat), ctypes.byref(ns)), "geom_in_parse")
+    return nat, ns.value
 
 
 def footprint_bytes(lx, ly, lz, precision=PREC_FP64, with_mp=1):
