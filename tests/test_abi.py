@@ -28,6 +28,16 @@ def test_library_exports_every_declared_symbol():
     assert sorted(lbx.SIGNATURES) == syms
 
 
+def test_library_gpu_code_is_built_for_gfx950():
+    """The offload bundle of liblbx.so holds gfx950 code objects only.  A
+    plain `-x c++` source on the hipcc line once made hipcc build the device
+    code for its default target, a library that loads and exports every
+    symbol but has no code for the MI355X."""
+    data = open(lbx.LIB_PATH, "rb").read()
+    targets = set(re.findall(rb"hipv4-amdgcn-amd-amdhsa--(gfx[0-9a-z]+)", data))
+    assert targets == {b"gfx950"}, targets
+
+
 def test_status_strings_are_the_reference_messages():
     L = lbx.lib()
     assert b"population n(x,y,z,vel) < 0" in L.lbx_status_string(lbx.ERR_NEG_POP)
