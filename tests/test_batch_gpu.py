@@ -75,11 +75,15 @@ def test_batch_matches_single_lattice_bitwise(prec):
     assert stats["lb_fluid_updates"] > 0 and stats["mp_fluid_updates"] > 0
 
 
-def test_batch_block128_no_graphs_and_mixed_rounds():
+@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT, lbx.PREC_FP32_STORE])
+def test_batch_block128_no_graphs_and_mixed_rounds(prec):
+    """128-node blocks, per-step launches, rounds of 37 LB / 23 MP steps: the
+    FP32 forms' l2err check steps (every 64th after the force switch) fall
+    inside rounds and on their last steps."""
     cases = [Case(n) for n in ["bcc12_ads", "random_3d", "cyl9_tau08"]]
     mp_steps = 100
-    ref, _ = run_batch(cases, lbx.PREC_FP64, mp_steps)
-    got, _ = run_batch(cases, lbx.PREC_FP64, mp_steps, block=128, graphs=False, lb_round=37, mp_round=23)
+    ref, _ = run_batch(cases, prec, mp_steps)
+    got, _ = run_batch(cases, prec, mp_steps, block=128, graphs=False, lb_round=37, mp_round=23)
     for c, r, g in zip(cases, ref, got):
         assert g["lb_steps"] == r["lb_steps"]
         assert np.array_equal(g["rho"], r["rho"]) and np.array_equal(g["j"], r["j"])
@@ -87,16 +91,17 @@ def test_batch_block128_no_graphs_and_mixed_rounds():
         assert np.all(np.abs(g["vacf"] - r["vacf"]) <= 1e-12 * scale)
 
 
-def test_batch_rounds_of_many_lengths_same_bits():
+@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT, lbx.PREC_FP32_STORE])
+def test_batch_rounds_of_many_lengths_same_bits(prec):
     """Graph-replayed rounds of 1, 2, ..., 90 steps (more distinct round
     lengths than the captured-graph cache keeps, so it starts over) give the
-    same bits as rounds of 64."""
+    same bits as rounds of 64, in every precision form."""
     cases = [Case(n) for n in ["bcc12_ads", "random_3d"]]
     mp_steps = 120
-    ref, _ = run_batch(cases, lbx.PREC_FP64, mp_steps)
+    ref, _ = run_batch(cases, prec, mp_steps)
     cap = sum(((int((c.nature == 0).sum()) + 255) // 256) * 256 for c in cases) + 4096
     box = sum(int(np.prod(c.dims)) for c in cases) + 4096
-    with lbx.Batch(cap, box, len(cases) + 1) as B:
+    with lbx.Batch(cap, box, len(cases) + 1, precision=prec) as B:
         ids = [B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, mp_steps,
                      max_lb_steps=20000) for c in cases]
         k = 0
