@@ -128,6 +128,15 @@ int lbx_lbin_char(const lbx_lbin* h, const char* tag, char* out, int64_t cap, in
 int lbx_geom_in_parse(const char* text, int64_t len, int32_t lx, int32_t ly, int32_t lz, int8_t* nature,
                       int64_t* nsolid);
 
+/* output/supercell.xsf of print_supercell_xsf (module_io.f90:86-160), byte
+ * for byte as the reference program writes it: nature and interfacial
+ * (lx,ly,lz, x fastest) to text in out (cap bytes, not NUL-terminated);
+ * *len receives the text's size, and cap = 0 (out may be NULL) only asks
+ * for it.  LBX_ERR_ARG when cap is too small or an axis has more than
+ * 950000 nodes (flang writes the coordinates from 950000 on in E form). */
+int lbx_supercell_xsf(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, const int8_t* interfacial,
+                      char* out, int64_t cap, int64_t* len);
+
 /* --------------------------------------------------------------- context */
 
 /* HBM bytes a context of this shape needs (admission control for batching

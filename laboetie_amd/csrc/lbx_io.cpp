@@ -14,8 +14,15 @@
 // geom.in (construct_custom, module_geometry.f90:380-426): one solid node
 // per line as 1-based `i j k`; everything else is fluid.  An index outside
 // 1..lx, 1..ly, 1..lz stops the reference; here it returns LBX_ERR_ARG.
+//
+// supercell.xsf (print_supercell_xsf, module_io.f90:86-160): the same bytes
+// as the reference program's file, so that the two can be diffed.  The
+// comment lines are part of that file; node lines are x slowest, z fastest,
+// in the list-directed form flang writes (a blank before each item,
+// integers at their width, integral REAL(4) values as `N.`).
 
 #include <cctype>
+#include <cstdio>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -212,6 +219,78 @@ int lbx_geom_in_parse(const char* text, int64_t len, int32_t lx, int32_t ly, int
         nature[r] = 1;
     }
     if (nsolid) *nsolid = marked;
+    return LBX_OK;
+}
+
+// The lines before the node list: each `(xA)` record is a blank and the
+// text; an empty record is an empty line.
+static const char* const kXsfHead1[] = {
+    "# this is the specification file of the supercell",
+    "# lines beginning with # are commented. There cannot be comment lines within the sections",
+    "# XSF format specifications can be found at http://www.xcrysden.org/doc/XSF.html",
+    "# I strongly recommends to read this documentation.",
+    nullptr,
+    "# for periodic structures one has to begin with word CRYSTAL",
+    "CRYSTAL",
+    nullptr,
+    "# Then one needs to specify the lattice vectors",
+    "# specification of PRIMVEC (in ANGSTROMS) like:",
+    "#         ax, ay, az    (first lattice vector)",
+    "#         bx, by, bz    (second lattice vector)",
+    "#         cx, cy, cz    (third lattice vector)",
+    "# pay attention to vectors as they are written in horizontal way which is quite unusual",
+    "# for now only orthorhombic structures allowed (a/=b/=c, all angles are 90 degrees)",
+    "PRIMVEC",
+};
+static const char* const kXsfHead2[] = {
+    nullptr,
+    "# Then one needs to specify the atoms belonging to the unit cell. ",
+    "# First number stands for number of atoms in the primitive cell (2 in this case).",
+    "# The second number is always 1 for PRIMCOORD coordinates.",
+    "# in angstroms and cartesian coordinates",
+    "PRIMCOORD",
+};
+
+int lbx_supercell_xsf(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, const int8_t* interfacial,
+                      char* out, int64_t cap, int64_t* len) {
+    // flang writes an integral REAL(4) of 9.5e5 or more in E form, which this
+    // writer does not reproduce: coordinates stay below 950000
+    constexpr int32_t kMaxAxis = 950000;
+    if (!nature || !interfacial || !len || lx < 1 || ly < 1 || lz < 1 || lx > kMaxAxis || ly > kMaxAxis ||
+        lz > kMaxAxis || cap < 0 || (cap > 0 && !out))
+        return LBX_ERR_ARG;
+    std::string s;
+    auto rec = [&](const char* t) {
+        if (t) s.append(" ").append(t);
+        s.push_back('\n');
+    };
+    for (const char* t : kXsfHead1) rec(t);
+    char buf[96];
+    const int32_t dims[3] = {lx, ly, lz};
+    for (int a = 0; a < 3; ++a) {  // (3(xF10.5)) of (lx,0,0), (0,ly,0), (0,0,lz)
+        for (int b = 0; b < 3; ++b) {
+            std::snprintf(buf, sizeof buf, " %10.5f", a == b ? (double)(float)dims[a] : 0.0);
+            s.append(buf);
+        }
+        s.push_back('\n');
+    }
+    for (const char* t : kXsfHead2) rec(t);
+    std::snprintf(buf, sizeof buf, " %lld 1\n", (long long)lx * ly * lz);
+    s.append(buf);
+    // VMD colours: solid 4, interfacial fluid 45, other fluid 1
+    for (int32_t i = 0; i < lx; ++i)
+        for (int32_t j = 0; j < ly; ++j)
+            for (int32_t k = 0; k < lz; ++k) {
+                const int64_t r = i + (int64_t)lx * (j + (int64_t)ly * k);
+                if (nature[r] != 0 && nature[r] != 1) return LBX_ERR_ARG;
+                const int colour = nature[r] == 1 ? 4 : (interfacial[r] ? 45 : 1);
+                std::snprintf(buf, sizeof buf, " %d %d. %d. %d.\n", colour, i, j, k);
+                s.append(buf);
+            }
+    *len = (int64_t)s.size();
+    if (cap == 0) return LBX_OK;
+    if (cap < (int64_t)s.size()) return LBX_ERR_ARG;
+    std::memcpy(out, s.data(), s.size());
     return LBX_OK;
 }
 

@@ -42,6 +42,7 @@ SIGNATURES = {
     "lbx_lbin_log": (ctypes.c_int, [_p, ctypes.c_char_p, _p, _p]),
     "lbx_lbin_char": (ctypes.c_int, [_p, ctypes.c_char_p, _p, _i64, _p]),
     "lbx_geom_in_parse": (ctypes.c_int, [ctypes.c_char_p, _i64, _i32, _i32, _i32, _p, _p]),
+    "lbx_supercell_xsf": (ctypes.c_int, [_i32, _i32, _i32, _p, _p, _p, _i64, _p]),
     "lbx_footprint_bytes": (_i64, [_i32] * 5),
     "lbx_create": (ctypes.c_int, [_i32] * 3 + [_p, _f64, _i32, _p]),
     "lbx_destroy": (ctypes.c_int, [_p]),
@@ -232,6 +233,18 @@ def geom_in_nature(text, lx, ly, lz):
     ns = _i64()
     _check(lib().lbx_geom_in_parse(raw, len(raw), lx, ly, lz, _ptr(nat), ctypes.byref(ns)), "geom_in_parse")
     return nat, ns.value
+
+
+def supercell_xsf(lx, ly, lz, nature, interfacial):
+    """The text of output/supercell.xsf (lbx_supercell_xsf)."""
+    nat = np.ascontiguousarray(nature, np.int8)
+    itf = np.ascontiguousarray(interfacial, np.int8)
+    n = _i64()
+    _check(lib().lbx_supercell_xsf(lx, ly, lz, _ptr(nat), _ptr(itf), None, 0, ctypes.byref(n)), "supercell_xsf")
+    buf = ctypes.create_string_buffer(n.value)
+    _check(lib().lbx_supercell_xsf(lx, ly, lz, _ptr(nat), _ptr(itf), buf, n.value, ctypes.byref(n)),
+           "supercell_xsf")
+    return buf.raw[:n.value].decode()
 
 
 def footprint_bytes(lx, ly, lz, precision=PREC_FP64, with_mp=1):

@@ -1,9 +1,17 @@
 """The library's C++ readers of lb.in and geom.in (lbx_lbin_*,
-lbx_geom_in_parse; host code, no device) against the Python reader
+lbx_geom_in_parse) and its supercell.xsf writer (lbx_supercell_xsf), host
+code with no device calls.  The readers against the Python reader
 (laboetie_amd/lbin.py) and the geometries the reference program built from
 the same files (the fixtures' nature arrays, dumped from its node%nature by
 oracle/ref_dump.f90): every tag of every fixture's lb.in, the reference
-grammar's corner cases, and every fixture's geom.in."""
+grammar's corner cases, and every fixture's geom.in.  The writer against
+the supercell.xsf the reference program writes for every fixture's
+geometry, byte for byte."""
+import os
+import re
+import subprocess
+import tempfile
+
 import numpy as np
 import pytest
 
@@ -112,3 +120,37 @@ def test_geom_in_edge_cases():
         assert e.value.status == lbx.ERR_ARG
     nat, ns = lbx.geom_in_nature("", 2, 2, 2)
     assert not nat.any() and ns == 0
+
+
+REF_BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "laboetie")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="reference program not built (needs the reference sources)")
+@pytest.mark.parametrize("name", FIXTURES)
+def test_supercell_xsf_same_bytes_as_the_reference(name):
+    """The reference run with no force and no moment propagation (it stops
+    at t = 4) writes output/supercell.xsf before anything else; the
+    library's writer gives the same bytes from the same nature and
+    interfacial arrays."""
+    g = load(name)
+    dims = [int(v) for v in g["dims"]]
+    lb_in = re.sub(r"(?m)^f_ext = .*$", "f_ext = 0.0 0.0 0.0", str(g["lb_in"]))
+    lb_in = re.sub(r"(?m)^maximum_moment_propagation_steps = .*$", "maximum_moment_propagation_steps = 0", lb_in)
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "lb.in"), "w").write(lb_in)
+        if str(g["geom_in"]):
+            open(os.path.join(d, "geom.in"), "w").write(str(g["geom_in"]))
+        p = subprocess.run([REF_BIN], cwd=d, env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True,
+                           text=True, timeout=300)
+        assert p.returncode == 0, p.stdout + p.stderr
+        ref = open(os.path.join(d, "output", "supercell.xsf")).read()
+    got = lbx.supercell_xsf(*dims, g["nature"], g["interfacial"])
+    assert got == ref
+
+
+def test_supercell_xsf_refusals():
+    nat = np.zeros(8, np.int8)
+    with pytest.raises(lbx.LbxError):
+        lbx.supercell_xsf(2, 2, 2, np.full(8, 2, np.int8), nat)   # undocumented nature
+    with pytest.raises(lbx.LbxError):
+        lbx.supercell_xsf(1, 1, 950001, np.zeros(950001, np.int8), np.zeros(950001, np.int8))  # coordinate 950000
