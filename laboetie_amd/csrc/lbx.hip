@@ -52,6 +52,7 @@ struct lbx_ctx {
     double tau = 1.0, A = 0, B = 0, C = 0;
     float A32 = 0, B32 = 0;      // FP32-shift collision: A + B = 1 exactly (fp32_relaxation)
     int32_t check_every = 1;     // stage-2 steps between l2err measurements (kCheckEvery in FP32 storage)
+    uint32_t* mp_tile_order = nullptr;  // MP: tiles in band-major order (NULL: packed order)
     int64_t nfi = 0;
     int64_t floor_window = 0;    // FP32-storage forms: l2err floor test (0: off)
     int device = 0;
@@ -160,7 +161,35 @@ void free_mp(lbx_ctx* c) {
     dfree(c->Pa[0]);
     dfree(c->Pa[1]);
     dfree(c->vtrace);
+    dfree(c->mp_tile_order);
     c->mp_ready = false;
+}
+
+#ifndef LBX_MP_BAND
+#define LBX_MP_BAND 32
+#endif
+// MP tile order: bands of kMpBandRows box rows, plane by plane within a band,
+// so that the z-neighbour rows a tile gathers were read by tiles that ran just
+// before it on the same XCD (0: the packed order).  Each tile still covers
+// the same nodes and writes its VACF partial by tile, so the bits do not
+// depend on the order.
+constexpr int32_t kMpBandRows = LBX_MP_BAND;
+
+int mp_tile_order(lbx_ctx* c) {
+    if (kMpBandRows <= 0 || c->ntiles < 2) return LBX_OK;
+    std::vector<std::pair<uint64_t, uint32_t>> key(c->ntiles);
+    const uint64_t lx = (uint64_t)c->lx, ly = (uint64_t)c->ly;
+    for (uint32_t t = 0; t < c->ntiles; ++t) {
+        const uint64_t r = c->nodes_h[std::min<uint64_t>((uint64_t)t * kBlock, c->Nf - 1)] & kIndexMask;
+        const uint64_t y = (r / lx) % ly, z = r / (lx * ly);
+        key[t] = {((y / kMpBandRows) * (uint64_t)c->lz + z) << 32 | t, t};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<uint32_t> order(c->ntiles);
+    for (uint32_t t = 0; t < c->ntiles; ++t) order[t] = key[t].second;
+    LBX_TRY(dalloc(&c->mp_tile_order, c->ntiles));
+    return hip_status(hipMemcpyAsync(c->mp_tile_order, order.data(), c->ntiles * sizeof(uint32_t),
+                                     hipMemcpyHostToDevice, c->stream));
 }
 
 void free_all(lbx_ctx* c) {
@@ -417,6 +446,7 @@ void launch_mp_step(lbx_ctx* c, int now, int64_t it, int64_t k, int32_t stop_on_
     a.Np = c->Np;
     a.any_ads = c->any_ads;
     a.tp = c->tp;
+    a.tile_order = c->mp_tile_order;
     with_tracers(c->ntr, [&](auto TT) {
         constexpr int T = TT;
         auto kern = c->wide ? k_mp_step<true, T, W> : k_mp_step<false, T, W>;
@@ -803,6 +833,7 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, int32_t ntr, co
         if (any_ads) LBX_TRY(hipMemsetAsync(c->Pa[b], 0, np3 * sizeof(double), c->stream));
     }
     LBX_TRY(reset_ctrl(c));
+    LBX_TRY(mp_tile_order(c));
     c->ntr = ntr;
     c->tp = tp;
     c->any_ads = any_ads;

@@ -770,6 +770,7 @@ struct MpStepArgs {
     uint64_t Np;
     int32_t any_ads;                 // some tracer of the batch adsorbs (Pan, Pax allocated)
     TracerParams tp;
+    const uint32_t* tile_order = nullptr;  // tile of each logical tile (NULL: the same)
 };
 
 // One moment-propagation step of T tracers, module_moment_propagation.f90:207-253:
@@ -883,7 +884,8 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
 
 template <bool Wide, int T, typename W>
 __global__ __launch_bounds__(kBlock, LBX_MP_WAVES) void k_mp_step(MpStepArgs<W> a) {
-    mp_step_tile<kBlock, Wide, T, W>(a, xcd_tile(), blockIdx.x == 0, gridDim.x);
+    mp_step_tile<kBlock, Wide, T, W>(a, a.tile_order ? a.tile_order[xcd_tile()] : xcd_tile(), blockIdx.x == 0,
+                                     gridDim.x);
 }
 
 // Plain streaming copy (16 B per lane per iteration, grid-stride): the
