@@ -175,21 +175,21 @@ void free_mp(lbx_ctx* c) {
 // depend on the order.
 constexpr int32_t kMpBandRows = LBX_MP_BAND;
 
-int mp_tile_order(lbx_ctx* c) {
-    if (kMpBandRows <= 0 || c->ntiles < 2) return LBX_OK;
+int band_tile_order(lbx_ctx* c, int32_t band_rows, uint32_t** out) {
+    if (band_rows <= 0 || c->ntiles < 2 || *out) return LBX_OK;
     std::vector<std::pair<uint64_t, uint32_t>> key(c->ntiles);
     const uint64_t lx = (uint64_t)c->lx, ly = (uint64_t)c->ly;
     for (uint32_t t = 0; t < c->ntiles; ++t) {
         const uint64_t r = c->nodes_h[std::min<uint64_t>((uint64_t)t * kBlock, c->Nf - 1)] & kIndexMask;
         const uint64_t y = (r / lx) % ly, z = r / (lx * ly);
-        key[t] = {((y / kMpBandRows) * (uint64_t)c->lz + z) << 32 | t, t};
+        key[t] = {((y / (uint64_t)band_rows) * (uint64_t)c->lz + z) << 32 | t, t};
     }
     std::sort(key.begin(), key.end());
     std::vector<uint32_t> order(c->ntiles);
     for (uint32_t t = 0; t < c->ntiles; ++t) order[t] = key[t].second;
-    LBX_TRY(dalloc(&c->mp_tile_order, c->ntiles));
-    return hip_status(hipMemcpyAsync(c->mp_tile_order, order.data(), c->ntiles * sizeof(uint32_t),
-                                     hipMemcpyHostToDevice, c->stream));
+    LBX_TRY(dalloc(out, c->ntiles));
+    LBX_TRY(hipMemcpyAsync(*out, order.data(), c->ntiles * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    return hip_status(hipStreamSynchronize(c->stream));  // order is a local
 }
 
 void free_all(lbx_ctx* c) {
@@ -833,7 +833,7 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, int32_t ntr, co
         if (any_ads) LBX_TRY(hipMemsetAsync(c->Pa[b], 0, np3 * sizeof(double), c->stream));
     }
     LBX_TRY(reset_ctrl(c));
-    LBX_TRY(mp_tile_order(c));
+    LBX_TRY(band_tile_order(c, kMpBandRows, &c->mp_tile_order));
     c->ntr = ntr;
     c->tp = tp;
     c->any_ads = any_ads;
