@@ -272,6 +272,14 @@ int lbx_kernel_time(lbx_ctx* ctx, int32_t which, double* total_ms, int64_t* laun
  * the reference. */
 int lbx_set_floor_window(lbx_ctx* ctx, int64_t steps);
 
+/* FP32-storage forms: the steps between the l2err measurements of stage 2
+ * of lbx_equilibrate (a power of two up to 4096; default 64).  l2err is then
+ * max|j(t) - j(t-steps)| / steps at the steps switch + m*steps, and NaN in
+ * the trace at the others; 1 measures every step as the FP64 form does.
+ * FP64 contexts accept only 1 (LBX_ERR_ARG otherwise).  No counterpart in
+ * the reference. */
+int lbx_set_check_interval(lbx_ctx* ctx, int32_t steps);
+
 /* Element addressing of the step kernels.  The population and moment arrays
  * are addressed through 32-bit buffer offsets while 19*Np*8 < 4 GiB (Np: the
  * fluid-node count rounded up to 64), and through 64-bit pointers above that;
@@ -347,6 +355,10 @@ int lbxb_result(lbxb* b, int32_t id, double deff[3], double* vacf, int64_t vacf_
 
 /* Block size (128 or 256 nodes) of the LB and MP launches. */
 int lbxb_set_block(lbxb* b, int32_t lb_block, int32_t mp_block);
+
+/* lbx_set_check_interval for every instance of a batch (set before the
+ * instances run; default 64 in the FP32-storage forms). */
+int lbxb_set_check_interval(lbxb* b, int32_t steps);
 /* How a round's steps are launched: one launch per step on the stream, the
  * same launches replayed from a captured HIP graph (default), or one
  * persistent cooperative launch per phase that loops over the steps with a

@@ -1060,6 +1060,13 @@ int lbx_set_floor_window(lbx_ctx* c, int64_t steps) {
     return LBX_OK;
 }
 
+int lbx_set_check_interval(lbx_ctx* c, int32_t steps) {
+    if (!c || steps < 1 || steps > 4096 || (steps & (steps - 1)) != 0) return LBX_ERR_ARG;
+    if (steps != 1 && !f32_storage(c->prec)) return LBX_ERR_ARG;
+    c->check_every = steps;
+    return LBX_OK;
+}
+
 int lbx_set_addressing(lbx_ctx* c, int32_t wide) {
     if (!c || wide < 0 || wide > 1) return LBX_ERR_ARG;
     c->wide = c->wide_auto || wide == 1;

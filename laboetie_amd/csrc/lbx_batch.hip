@@ -375,6 +375,7 @@ struct lbxb {
     bool graphs = true;
     bool persistent = false;  // a round's launches of each phase as one cooperative launch
     int32_t num_cu = 0;
+    int32_t check_every = kCheckEvery;  // FP32 storage: stage-2 steps between l2err measurements
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     // device arrays
@@ -1047,7 +1048,7 @@ int lbxb_round(lbxb* b, int32_t lb_steps, int32_t mp_steps, int32_t* active) {
             in.check_after = h.check_after;
             // l2err of every step in stage 1; in stage 2 of the FP32 storage
             // forms every kCheckEvery steps from the force switch
-            in.check_every = (h.stage == 2 && shift(b)) ? kCheckEvery : 1;
+            in.check_every = (h.stage == 2 && shift(b)) ? b->check_every : 1;
             in.check_base = h.stage == 2 ? h.switch_step : 0;
             in.check_inv = 1.0 / in.check_every;
             for (int a = 0; a < 3; ++a) {
@@ -1190,6 +1191,13 @@ int lbxb_set_block(lbxb* b, int32_t lb_block, int32_t mp_block) {
     if (!b || (lb_block != 128 && lb_block != 256) || (mp_block != 128 && mp_block != 256)) return LBX_ERR_ARG;
     b->lb_block = lb_block;
     b->mp_block = mp_block;
+    return LBX_OK;
+}
+
+int lbxb_set_check_interval(lbxb* b, int32_t steps) {
+    if (!b || steps < 1 || steps > 4096 || (steps & (steps - 1)) != 0) return LBX_ERR_ARG;
+    if (steps != 1 && !shift(b)) return LBX_ERR_ARG;
+    b->check_every = steps;
     return LBX_OK;
 }
 

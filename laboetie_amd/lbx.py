@@ -65,6 +65,7 @@ SIGNATURES = {
     "lbx_timing_enable": (ctypes.c_int, [_p, _i32]),
     "lbx_kernel_time": (ctypes.c_int, [_p, _i32, _p, _p]),
     "lbx_set_floor_window": (ctypes.c_int, [_p, _i64]),
+    "lbx_set_check_interval": (ctypes.c_int, [_p, _i32]),
     "lbx_set_addressing": (ctypes.c_int, [_p, _i32]),
     "lbx_addressing": (ctypes.c_int, [_p]),
     "lbx_fluid_nodes": (_i64, [_p]),
@@ -78,6 +79,7 @@ SIGNATURES = {
     "lbxb_info": (ctypes.c_int, [_p, _i32, _p, _p, _p, _p, _p]),
     "lbxb_result": (ctypes.c_int, [_p, _i32, _p, _p, _i64, _p, _p, _p, _p]),
     "lbxb_set_block": (ctypes.c_int, [_p, _i32, _i32]),
+    "lbxb_set_check_interval": (ctypes.c_int, [_p, _i32]),
     "lbxb_autotune": (ctypes.c_int, [_p, _i32, _p, _p, _p]),
     "lbxb_set_launch_mode": (ctypes.c_int, [_p, _i32]),
     "lbxb_set_addressing": (ctypes.c_int, [_p, _i32]),
@@ -318,6 +320,11 @@ class Lattice:
         `steps` steps (0: off; default 1000 for FP32-shift contexts)."""
         _check(lib().lbx_set_floor_window(self._h, int(steps)), "set_floor_window")
 
+    def set_check_interval(self, steps):
+        """FP32-storage forms: steps between stage-2 l2err measurements (a
+        power of two; default 64; 1 measures every step)."""
+        _check(lib().lbx_set_check_interval(self._h, int(steps)), "set_check_interval")
+
     def set_addressing(self, wide):
         """1: the 64-bit addressing form of the step kernels on any size (the
         library selects it by itself above 4 GiB of populations); 0: the
@@ -527,6 +534,10 @@ class Batch:
 
     def set_block(self, lb_block, mp_block):
         _check(lib().lbxb_set_block(self._h, int(lb_block), int(mp_block)), "lbxb_set_block")
+
+    def set_check_interval(self, steps):
+        """lbxb_set_check_interval: the FP32-storage forms' stage-2 l2err interval."""
+        _check(lib().lbxb_set_check_interval(self._h, int(steps)), "lbxb_set_check_interval")
 
     def autotune(self, launches=10):
         lb, mp = _i32(), _i32()

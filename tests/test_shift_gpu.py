@@ -115,6 +115,49 @@ def test_fp32_forms_floor_and_periodic_box_match_restatement(case, prec):
         assert np.array_equal(n, ref["n"]) and np.array_equal(rho, ref["rho"]) and np.array_equal(j, ref["j"])
 
 
+@pytest.mark.parametrize("prec", [SHIFT, STORE])
+@pytest.mark.parametrize("every", [1, 16])
+def test_check_interval_matches_restatement(every, prec):
+    """lbx_set_check_interval(1): l2err of every step, as the FP64 form;
+    16: every 16th step.  Bit-identical to the restatement with the same
+    interval (trace, exit, fields), single lattice and batch."""
+    c = Case("cyl9_tau08")
+    say("check interval", every, prec)
+    ref = lbref.equilibrate_shift(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, max_steps=1400,
+                                  arith=ARITH[prec], check_every=every)
+    tr = ref["l2err"]
+    assert np.isnan(tr).any() == (every > 1)
+    with lbx.Lattice(c.dims, c.nature, c.tau, precision=prec) as L:
+        L.set_check_interval(every)
+        L.init_rest(c.rho0)
+        r = L.equilibrate(c.fext, c.target, max_steps=1400)
+        assert (r["steps"], r["exit_reason"]) == (ref["steps"], ref["exit_reason"])
+        assert np.array_equal(r["l2err"], tr, equal_nan=True)
+        n, rho, j = L.get_state()
+        assert np.array_equal(n, ref["n"]) and np.array_equal(rho, ref["rho"]) and np.array_equal(j, ref["j"])
+    nf = int((c.nature == 0).sum())
+    with lbx.Batch((nf + 255) // 256 * 256 + 4096, int(np.prod(c.dims)) + 4096, 2, precision=prec) as B:
+        B.set_check_interval(every)
+        iid = B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, 10, max_lb_steps=1400)
+        while B.round(37, 37) > 0:
+            pass
+        g = B.result(iid, vacf_rows=11, flow=True)
+        assert B.info(iid)["lb_steps"] == ref["steps"]
+        assert np.array_equal(g["rho"], ref["rho"]) and np.array_equal(g["j"], ref["j"])
+
+
+def test_check_interval_refusals():
+    c = Case("bcc12_ads")
+    with lbx.Lattice(c.dims, c.nature, c.tau) as L:  # FP64: per-step l2err only
+        L.set_check_interval(1)
+        with pytest.raises(lbx.LbxError):
+            L.set_check_interval(64)
+    with lbx.Lattice(c.dims, c.nature, c.tau, precision=SHIFT) as L:
+        for bad in (0, 3, 8192):
+            with pytest.raises(lbx.LbxError):
+                L.set_check_interval(bad)
+
+
 def test_shift_batched_tracers_and_floor_window():
     c = Case("bcc12_ads")
     ka = [c.ka, 0.0]
