@@ -61,6 +61,41 @@ def test_dropin_matches_reference_program(name, mp_steps):
         assert np.all(np.abs(vr[:, 1:] - vg[:, 1:]) <= 1e-12 * np.abs(vr[0, 1:]).max())
 
 
+BCC32_LB_IN = """lx = 32
+ly = 32
+lz = 32
+geometryLabel = 3
+relaxation_time = 1.0
+initialSolventDensity = 1.0
+f_ext = 0.0 0.0 1e-6
+target_error = 1e-10
+maximum_moment_propagation_steps = 200
+tracer_Db = 0.01
+tracer_ka = 0.027182818284590453
+tracer_kd = 0.01
+"""
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_BIN) and os.path.exists(GPU_BIN)),
+                    reason="drop-in or reference binary not built (needs the reference sources at build time)")
+def test_dropin_matches_reference_program_bcc32():
+    """The bench's geometry and forcing at 32^3 (BCC spheres, F = 1e-6
+    along z, an adsorbing tracer at dF = -1): 43 tiles of fluid nodes
+    instead of the fixtures' few, same outputs as the reference program."""
+    print("drop-in case bcc32", flush=True)
+    with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg:
+        rc_r, out_r = run_case(REF_BIN, BCC32_LB_IN, "", dr)
+        rc_g, out_g = run_case(GPU_BIN, BCC32_LB_IN, "", dg)
+        assert rc_r == 0, out_r
+        assert rc_g == 0, out_g
+        for f in TEXT_OUTPUTS:
+            assert open(os.path.join(dr, "output", f)).read() == open(os.path.join(dg, "output", f)).read(), f
+        vr = np.loadtxt(os.path.join(dr, "output", "vacf.dat"), comments="#")
+        vg = np.loadtxt(os.path.join(dg, "output", "vacf.dat"), comments="#")
+        assert vr.shape == vg.shape == (201, 4)
+        assert np.all(np.abs(vr[:, 1:] - vg[:, 1:]) <= 1e-12 * np.abs(vr[0, 1:]).max())
+
+
 @pytest.mark.skipif(not (os.path.exists(REF_BIN) and os.path.exists(GPU_BIN)),
                     reason="drop-in or reference binary not built (needs the reference sources at build time)")
 @pytest.mark.parametrize("key", ["fp32_shift", "fp32_store"])
