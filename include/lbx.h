@@ -13,7 +13,7 @@
  *      population     n[node + N*l], l = 0..18         n(lx,ly,lz,19)
  *      moment field   P[a + 3*node], a = 0..2          P(3,lx,ly,lz)
  *    l = 0..18 stands for the reference's velocity index 1..19
- *    (module_lbmodel.f90:217-237).
+ *    (module_lbmodel.f90:66-86).
  *  - nature: int8, 0 fluid, 1 solid (module_system.f90:32).
  *  - Every function returns an int status (LBX_OK = 0); the reference's fatal
  *    stops become status codes the caller maps back to the same messages.
@@ -64,12 +64,12 @@ const char* lbx_status_string(int status);
 /* Library version, e.g. 100 for 1.0.0. */
 int lbx_version(void);
 
-/* D3Q19 tables exactly as module_lbmodel.f90:217-237,273-313 builds them:
+/* D3Q19 tables exactly as module_lbmodel.f90:66-86,122-162 builds them:
  * c[3*l+a], a0 = w, a1 = w/cs^2, a2 = w/(2 cs^4), inv (0-based). */
 int lbx_d3q19_tables(int32_t c[57], double a0[19], double a1[19], double a2[19], int32_t inv[19]);
 
 /* ------------------------------------------------------------- geometry
- * Host-side builders of node%nature, replacing supercell_definition.f90:107-169
+ * Host-side builders of node%nature, replacing supercell_definition.f90:48-90
  * for geometryLabel -1 (no solid), 0 (custom list, module_geometry.f90:380-426:
  * solid_ijk holds npts 1-based (i,j,k) triples), 1 (slit, :158-166),
  * 2 (cylinder along z, :253-277), 3 (BCC of touching spheres, :206-245).
@@ -78,7 +78,7 @@ int lbx_d3q19_tables(int32_t c[57], double a0[19], double a1[19], double a2[19],
 int lbx_build_geometry(int32_t label, int32_t lx, int32_t ly, int32_t lz,
                        const int32_t* solid_ijk, int64_t npts, int8_t* nature);
 
-/* detectInterfacialNodes, supercell_definition.f90:194-226. */
+/* detectInterfacialNodes, supercell_definition.f90:115-147. */
 int lbx_detect_interfacial(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, int8_t* interfacial);
 
 /* Seeded synthetic media (no counterpart in the reference, which reads such

@@ -69,6 +69,7 @@ struct lbx_ctx {
     double* part = nullptr;      // per-block VACF partials, two buffers of 3*kMaxTracers per tile
     unsigned long long* slots = nullptr;  // [3][kSlots] l2err slots
     Ctrl* ctrl = nullptr;
+    Ctrl ctrl_h{};               // host copy that reset_ctrl uploads from (read by the stream's next copy)
     double* l2ring = nullptr;
     int pre = 0;                 // buffer holding the post-streaming state
     std::vector<uint32_t> nodes_h;
@@ -220,9 +221,12 @@ void relaxation_coeffs(double tau, double* A, double* B, double* C) {
 }
 
 // check_after: the step after which an LB stage's stop tests apply; the
-// floor-test record starts there (no minimum yet)
+// floor-test record starts there (no minimum yet).  The upload reads the
+// context's ctrl_h, which stays untouched until the stream has synchronised
+// (every caller synchronises before it resets the control block again).
 int reset_ctrl(lbx_ctx* c, int64_t check_after = 2) {
-    Ctrl h{};
+    Ctrl& h = c->ctrl_h;
+    h = Ctrl{};
     h.negpop = -1;
     for (int k = 0; k < 2; ++k) {
         h.l2min[k] = HUGE_VAL;

@@ -402,6 +402,7 @@ struct lbxb {
     double* vtrace = nullptr;
     // host state
     std::vector<BInst> tab;
+    std::vector<Ctrl> ctrl_h;  // per instance slot: what reset_inst_ctrl uploads (kept until the stream syncs)
     std::vector<HostInst> hi;
     std::vector<uint32_t> tile_inst_h;
     RangeAlloc node_alloc, map_alloc;
@@ -460,8 +461,12 @@ void free_all(lbxb* b) {
 // their moment propagation keeps its probabilities in FP32
 bool shift(const lbxb* b) { return b->prec == LBX_PREC_FP32_SHIFT || b->prec == LBX_PREC_FP32_STORE; }
 
+// The upload reads the batch's ctrl_h[id], which no other call changes before
+// the stream has synchronised (an instance is reset at most once between two
+// synchronisations).
 int reset_inst_ctrl(lbxb* b, uint32_t id, int64_t check_after) {
-    Ctrl h{};
+    Ctrl& h = b->ctrl_h[id];
+    h = Ctrl{};
     h.negpop = -1;
     for (int k = 0; k < 2; ++k) {
         h.l2min[k] = HUGE_VAL;
@@ -831,6 +836,7 @@ int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
         st = LBX_ERR_DEVICE;
     if (!st) {
         b->tab.assign(max_instances, BInst{});
+        b->ctrl_h.assign(max_instances, Ctrl{});
         b->hi.assign(max_instances, HostInst{});
         b->tile_inst_h.assign(b->ntiles_cap, kNoInst);
         b->node_alloc.init(b->ntiles_cap);

@@ -104,11 +104,11 @@ extern "C" int lbx_build_geometry(int32_t label, int32_t lx, int32_t ly, int32_t
         break;
     }
     default:
-        return LBX_ERR_ARG;  // supercell_definition.f90:156-157
+        return LBX_ERR_ARG;  // supercell_definition.f90:77-78
     }
     for (int64_t p = 0; p < N; ++p)
         if (nature[p] == 0) return LBX_OK;
-    return LBX_ERR_ARG;  // "All nodes are solid", supercell_definition.f90:167-169
+    return LBX_ERR_ARG;  // "All nodes are solid", supercell_definition.f90:88-90
 }
 
 extern "C" int lbx_detect_interfacial(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature,

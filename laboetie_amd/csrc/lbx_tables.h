@@ -1,8 +1,8 @@
 // D3Q19 tables shared by the host and device code of liblbx.
 //
 // Velocity order and weights as module_lbmodel.f90 builds them
-// (init_velocities :217-237, init_weight_factors :273-303,
-// determine_velocity_inverse :307-313); index l = 0..18 here stands for the
+// (init_velocities :66-86, init_weight_factors :122-152,
+// determine_velocity_inverse :156-162); index l = 0..18 here stands for the
 // reference's l = 1..19.  The weight expressions are the reference's, folded
 // at compile time in IEEE double exactly as flang folds its parameters.
 #pragma once
@@ -21,7 +21,7 @@ constexpr int kCY[kQ] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 
 constexpr int kCZ[kQ] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
 constexpr int kInv[kQ] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
 
-constexpr double kCsq = 1.0 / 3.0;   // module_lbmodel.f90:278
+constexpr double kCsq = 1.0 / 3.0;   // module_lbmodel.f90:67,127
 constexpr double kW0 = 1.0 / 3.0, kW1 = 1.0 / 18.0, kW2 = 1.0 / 36.0;
 
 constexpr LBX_HD int wclass(int l) { return l == 0 ? 0 : (l <= 6 ? 1 : 2); }

@@ -9,8 +9,8 @@
  *
  * Parity status per function (what pins it):
  *   lbref_tables           reference dump of lbm%vel (ref_dump tables mode)
- *   lbref_norm2            exhaustive comparison with flang NORM2 on the
- *                          geometry inputs (tests/test_oracle_pins.py)
+ *   lbref_norm2            bit-for-bit comparison with the compiler's NORM2 on
+ *                          grids of geometry inputs (tests/test_oracle_pins.py)
  *   lbref_geometry         reference node%nature dumps (labels 0-3)
  *   lbref_interfacial      reference node%isInterfacial dumps
  *   lbref_collide          reference collide() on random inputs
@@ -29,7 +29,7 @@
 #include <string.h>
 #include <float.h>
 
-/* D3Q19 velocity set, module_lbmodel.f90:217-237 (l = 1..19 -> 0..18) */
+/* D3Q19 velocity set, module_lbmodel.f90:66-86 (l = 1..19 -> 0..18) */
 static const int32_t CV[19][3] = {
     {0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},  {0, 0, -1},
     {1, 1, 0},  {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1}, {-1, 0, 1}, {1, 0, -1},
@@ -39,8 +39,8 @@ static double A0[19], A1[19], A2[19];
 static int32_t INV[19];
 static int tables_ready = 0;
 
-/* module_lbmodel.f90:273-303 (weights; the parameters are folded in double)
- * and :307-313 (inverse velocity). */
+/* module_lbmodel.f90:122-152 (weights; the parameters are folded in double)
+ * and :156-162 (inverse velocity). */
 static void init_tables(void) {
     if (tables_ready) return;
     const double csq = 1.0 / 3.0;
@@ -80,7 +80,7 @@ static inline int32_t pbc(int32_t i, int32_t n) {
 /* The Fortran NORM2 intrinsic as the flang runtime evaluates it: a scaled sum
  * of squares, max*sqrt(1 + sum((x/max)^2)) with the running rescale when a
  * larger element arrives.  Reproduced from black-box comparison with flang's
- * NORM2 (identical on 1.08e9 geometry inputs); the plain sqrt(sum x^2)
+ * NORM2 (identical on the grids of tests/test_oracle_pins.py); the plain sqrt(sum x^2)
  * differs in the last bit often enough to flip BCC solid/fluid decisions. */
 double lbref_norm2(const double* v, int n) {
     double mx = 0.0, s = 0.0;
@@ -102,7 +102,7 @@ double lbref_norm2(const double* v, int n) {
     return mx * sqrt(1.0 + s);
 }
 
-/* supercell_definition.f90:107-169 with the builders of module_geometry.f90:
+/* supercell_definition.f90:48-90 with the builders of module_geometry.f90:
  * label -1 all fluid; 0 construct_custom (:380-426, solid_ijk 1-based triples);
  * 1 construct_slit (:158-166); 2 construct_cylinder (:253-277);
  * 3 construct_cc (:206-245). Returns LBREF_BAD_ARG on the reference's stops. */
@@ -169,11 +169,11 @@ int lbref_geometry(int32_t label, int32_t lx, int32_t ly, int32_t lz,
     }
     int64_t nf = 0;
     for (int64_t p = 0; p < N; ++p) nf += nature[p] == 0;
-    if (nf == 0) return LBREF_BAD_ARG; /* supercell_definition.f90:167-169 */
+    if (nf == 0) return LBREF_BAD_ARG; /* supercell_definition.f90:88-90 */
     return LBREF_OK;
 }
 
-/* detectInterfacialNodes, supercell_definition.f90:194-226: a node (fluid or
+/* detectInterfacialNodes, supercell_definition.f90:115-147: a node (fluid or
  * solid) is interfacial when any of its 18 D3Q19 neighbours differs in nature. */
 void lbref_interfacial(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, int8_t* interfacial) {
     for (int32_t k = 0; k < lz; ++k)
@@ -536,4 +536,10 @@ void lbref_deff(const double* vacf, int64_t nsteps, double deff[3]) {
         }
         deff[a] = s + c;
     }
+}
+
+/* lbref_norm2 over count vectors of n components (v[n*k + a]), for the
+ * comparison with the compiler's NORM2 in the tests. */
+void lbref_norm2_rows(const double* v, int n, int64_t count, double* out) {
+    for (int64_t k = 0; k < count; ++k) out[k] = lbref_norm2(v + (int64_t)n * k, n);
 }

@@ -37,6 +37,7 @@ extern "C" {
 void lbref_tables(int32_t c[19 * 3], double a0[19], double a1[19], double a2[19], int32_t inv[19]);
 
 double lbref_norm2(const double* v, int n);
+void lbref_norm2_rows(const double* v, int n, int64_t count, double* out);
 
 int lbref_geometry(int32_t label, int32_t lx, int32_t ly, int32_t lz,
                    const int32_t* solid_ijk, int64_t npts, int8_t* nature);

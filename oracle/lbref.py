@@ -31,6 +31,7 @@ def lib():
         L.lbref_tables.argtypes = [_p] * 5
         L.lbref_norm2.argtypes = [_p, ctypes.c_int]
         L.lbref_norm2.restype = _f64
+        L.lbref_norm2_rows.argtypes = [_p, ctypes.c_int, _i64, _p]
         L.lbref_geometry.argtypes = [_i32] * 4 + [_p, _i64, _p]
         L.lbref_interfacial.argtypes = [_i32] * 3 + [_p, _p]
         L.lbref_collide.argtypes = [_i32] * 3 + [_p, _f64] + [_p] * 8

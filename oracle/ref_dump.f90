@@ -17,7 +17,7 @@
 !   geom.bin      int32 lx,ly,lz; int8 nature(lx,ly,lz); int8 interfacial(lx,ly,lz)
 !   eq_state.bin  int32 lx,ly,lz; f64 n(lx,ly,lz,19); f64 rho(lx,ly,lz);
 !                 f64 jx, jy, jz (lx,ly,lz)           (equilibration.f90:551-554)
-!   tracer_n.bin  f64 n(19,lx,ly,lz)                   (drop_tracers.f90:453-473)
+!   tracer_n.bin  f64 n(19,lx,ly,lz)                   (drop_tracers.f90:85-105)
 !   collide_in.bin / collide_out.bin
 !                 f64 n(lx,ly,lz,19) [in: then density, jx, jy, jz, fx, fy, fz]
 !   tables.bin    int32 coo(3,19); int32 inv(19); f64 a0(19), a1(19), a2(19)

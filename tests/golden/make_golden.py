@@ -161,7 +161,7 @@ def make_geometries():
             open(os.path.join(d, "lb.in"), "w").write(lb_in(dict(lx=lx, ly=ly, lz=lz, geometryLabel=label)))
             rc, o = run(d, [os.path.join(REF, "ref_dump"), "geom"])
             if rc != 0:
-                # supercell_definition.f90:167-169 stops on an all-solid box
+                # supercell_definition.f90:88-90 stops on an all-solid box
                 assert "All nodes are solid" in o, o
                 out[f"{kind}_{lx}_{ly}_{lz}"] = np.zeros((0,), np.int8)
                 continue
