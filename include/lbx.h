@@ -40,6 +40,7 @@ extern "C" {
 #define LBX_ERR_ARG       5  /* invalid argument (also the reference's input stops) */
 #define LBX_ERR_DEVICE    6  /* HIP runtime error or no device */
 #define LBX_ERR_STATE     7  /* call out of order (e.g. mp step before mp init) */
+#define LBX_ERR_SIZE      8  /* input larger than a fixed-size staging area (lbx_collide_noalloc) */
 
 /* precision forms */
 #define LBX_PREC_FP64        0  /* populations and arithmetic in FP64 (parity form) */
@@ -252,6 +253,20 @@ int lbx_mp_free(lbx_ctx* ctx);
 int lbx_collide(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double* n,
                 const double* density, const double* jx, const double* jy, const double* jz,
                 const double* f_ext_x, const double* f_ext_y, const double* f_ext_z);
+
+/* lbx_collide without any device allocation: no counterpart in the reference.
+ * It is for callers that cannot allocate device memory, such as a loop that
+ * forwards the reference's per-step collide() call and must not hipMalloc /
+ * hipFree five fields every step.  Same arguments, same results bit for bit
+ * as lbx_collide.  The fields are staged in a static device array of 64 MiB
+ * that each GPU the library runs on holds until the process ends; lattices of
+ * more than LBX_NOALLOC_MAX_NODES nodes (lx*ly*lz) return LBX_ERR_SIZE.  The
+ * call runs on the null stream and returns when n holds the result.  All
+ * calls on one GPU share its array: calls that overlap on one GPU race. */
+#define LBX_NOALLOC_MAX_NODES 320000
+int lbx_collide_noalloc(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double* n,
+                        const double* density, const double* jx, const double* jy, const double* jz,
+                        const double* f_ext_x, const double* f_ext_y, const double* f_ext_z);
 
 /* ------------------------------------------------------------- timing
  * Average device duration (ms) of the launches of one kernel family since

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -477,6 +478,7 @@ const char* lbx_status_string(int status) {
     case LBX_ERR_ARG: return "invalid argument";
     case LBX_ERR_DEVICE: return "HIP runtime error";
     case LBX_ERR_STATE: return "call out of order";
+    case LBX_ERR_SIZE: return "lattice larger than the no-allocation staging area (LBX_NOALLOC_MAX_NODES nodes)";
     default: return "unknown status";
     }
 }
@@ -1040,6 +1042,98 @@ int lbx_collide(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double
     dfree(dF);
     dfree(dnat);
     return st;
+}
+
+}  // extern "C"
+
+// ------------------------------------------- collide() without allocation
+// lbx_collide_noalloc stages its fields in one file-scope device array of
+// kNoAllocBytes instead of allocating them: each GPU the library runs on holds
+// its own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in, never the device memory, so the array is reached only
+// through the per-device address that hipGetSymbolAddress returns on the
+// first call on each device (with that device current).  Every call on a GPU
+// shares that GPU's array: calls that overlap on one GPU race.
+namespace {
+
+constexpr uint64_t kNoAllocBytes = 64ull << 20;
+static_assert((uint64_t)LBX_NOALLOC_MAX_NODES * (kQ * 8 + 8 + 3 * 8 + 3 * 8 + 1) + 5 * 256 <= kNoAllocBytes,
+              "LBX_NOALLOC_MAX_NODES must fit the staging array");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device
+int scratch_address(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return LBX_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return LBX_ERR_DEVICE;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return LBX_OK;
+}
+
+uint64_t up256(uint64_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+int lbx_collide_noalloc(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double tau, double* n,
+                        const double* density, const double* jx, const double* jy, const double* jz,
+                        const double* f_ext_x, const double* f_ext_y, const double* f_ext_z) {
+    if (lx <= 0 || ly <= 0 || lz <= 0 || !nature || !n || !density || !jx || !jy || !jz || !f_ext_x ||
+        !f_ext_y || !f_ext_z)
+        return LBX_ERR_ARG;
+    if (!(tau >= 0.5)) return LBX_ERR_TAU;
+    const uint64_t N = (uint64_t)lx * ly * lz;
+    if (N > LBX_NOALLOC_MAX_NODES) return LBX_ERR_SIZE;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LBX_ERR_DEVICE;
+    void* base = nullptr;
+    LBX_TRY(scratch_address(&base));
+    // the fields of collide() side by side in the array, each 256-byte aligned
+    const size_t b = N * sizeof(double);
+    char* p = (char*)base;
+    double* dn = (double*)p;
+    p += up256(kQ * b);
+    double* dr = (double*)p;
+    p += up256(b);
+    double* dj = (double*)p;
+    p += up256(3 * b);
+    double* dF = (double*)p;
+    p += up256(3 * b);
+    int8_t* dnat = (int8_t*)p;
+    // every byte the kernel reads is written by these copies first, so
+    // nothing a previous call left in the array is seen
+    hipStream_t s = nullptr;  // the null stream: no stream object to create
+    LBX_TRY(hipMemcpyAsync(dn, n, kQ * b, hipMemcpyHostToDevice, s));
+    LBX_TRY(hipMemcpyAsync(dr, density, b, hipMemcpyHostToDevice, s));
+    LBX_TRY(hipMemcpyAsync(dj, jx, b, hipMemcpyHostToDevice, s));
+    LBX_TRY(hipMemcpyAsync(dj + N, jy, b, hipMemcpyHostToDevice, s));
+    LBX_TRY(hipMemcpyAsync(dj + 2 * N, jz, b, hipMemcpyHostToDevice, s));
+    LBX_TRY(hipMemcpyAsync(dF, f_ext_x, b, hipMemcpyHostToDevice, s));
+    LBX_TRY(hipMemcpyAsync(dF + N, f_ext_y, b, hipMemcpyHostToDevice, s));
+    LBX_TRY(hipMemcpyAsync(dF + 2 * N, f_ext_z, b, hipMemcpyHostToDevice, s));
+    LBX_TRY(hipMemcpyAsync(dnat, nature, N, hipMemcpyHostToDevice, s));
+    double A, B, C;
+    relaxation_coeffs(tau, &A, &B, &C);
+    hipLaunchKernelGGL(k_collide_box, dim3(nblocks(N)), dim3(kBlock), 0, s, dn, dr, dj, dF, dnat, (uint32_t)N, A, B,
+                       C);
+    LBX_TRY(hipGetLastError());
+    LBX_TRY(hipMemcpyAsync(n, dn, kQ * b, hipMemcpyDeviceToHost, s));
+    return hip_status(hipStreamSynchronize(s));
 }
 
 int lbx_timing_enable(lbx_ctx* c, int32_t on) {

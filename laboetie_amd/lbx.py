@@ -16,7 +16,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblbx.so")
 
-OK, ERR_NEG_POP, ERR_RESTPART, ERR_TAU, ERR_NOMEM, ERR_ARG, ERR_DEVICE, ERR_STATE = range(8)
+OK, ERR_NEG_POP, ERR_RESTPART, ERR_TAU, ERR_NOMEM, ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_SIZE = range(9)
+NOALLOC_MAX_NODES = 320000
 PREC_FP64, PREC_FP32_SHIFT, PREC_FP32_STORE = 0, 1, 2
 EXIT_CONVERGED, EXIT_CAP, EXIT_FLOOR = 1, 2, 3
 KERNEL_LB, KERNEL_MP = 0, 1
@@ -62,6 +63,7 @@ SIGNATURES = {
     "lbx_mp_deff_batch": (ctypes.c_int, [_p, _p]),
     "lbx_mp_free": (ctypes.c_int, [_p]),
     "lbx_collide": (ctypes.c_int, [_i32] * 3 + [_p, _f64] + [_p] * 8),
+    "lbx_collide_noalloc": (ctypes.c_int, [_i32] * 3 + [_p, _f64] + [_p] * 8),
     "lbx_timing_enable": (ctypes.c_int, [_p, _i32]),
     "lbx_kernel_time": (ctypes.c_int, [_p, _i32, _p, _p]),
     "lbx_set_floor_window": (ctypes.c_int, [_p, _i64]),
@@ -273,13 +275,16 @@ def copy_bandwidth(bytes_=2 << 30, iters=20):
     return g.value
 
 
-def collide(dims, nature, tau, n, density, j, f):
-    """module_collision::collide on host arrays; returns the new n."""
+def collide(dims, nature, tau, n, density, j, f, noalloc=False):
+    """module_collision::collide on host arrays; returns the new n.
+    noalloc: through lbx_collide_noalloc (no device allocation; at most
+    NOALLOC_MAX_NODES nodes)."""
     lx, ly, lz = dims
     n = np.array(n, np.float64)
     arrs = [np.ascontiguousarray(a, np.float64) for a in (density, j[0], j[1], j[2], f[0], f[1], f[2])]
-    _check(lib().lbx_collide(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)), tau, _ptr(n),
-                             *[_ptr(a) for a in arrs]), "collide")
+    fn = lib().lbx_collide_noalloc if noalloc else lib().lbx_collide
+    _check(fn(lx, ly, lz, _ptr(np.ascontiguousarray(nature, np.int8)), tau, _ptr(n), *[_ptr(a) for a in arrs]),
+           "collide_noalloc" if noalloc else "collide")
     return n
 
 
