@@ -179,6 +179,35 @@ int lbx_set_state(lbx_ctx* ctx, const double* n, const double* rho,
 int lbx_equilibrate(lbx_ctx* ctx, double target_error, const double fext[3], int64_t max_steps,
                     double* l2err_trace, int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason);
 
+/* The same loop in pieces, for callers that write outputs along the run
+ * (the profiles every print_files_frequency steps and output/
+ * total_mass_flux.dat of equilibration.f90:154-176,257-261, the stdout lines
+ * of :149).  lbx_equilibrate_begin starts a run from the current state (the
+ * opening collision of step 1); with flux_sums != 0 every step also sums j
+ * over the fluid nodes (lbx_flux_sums).  lbx_equilibrate_advance runs it on
+ * until step until_step (INT64_MAX: to its end) or its end, whichever comes
+ * first: *exit_reason = 0 while the run goes on, else the reason as for
+ * lbx_equilibrate, with the state extracted as lbx_equilibrate leaves it.
+ * lbx_equilibrate is begin + advance to the end.  Between the calls of a
+ * paused run, lbx_flow gives the density and flux after the last step taken,
+ * lbx_l2err_trace and lbx_flux_sums the steps taken so far; lbx_get_state
+ * and lbx_mp_init* need the run's end (LBX_ERR_STATE); a new begin,
+ * lbx_init_rest or lbx_set_state ends or drops a paused run. */
+int lbx_equilibrate_begin(lbx_ctx* ctx, double target_error, const double fext[3], int64_t max_steps,
+                          int32_t flux_sums);
+int lbx_equilibrate_advance(lbx_ctx* ctx, int64_t until_step, int64_t* steps_done, int32_t* exit_reason);
+
+/* sum over the fluid nodes of j(t) = F(t)/2 + sum n c, 3 values per step, for
+ * steps first_step .. first_step+count-1 of a run begun with flux_sums (the
+ * value output/total_mass_flux.dat prints, in single precision, at step t+1);
+ * *available (may be NULL) receives how many steps are held. */
+int lbx_flux_sums(lbx_ctx* ctx, int64_t first_step, int64_t count, double* out, int64_t* available);
+
+/* Density and flux (full box, 0 at solid nodes) after the last step taken:
+ * of a paused run (without disturbing it), else what lbx_get_state gives.
+ * LBX_ERR_STATE before the first step of a run. */
+int lbx_flow(lbx_ctx* ctx, double* rho, double* jx, double* jy, double* jz);
+
 /* The per-step l2err of the last lbx_equilibrate call, steps first_step ..
  * first_step+count-1 (1-based, as written to output/l2err.dat by
  * equilibration.f90:339-344); *available (may be NULL) receives how many

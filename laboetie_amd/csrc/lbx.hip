@@ -42,6 +42,24 @@ using namespace lbx;
 
 // ====================================================================== host
 
+// The host state of an equilibration between lbx_equilibrate_begin and the
+// end of its run (the loop of equilibration.f90:143-491, in launch runs).
+struct EqRun {
+    bool active = false;    // a run is under way
+    bool began = false;     // a run has been started and has ended
+    bool flux_sums = false; // per-step sums of j
+    double target = 0;
+    double F[3] = {0, 0, 0};
+    int64_t max_steps = 0;
+    int stage = 1;
+    int64_t switch_step = INT64_MAX;  // the stage-1 convergence step: F acts from the step after it
+    int64_t t = 0;                    // last completed step
+    int64_t check_after = 2;          // stop tests apply to steps > check_after (t > 2, equilibration.f90:346)
+    int64_t chunk = 4;                // steps of the next launch run
+    int status = LBX_OK;
+    int32_t reason = 0;
+};
+
 struct lbx_ctx {
     int32_t lx = 0, ly = 0, lz = 0;
     FastDiv dlx{}, dly{};
@@ -98,6 +116,12 @@ struct lbx_ctx {
     int64_t t_n[2] = {0, 0};
     std::vector<double> l2host;
     std::vector<double> l2trace;  // every l2err of the last lbx_equilibrate call
+    // a running equilibration (lbx_equilibrate_begin .. the end of the run)
+    EqRun eq;
+    double* jpart = nullptr;          // flux sums: per-tile partial sums of j, [step % kChunk][tile][3]
+    std::vector<double> jpart_h;
+    std::vector<double> jsum_trace;   // sum of j over the fluid nodes, 3 per step
+    double* jflow = nullptr;          // lbx_flow: the flux of the last step
 };
 
 // the forms that store populations as FP32 deviations (FP32-shift: FP32
@@ -209,6 +233,8 @@ void free_all(lbx_ctx* c) {
     dfree(c->slots);
     dfree(c->ctrl);
     dfree(c->l2ring);
+    dfree(c->jpart);
+    dfree(c->jflow);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e), e = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream), c->stream = nullptr;
@@ -303,6 +329,7 @@ StepArgs<E, R> step_args(lbx_ctx* c, int64_t t, const double fm[3], const double
     a.target = target;
     a.write_l2 = write_l2;
     a.have_prev = 0;
+    a.jpart = c->eq.flux_sums ? c->jpart + (uint64_t)((t - 1) % kChunk) * 3 * c->ntiles : nullptr;
     a.check_every = check_every;
     a.check_base = check_base;
     a.check_inv = (R)(1.0 / check_every);
@@ -613,6 +640,7 @@ int lbx_destroy(lbx_ctx* c) {
 
 int lbx_init_rest(lbx_ctx* c, double rho0) {
     if (!c) return LBX_ERR_ARG;
+    c->eq.active = false;  // a paused run is abandoned
     if (f32_storage(c->prec)) c->rho0 = rho0;  // the deviations follow the rest state
     hipLaunchKernelGGL(k_init_rest, dim3(c->ntiles), dim3(kBlock), 0, c->stream, c->f[0], c->g[0], c->j32, c->rho,
                        c->j, 0u, c->Nf, c->Np, rho0);
@@ -624,6 +652,7 @@ int lbx_init_rest(lbx_ctx* c, double rho0) {
 int lbx_set_state(lbx_ctx* c, const double* n, const double* rho, const double* jx, const double* jy,
                   const double* jz) {
     if (!c || !n || !rho || !jx || !jy || !jz) return LBX_ERR_ARG;
+    c->eq.active = false;  // a paused run is abandoned
     if (f32_storage(c->prec)) {
         // the rest density of the deviations: the density of the first fluid
         // node (the uniform rest density when the state is a rest state)
@@ -655,6 +684,7 @@ int lbx_l2err_trace(lbx_ctx* c, int64_t first_step, int64_t count, double* out, 
 
 int lbx_get_state(lbx_ctx* c, double* n, double* rho, double* jx, double* jy, double* jz) {
     if (!c) return LBX_ERR_ARG;
+    if (c->eq.active) return LBX_ERR_STATE;  // during a paused run: lbx_flow
     if (n && f32_storage(c->prec)) LBX_TRY(download_shift(c, c->g[c->pre], n));
     if (n && !f32_storage(c->prec)) LBX_TRY(download_full(c, c->f[c->pre], kQ, n));
     if (rho) LBX_TRY(download_full(c, c->rho, 1, rho));
@@ -664,13 +694,44 @@ int lbx_get_state(lbx_ctx* c, double* n, double* rho, double* jx, double* jy, do
     return LBX_OK;
 }
 
-int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64_t max_steps,
-                    double* l2err_trace, int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason) {
-    if (!c || !fext) return LBX_ERR_ARG;
-    if (steps_done) *steps_done = 0;
-    if (exit_reason) *exit_reason = 0;
+// The end of an equilibration run: the post-streaming state of its last step
+// into the buffer lbx_get_state reads (f[t&1] holds the unused opening
+// collision of step t+1), density alongside.
+int end_run(lbx_ctx* c) {
+    EqRun& q = c->eq;
     const double zero[3] = {0.0, 0.0, 0.0};
-    const double F[3] = {fext[0], fext[1], fext[2]};
+    const uint32_t gt = c->ntiles;
+    const bool shift = f32_storage(c->prec);
+    const int64_t t = q.t;
+    if (t > 0) {
+        // post-streaming state of step t into the other buffer (f[t&1] holds
+        // the unused opening collision of step t+1), density alongside
+        const int src = (t - 1) & 1, dst = t & 1;
+        if (shift) {
+            // FP64 density and flux of step t from its FP32 deviations; j(t)
+            // carries F(t)/2 as in the step kernel
+            const double* Ft = t > q.switch_step ? q.F : zero;
+            hipLaunchKernelGGL(k_extract_shift, dim3(gt), dim3(kBlock), 0, c->stream, c->g[src], c->g[dst], c->rho,
+                               c->j, c->nodes, c->map, c->dlx, c->dly, c->lz, 0u, c->Nf, c->Np, c->rho0,
+                               Ft[0] / 2.0, Ft[1] / 2.0, Ft[2] / 2.0);
+        } else {
+            hipLaunchKernelGGL(k_extract, dim3(gt), dim3(kBlock), 0, c->stream, c->f[src], c->f[dst], c->rho,
+                               c->nodes, c->map, c->dlx, c->dly, c->lz, 0u, c->Nf, c->Np);
+        }
+        LBX_TRY(hipGetLastError());
+        c->pre = dst;
+    } else {
+        c->pre = 0;
+    }
+    q.active = false;
+    q.began = true;
+    return hip_status(hipStreamSynchronize(c->stream));
+}
+
+int lbx_equilibrate_begin(lbx_ctx* c, double target_error, const double fext[3], int64_t max_steps,
+                          int32_t flux_sums) {
+    if (!c || !fext || max_steps < 0) return LBX_ERR_ARG;
+    if (c->eq.active) LBX_TRY(end_run(c));  // a paused run ends where it is
     const uint32_t gt = c->ntiles;
     const bool shift = f32_storage(c->prec);
     // opening collision of step 1 (stage 1: no force), into buffer 0
@@ -694,32 +755,53 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
     LBX_TRY(hipGetLastError());
     LBX_TRY(reset_ctrl(c));
     c->l2trace.clear();
+    c->jsum_trace.clear();
     LBX_TRY(hipMemsetAsync(c->l2ring, 0, kRing * sizeof(double), c->stream));
     LBX_TRY(hipMemsetAsync(c->slots, 0, 3 * kSlots * sizeof(unsigned long long), c->stream));
+    if (flux_sums && !c->jpart) LBX_TRY(dalloc(&c->jpart, (uint64_t)kChunk * 3 * gt));
+    if (flux_sums) c->jpart_h.resize((size_t)kChunk * 3 * gt);
+    c->eq = EqRun{};
+    c->eq.active = true;
+    c->eq.target = target_error;
+    for (int a = 0; a < 3; ++a) c->eq.F[a] = fext[a];
+    c->eq.max_steps = max_steps;
+    c->eq.flux_sums = flux_sums != 0;
+    c->pre = -1;  // no extracted state until the run ends
+    return hip_status(hipStreamSynchronize(c->stream));
+}
 
-    int stage = 1;
-    int64_t switch_step = INT64_MAX;  // the stage-1 convergence step: F acts from the step after it
-    int64_t t = 0;            // last completed step
-    int64_t check_after = 2;  // stop tests apply to steps > check_after (t > 2, equilibration.f90:346)
-    int status = LBX_OK;
-    int32_t reason = 0;
-    int64_t chunk = 4;  // grows to kChunk: the force switch normally comes at t = 3
+int lbx_equilibrate_advance(lbx_ctx* c, int64_t until_step, int64_t* steps_done, int32_t* exit_reason) {
+    if (!c) return LBX_ERR_ARG;
+    EqRun& q = c->eq;
+    if (steps_done) *steps_done = q.t;
+    if (exit_reason) *exit_reason = q.reason;
+    if (!q.active) return q.began ? q.status : LBX_ERR_STATE;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    const uint32_t gt = c->ntiles;
+    const bool shift = f32_storage(c->prec);
     for (;;) {
-        int64_t S = chunk;
-        chunk = std::min<int64_t>(2 * chunk, kChunk);
-        if (max_steps > 0) S = std::min<int64_t>(S, max_steps - t);
-        if (S <= 0) {
-            reason = LBX_EXIT_CAP;
+        if (q.max_steps > 0 && q.t >= q.max_steps) {
+            // a convergence at the capped step is still reported as such
+            const double l2 = q.t > 0 ? c->l2trace[q.t - 1] : HUGE_VAL;
+            q.reason = (q.stage == 2 && q.t > q.check_after && l2 <= q.target) ? LBX_EXIT_CONVERGED : LBX_EXIT_CAP;
             break;
         }
-        const double* Fs = stage == 1 ? zero : F;
+        if (q.t >= until_step) {  // paused: the run goes on at the next call
+            if (steps_done) *steps_done = q.t;
+            return LBX_OK;
+        }
+        int64_t S = std::min<int64_t>(q.chunk, until_step - q.t);
+        q.chunk = std::min<int64_t>(2 * q.chunk, kChunk);  // grows to kChunk: the force switch normally comes at t = 3
+        if (q.max_steps > 0) S = std::min<int64_t>(S, q.max_steps - q.t);
+        const double* Fs = q.stage == 1 ? zero : q.F;
         // l2err of every step in stage 1; in stage 2 every check_every steps
         // from the switch (every step in FP64)
-        const int32_t every = stage == 1 ? 1 : c->check_every;
-        const int64_t base = stage == 1 ? 0 : switch_step;
+        const int32_t every = q.stage == 1 ? 1 : c->check_every;
+        const int64_t base = q.stage == 1 ? 0 : q.switch_step;
+        const int64_t t = q.t;
         timer_begin(c);
         for (int64_t k = 1; k <= S; ++k)
-            launch_lb_step(c, t + k, Fs, Fs, check_after, target_error, 1, t + k > 1, every, base);
+            launch_lb_step(c, t + k, Fs, Fs, q.check_after, q.target, 1, t + k > 1, every, base);
         timer_stop(c);
         hipLaunchKernelGGL(k_lb_finalize, dim3(1), dim3(kSlots), 0, c->stream, c->slots, c->l2ring, c->ctrl, t + S,
                            base, every);
@@ -727,71 +809,105 @@ int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64
         Ctrl h;
         LBX_TRY(hipMemcpyAsync(c->l2host.data(), c->l2ring, kRing * sizeof(double), hipMemcpyDeviceToHost,
                                c->stream));
+        if (q.flux_sums)
+            LBX_TRY(hipMemcpyAsync(c->jpart_h.data(), c->jpart, (size_t)kChunk * 3 * gt * sizeof(double),
+                                   hipMemcpyDeviceToHost, c->stream));
         LBX_TRY(read_ctrl(c, &h));
         int64_t last = t + S;  // steps actually executed in this chunk
         if (h.negpop >= 0) last = std::min<int64_t>(last, h.negpop);
         if (h.stop) last = std::min<int64_t>(last, h.exit_step);
         timer_end(c, LBX_KERNEL_LB, last - t);
-        // l2err of the steps that completed
+        // l2err (and flux sums) of the steps that completed
         const int64_t last_l2 = (h.negpop >= 0 && h.negpop <= last) ? last - 1 : last;
-        for (int64_t s = t + 1; s <= last_l2; ++s) {
-            if ((int64_t)c->l2trace.size() < s) c->l2trace.resize(s);
-            c->l2trace[s - 1] = c->l2host[s % kRing];
-            if (l2err_trace && s <= trace_cap) l2err_trace[s - 1] = c->l2host[s % kRing];
+        for (int64_t st = t + 1; st <= last_l2; ++st) {
+            if ((int64_t)c->l2trace.size() < st) c->l2trace.resize(st);
+            c->l2trace[st - 1] = c->l2host[st % kRing];
+            if (q.flux_sums) {
+                // sum over the tiles' partials in tile order
+                const double* pp = c->jpart_h.data() + (size_t)((st - 1) % kChunk) * 3 * gt;
+                double sum[3] = {pp[0], pp[1], pp[2]};
+                for (uint32_t b = 1; b < gt; ++b)
+                    for (int a = 0; a < 3; ++a) sum[a] = sum[a] + pp[3 * (size_t)b + a];
+                c->jsum_trace.resize(3 * (size_t)st);
+                for (int a = 0; a < 3; ++a) c->jsum_trace[3 * (size_t)(st - 1) + a] = sum[a];
+            }
         }
         if (h.negpop >= 0) {
-            status = LBX_ERR_NEG_POP;
-            t = h.negpop;
+            q.status = LBX_ERR_NEG_POP;
+            q.t = h.negpop;
             break;
         }
         if (h.stop) {
             const int64_t e = h.exit_step;
-            if (stage == 1) {
+            if (q.stage == 1) {
                 // the first convergence switches the force on at the end of step e
                 // (equilibration.f90:377-386): redo the collision that opens e+1
                 LBX_TRY(reset_ctrl(c, e));
-                launch_lb_step(c, e, zero, F, INT64_MAX, target_error, 0, 0);
+                launch_lb_step(c, e, zero, q.F, INT64_MAX, q.target, 0, 0);
                 LBX_TRY(hipGetLastError());
-                stage = 2;
-                switch_step = e;
-                t = e;
-                check_after = e;  // l2err(e) belongs to stage 1
+                q.stage = 2;
+                q.switch_step = e;
+                q.t = e;
+                q.check_after = e;  // l2err(e) belongs to stage 1
                 continue;
             }
-            t = e;
-            reason = h.floor_hit ? LBX_EXIT_FLOOR : LBX_EXIT_CONVERGED;
+            q.t = e;
+            q.reason = h.floor_hit ? LBX_EXIT_FLOOR : LBX_EXIT_CONVERGED;
             break;
         }
-        t += S;
-        if (max_steps > 0 && t >= max_steps) {
-            // a convergence at the capped step is still reported as such
-            const double l2 = c->l2host[t % kRing];
-            reason = (stage == 2 && t > check_after && l2 <= target_error) ? LBX_EXIT_CONVERGED : LBX_EXIT_CAP;
-            break;
-        }
+        q.t += S;
     }
-    if (steps_done) *steps_done = t;
-    if (exit_reason) *exit_reason = reason;
-    if (t > 0) {
-        // post-streaming state of step t into the other buffer (f[t&1] holds
-        // the unused opening collision of step t+1), density alongside
-        const int src = (t - 1) & 1, dst = t & 1;
-        if (shift) {
-            // FP64 density and flux of step t from its FP32 deviations; j(t)
-            // carries F(t)/2 as in the step kernel
-            const double* Ft = t > switch_step ? F : zero;
-            hipLaunchKernelGGL(k_extract_shift, dim3(gt), dim3(kBlock), 0, c->stream, c->g[src], c->g[dst], c->rho,
-                               c->j, c->nodes, c->map, c->dlx, c->dly, c->lz, 0u, c->Nf, c->Np, c->rho0,
-                               Ft[0] / 2.0, Ft[1] / 2.0, Ft[2] / 2.0);
-        } else {
-            hipLaunchKernelGGL(k_extract, dim3(gt), dim3(kBlock), 0, c->stream, c->f[src], c->f[dst], c->rho,
-                               c->nodes, c->map, c->dlx, c->dly, c->lz, 0u, c->Nf, c->Np);
-        }
-        LBX_TRY(hipGetLastError());
-        c->pre = dst;
-    }
-    LBX_TRY(hipStreamSynchronize(c->stream));
-    return status;
+    LBX_TRY(end_run(c));
+    if (steps_done) *steps_done = q.t;
+    if (exit_reason) *exit_reason = q.reason;
+    return q.status;
+}
+
+int lbx_equilibrate(lbx_ctx* c, double target_error, const double fext[3], int64_t max_steps,
+                    double* l2err_trace, int64_t trace_cap, int64_t* steps_done, int32_t* exit_reason) {
+    if (!c || !fext) return LBX_ERR_ARG;
+    if (steps_done) *steps_done = 0;
+    if (exit_reason) *exit_reason = 0;
+    LBX_TRY(lbx_equilibrate_begin(c, target_error, fext, max_steps, 0));
+    const int st = lbx_equilibrate_advance(c, INT64_MAX, steps_done, exit_reason);
+    if (l2err_trace)
+        for (int64_t s = 0; s < std::min<int64_t>(trace_cap, (int64_t)c->l2trace.size()); ++s)
+            l2err_trace[s] = c->l2trace[s];
+    return st;
+}
+
+int lbx_flux_sums(lbx_ctx* c, int64_t first_step, int64_t count, double* out, int64_t* available) {
+    if (!c || first_step < 1 || count < 0 || (count > 0 && !out)) return LBX_ERR_ARG;
+    const int64_t n = (int64_t)c->jsum_trace.size() / 3;
+    if (available) *available = n;
+    if (first_step + count - 1 > n) return LBX_ERR_ARG;
+    for (int64_t k = 0; k < 3 * count; ++k) out[k] = c->jsum_trace[3 * (size_t)(first_step - 1) + k];
+    return LBX_OK;
+}
+
+int lbx_flow(lbx_ctx* c, double* rho, double* jx, double* jy, double* jz) {
+    if (!c) return LBX_ERR_ARG;
+    if (!c->eq.active) return lbx_get_state(c, nullptr, rho, jx, jy, jz);
+    const int64_t t = c->eq.t;
+    if (t == 0) return LBX_ERR_STATE;  // no step taken yet: the caller holds the initial state
+    if (!c->jflow) LBX_TRY(dalloc(&c->jflow, 4 * c->Np));
+    const uint32_t gt = c->ntiles;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    const double* Ft = t > c->eq.switch_step ? c->eq.F : zero;
+    const int src = (t - 1) & 1;
+    if (f32_storage(c->prec))
+        hipLaunchKernelGGL(k_flow_shift, dim3(gt), dim3(kBlock), 0, c->stream, c->g[src], c->jflow, c->nodes,
+                           c->map, c->dlx, c->dly, c->lz, c->Nf, c->Np, c->rho0, Ft[0] / 2.0, Ft[1] / 2.0,
+                           Ft[2] / 2.0);
+    else
+        hipLaunchKernelGGL(k_flow, dim3(gt), dim3(kBlock), 0, c->stream, c->f[src], c->jflow, c->nodes,
+                           c->map, c->dlx, c->dly, c->lz, c->Nf, c->Np, Ft[0] / 2.0, Ft[1] / 2.0, Ft[2] / 2.0);
+    LBX_TRY(hipGetLastError());
+    if (rho) LBX_TRY(download_full(c, c->jflow, 1, rho));
+    double* jp[3] = {jx, jy, jz};
+    for (int a = 0; a < 3; ++a)
+        if (jp[a]) LBX_TRY(download_full(c, c->jflow + (a + 1) * c->Np, 1, jp[a]));
+    return LBX_OK;
 }
 
 static int mp_setup(lbx_ctx* c, const double fext[3], double Db, int32_t ntr, const double* ka, const double* kd,
@@ -861,12 +977,14 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, int32_t ntr, co
 
 int lbx_mp_init(lbx_ctx* c, const double fext[3], double Db, double ka, double kd, double vacf0[3]) {
     if (!c || !fext) return LBX_ERR_ARG;
+    if (c->eq.active) return LBX_ERR_STATE;  // the flow of a paused run is not extracted
     return mp_setup(c, fext, Db, 1, &ka, &kd, vacf0);
 }
 
 int lbx_mp_init_batch(lbx_ctx* c, const double fext[3], double Db, int32_t ntracers, const double* ka,
                       const double* kd, double* vacf0) {
     if (!c || !fext || !ka || !kd) return LBX_ERR_ARG;
+    if (c->eq.active) return LBX_ERR_STATE;  // the flow of a paused run is not extracted
     return mp_setup(c, fext, Db, ntracers, ka, kd, vacf0);
 }
 

@@ -121,6 +121,9 @@ struct StepArgs {
     int32_t pad_ = 0;
     int64_t check_base = 0;
     R check_inv = R(1);     // 1 / check_every
+    // non-NULL: each block writes the sum of j(t) over its nodes (3 doubles,
+    // in a fixed order) to jpart[3 * tile ...] (the per-step total mass flux)
+    double* jpart = nullptr;
 };
 
 // A = 1 - 1/tau and B = 1/tau rounded to FP32 with A + B = 1 exactly, so that
@@ -170,6 +173,27 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 // max over the kSlots slots of one step (called by the first wave of a block)
 __device__ __forceinline__ unsigned long long slots_max(const unsigned long long* set) {
     return wave_max_u64(vload(&set[threadIdx.x & (kSlots - 1)]));
+}
+
+// Per-block sums of NV values in a fixed order, written by thread 0 to
+// out[0 .. NV-1].
+template <int NV, int BLOCK>
+__device__ __forceinline__ void block_partial(const double (&v)[NV], double* out, double* lds /* [BLOCK/64*NV] */) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+        const double s = wave_sum(v[a]);
+        if (lane == 0) lds[w * NV + a] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < NV; ++a) {
+            double s = lds[a];
+            for (int k = 1; k < BLOCK / 64; ++k) s = s + lds[k * NV + a];
+            out[a] = s;
+        }
+    }
+    __syncthreads();
 }
 
 // Pull-streaming with half-way bounce-back (equilibration.f90:204-243): the
@@ -275,6 +299,7 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
     if (a.write_l2 && first_block && threadIdx.x < kSlots) a.slots[((a.t + 1) % 3) * kSlots + threadIdx.x] = 0ull;
     double l2 = 0.0;
     bool neg = false;
+    double jsum[3] = {0.0, 0.0, 0.0};  // this node's j(t), for the flux sums
     if (i < a.Nf) {
         const uint32_t Np = (uint32_t)a.Np;
         const View<Wide, E> fin(a.fin, kQ * a.Np), fout(a.fout, kQ * a.Np);
@@ -319,6 +344,9 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
             jv.st(Np + i, jy);
             jv.st(2 * Np + i, jz);
         }
+        jsum[0] = (double)jx;
+        jsum[1] = (double)jy;
+        jsum[2] = (double)jz;
         if constexpr (kShift) {
             const R drho = rho, full = a.rho0 + drho;
             const R ux = jx / full, uy = jy / full, uz = jz / full;
@@ -352,6 +380,10 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
             if (bneg || m > 0.0)
                 atomicMax(&a.slots[(a.t % 3) * kSlots + (stripe & (kSlots - 1))],
                           bneg ? kNegMark : (unsigned long long)__double_as_longlong(m));
+        }
+        if (a.jpart) {
+            __shared__ double jl[BLOCK / 64 * 3];
+            block_partial<3, BLOCK>(jsum, a.jpart + 3 * (uint64_t)tile, jl);
         }
     }
 }
@@ -431,6 +463,62 @@ __global__ __launch_bounds__(kBlock) void k_extract_shift(const float* __restric
     j_out[i] = jx;
     j_out[Np + i] = jy;
     j_out[2 * Np + i] = jz;
+}
+
+// The density SUM(n,4) (equilibration.f90:254) and flux F(t)/2 + sum n c
+// (:293-300) of the post-streaming state of the step whose input is fin,
+// without writing the populations (the run goes on from fin): out[0][Np] the
+// density, out[1..3][Np] the flux.  The same sums in the same order as
+// k_extract and the step kernel.
+__global__ __launch_bounds__(kBlock) void k_flow(const double* __restrict__ fin, double* __restrict__ out,
+                                                 const uint32_t* __restrict__ nodes, const uint32_t* __restrict__ map,
+                                                 FastDiv dlx, FastDiv dly, uint32_t lz, uint32_t Nf, uint64_t Np,
+                                                 double fmx, double fmy, double fmz) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= Nf) return;
+    Neighbours nb;
+    nb.load(map, nodes[i] & kIndexMask, dlx, dly, lz);
+    const View<true> fv(fin, kQ * Np);
+    double rho = 0.0, jx = fmx, jy = fmy, jz = fmz;
+    static_for<0, kQ>([&](auto I) {
+        constexpr int L = I;
+        const double v = pull<L, true>(fv, (uint32_t)Np, i, nb);
+        rho = (L == 0) ? v : rho + v;
+        if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(v);
+        if constexpr (kCY[L] != 0) jy = jy + sgn<kCY[L]>(v);
+        if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(v);
+    });
+    out[i] = rho;
+    out[Np + i] = jx;
+    out[2 * Np + i] = jy;
+    out[3 * Np + i] = jz;
+}
+
+// The same for the FP32 storage forms, as k_extract_shift sums them: FP64
+// density rho0 + sum g and flux F(t)/2 + sum g c from the FP32 deviations.
+__global__ __launch_bounds__(kBlock) void k_flow_shift(const float* __restrict__ gin, double* __restrict__ out,
+                                                       const uint32_t* __restrict__ nodes,
+                                                       const uint32_t* __restrict__ map, FastDiv dlx, FastDiv dly,
+                                                       uint32_t lz, uint32_t Nf, uint64_t Np, double rho0,
+                                                       double fmx, double fmy, double fmz) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= Nf) return;
+    Neighbours nb;
+    nb.load(map, nodes[i] & kIndexMask, dlx, dly, lz);
+    const View<true, float> gv(gin, kQ * Np);
+    double drho = 0.0, jx = fmx, jy = fmy, jz = fmz;
+    static_for<0, kQ>([&](auto I) {
+        constexpr int L = I;
+        const double d = pull<L, true>(gv, (uint32_t)Np, i, nb);
+        drho = (L == 0) ? d : drho + d;
+        if constexpr (kCX[L] != 0) jx = jx + sgn<kCX[L]>(d);
+        if constexpr (kCY[L] != 0) jy = jy + sgn<kCY[L]>(d);
+        if constexpr (kCZ[L] != 0) jz = jz + sgn<kCZ[L]>(d);
+    });
+    out[i] = rho0 + drho;
+    out[Np + i] = jx;
+    out[2 * Np + i] = jy;
+    out[3 * Np + i] = jz;
 }
 
 // The collision on packed fields with a uniform force (the opening collision
@@ -537,27 +625,6 @@ struct LdsView {
 // arithmetic of each tracer is exactly that of a single-tracer run.
 
 constexpr int kMaxTracers = 4;
-
-// Per-block sums of NV values in a fixed order, written by thread 0 to
-// out[0 .. NV-1].
-template <int NV, int BLOCK>
-__device__ __forceinline__ void block_partial(const double (&v)[NV], double* out, double* lds /* [BLOCK/64*NV] */) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int a = 0; a < NV; ++a) {
-        const double s = wave_sum(v[a]);
-        if (lane == 0) lds[w * NV + a] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int a = 0; a < NV; ++a) {
-            double s = lds[a];
-            for (int k = 1; k < BLOCK / 64; ++k) s = s + lds[k * NV + a];
-            out[a] = s;
-        }
-    }
-    __syncthreads();
-}
 
 // VACF of one moment-propagation step (NV = 3 per tracer) from the per-block
 // partial sums, in a fixed order that does not depend on which kernel runs it

@@ -2,7 +2,10 @@
 ! argument-less interface, same lb.in keys, same files under output/, same
 ! module state on exit (n, node%solventDensity, node%solventFlux).  The time
 ! loop (collide, bounce-back, streaming, moments, convergence and the force
-! switch) runs on the GPU in one lbx_equilibrate call.
+! switch) runs on the GPU (lbx_equilibrate_begin / _advance) in runs of up to
+! 256 steps; between them the shim writes the outputs of the steps taken: the
+! stdout step lines and l2err.dat, total_mass_flux.dat, and the profiles every
+! print_files_frequency steps.
 subroutine equilibration
 
     use iso_c_binding, only: c_double, c_int64_t, c_int32_t, c_null_ptr
@@ -15,11 +18,13 @@ subroutine equilibration
 
     implicit none
     integer :: n1, n2, n3, print_frequency, print_files_frequency, u13
-    integer(c_int64_t) :: steps, held, t, max_steps
-    integer(c_int32_t) :: reason
+    integer(c_int64_t) :: steps, done, until, next_prof, t, max_steps
+    integer(c_int64_t), parameter :: run_steps = 256
+    integer(c_int32_t) :: reason, flux_sums
     integer(c_int) :: status
+    logical :: write_total_mass_flux
     real(dp) :: target_error, tau, f_ext_loc(3), l2prev
-    real(dp), allocatable :: density(:,:,:), jx(:,:,:), jy(:,:,:), jz(:,:,:), trace(:)
+    real(dp), allocatable :: density(:,:,:), jx(:,:,:), jy(:,:,:), jz(:,:,:)
     real(dp), parameter :: eps = epsilon(1._dp)
 
     ! laboetie's path is uncharged only (equilibration.f90:28-33)
@@ -29,8 +34,7 @@ subroutine equilibration
     end if
     if (getinput%log("compensate_f_ext", .false.)) &
         error stop "compensate_f_ext = T is not available on the lbx path"
-    if (getinput%log("write_total_mass_flux", .false.)) &
-        error stop "write_total_mass_flux = T is not available on the lbx path"
+    write_total_mass_flux = getinput%log("write_total_mass_flux", .false.)
 
     n1 = getinput%int("lx", assert=">0")
     n2 = getinput%int("ly", assert=">0")
@@ -43,8 +47,6 @@ subroutine equilibration
     f_ext_loc = getinput%dp3("f_ext", [0._dp, 0._dp, 0._dp])
     ! extension key (absent from the reference): cap on the number of steps, 0 = none
     max_steps = int(getinput%int("maximum_equilibration_steps", 0), c_int64_t)
-    if (print_files_frequency /= huge(1)) &
-        print *, "note: the lbx path writes the profiles at t=1 and at the steady state only"
 
     allocate (density(n1, n2, n3), jx(n1, n2, n3), jy(n1, n2, n3), jz(n1, n2, n3))
     density = node%solventdensity
@@ -61,6 +63,7 @@ subroutine equilibration
     open (56, file="output/mean-density_profile_along_z.dat")
     open (57, file="output/mean-density_profile_along_y.dat")
     open (58, file="output/mean-density_profile_along_x.dat")
+    if (write_total_mass_flux) open (65, file="output/total_mass_flux.dat")
 
     print *
     print *, 'Lattice Boltzmann'
@@ -73,31 +76,35 @@ subroutine equilibration
 
     call lbx_ensure_lattice(real(tau, c_double))
     call lbx_check(lbx_set_state(lbx_lattice, n, density, jx, jy, jz), "lbx_set_state")
-    status = lbx_equilibrate(lbx_lattice, real(target_error, c_double), real(f_ext_loc, c_double), max_steps, &
-                             c_null_ptr, 0_c_int64_t, steps, reason)
+    flux_sums = 0
+    if (write_total_mass_flux) flux_sums = 1
+    call lbx_check(lbx_equilibrate_begin(lbx_lattice, real(target_error, c_double), real(f_ext_loc, c_double), &
+                                         max_steps, flux_sums), "lbx_equilibrate_begin")
 
-    ! per-step outputs, from the trace of the run (equilibration.f90:149,339-344)
-    held = 0
-    allocate (trace(1))
-    call lbx_check(lbx_l2err_trace(lbx_lattice, 1_c_int64_t, 0_c_int64_t, trace, held), "lbx_l2err_trace")
-    if (held > 0) then
-        deallocate (trace)
-        allocate (trace(held))
-        call lbx_check(lbx_l2err_trace(lbx_lattice, 1_c_int64_t, held, trace, held), "lbx_l2err_trace")
-    end if
     open (newunit=u13, file="./output/l2err.dat")
     l2prev = 0
-    do t = 1, steps
-        if (modulo(t, int(print_frequency, c_int64_t)) == 0) &
-            print *, int(t), real(l2prev), "(target", real(target_error, 4), ")"
-        if (t > held) exit
-        ! the FP32 storage forms measure l2err every 64 steps in stage 2: the
-        ! other steps (NaN in the trace) have no line
-        if (ieee_is_nan(trace(t))) cycle
-        write (u13, *) int(t), trace(t)
-        l2prev = trace(t)
+    done = 0
+    ! the next step that opens with a profile (the one of t = 1 is written)
+    next_prof = huge(1_c_int64_t)
+    if (print_files_frequency < huge(1)) next_prof = max(print_files_frequency, 2)
+    if (next_prof < huge(1_c_int64_t) .and. modulo(next_prof, int(print_files_frequency, c_int64_t)) /= 0) &
+        next_prof = next_prof + print_files_frequency - modulo(next_prof, int(print_files_frequency, c_int64_t))
+    do
+        until = done + run_steps
+        if (next_prof < huge(1_c_int64_t)) until = min(until, next_prof - 1)
+        status = lbx_equilibrate_advance(lbx_lattice, until, steps, reason)
+        call write_step_outputs(done + 1, steps)
+        done = steps
+        if (status /= LBX_OK .or. reason /= 0) exit
+        ! the state that opens step next_prof (equilibration.f90:154-176)
+        if (done == next_prof - 1) then
+            call lbx_check(lbx_flow(lbx_lattice, density, jx, jy, jz), "lbx_flow")
+            call write_profiles("# timestep", int(next_prof), .true.)
+            next_prof = next_prof + print_files_frequency
+        end if
     end do
     close (u13)
+    if (write_total_mass_flux) close (65)
     call lbx_check(status, "lbx_equilibrate")
 
     call lbx_check(lbx_get_state(lbx_lattice, n, density, jx, jy, jz), "lbx_get_state")
@@ -118,6 +125,45 @@ subroutine equilibration
     node%solventflux(z) = jz
 
 contains
+
+    ! the outputs of steps first..last of the run (equilibration.f90:149,
+    ! 257-261,339-344): the stdout line that opens step t (every
+    ! print_frequency steps, with the l2err of the step before), the
+    ! total_mass_flux.dat line (the flux sum of the step before, written once
+    ! step t has no negative population) and the l2err.dat line.  The FP32
+    ! storage forms measure l2err every 64 steps in stage 2: the other steps
+    ! (NaN in the trace) have no l2err.dat line.
+    subroutine write_step_outputs(first, last)
+        integer(c_int64_t), intent(in) :: first, last
+        integer(c_int64_t) :: held, fheld, t, n
+        real(dp) :: tr(max(last - first + 1, 1_c_int64_t)), js(3, max(last - first + 1, 1_c_int64_t))
+        if (last < first) return
+        ! l2err of steps first..min(last, held): a step that stopped on a
+        ! negative population has none
+        call lbx_check(lbx_l2err_trace(lbx_lattice, 1_c_int64_t, 0_c_int64_t, tr, held), "lbx_l2err_trace")
+        n = min(last, held) - first + 1
+        if (n > 0) call lbx_check(lbx_l2err_trace(lbx_lattice, first, n, tr, held), "lbx_l2err_trace")
+        ! flux sums of steps first-1..last-1 (the one of step 0 is 0)
+        js = 0
+        if (write_total_mass_flux .and. n > 0) then
+            call lbx_check(lbx_flux_sums(lbx_lattice, 1_c_int64_t, 0_c_int64_t, js, fheld), "lbx_flux_sums")
+            if (first == 1) then
+                if (n > 1) call lbx_check(lbx_flux_sums(lbx_lattice, 1_c_int64_t, n - 1, js(:, 2:), fheld), &
+                                          "lbx_flux_sums")
+            else
+                call lbx_check(lbx_flux_sums(lbx_lattice, first - 1, n, js, fheld), "lbx_flux_sums")
+            end if
+        end if
+        do t = first, last
+            if (modulo(t, int(print_frequency, c_int64_t)) == 0) &
+                print *, int(t), real(l2prev), "(target", real(target_error, 4), ")"
+            if (t > held) exit
+            if (write_total_mass_flux) write (65, *) int(t), real(js(:, t - first + 1))
+            if (ieee_is_nan(tr(t - first + 1))) cycle
+            write (u13, *) int(t), tr(t - first + 1)
+            l2prev = tr(t - first + 1)
+        end do
+    end subroutine write_step_outputs
 
     ! the six profile files: a header line, then per plane of each axis the
     ! mean density of its non-empty nodes and the summed flux components

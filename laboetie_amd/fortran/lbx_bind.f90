@@ -53,6 +53,39 @@ module lbx_bind
             integer(c_int32_t), intent(out) :: exit_reason
         end function
 
+        integer(c_int) function lbx_equilibrate_begin(ctx, target_error, fext, max_steps, flux_sums) &
+                bind(c, name='lbx_equilibrate_begin')
+            import :: c_int, c_ptr, c_double, c_int64_t, c_int32_t
+            type(c_ptr), value :: ctx
+            real(c_double), value :: target_error
+            real(c_double), intent(in) :: fext(3)
+            integer(c_int64_t), value :: max_steps
+            integer(c_int32_t), value :: flux_sums
+        end function
+
+        integer(c_int) function lbx_equilibrate_advance(ctx, until_step, steps_done, exit_reason) &
+                bind(c, name='lbx_equilibrate_advance')
+            import :: c_int, c_ptr, c_int64_t, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: until_step
+            integer(c_int64_t), intent(out) :: steps_done
+            integer(c_int32_t), intent(out) :: exit_reason
+        end function
+
+        integer(c_int) function lbx_flux_sums(ctx, first_step, count, out, available) bind(c, name='lbx_flux_sums')
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: first_step, count
+            real(c_double), intent(inout) :: out(*)
+            integer(c_int64_t), intent(out) :: available
+        end function
+
+        integer(c_int) function lbx_flow(ctx, rho, jx, jy, jz) bind(c, name='lbx_flow')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: rho(*), jx(*), jy(*), jz(*)
+        end function
+
         integer(c_int) function lbx_l2err_trace(ctx, first_step, count, out, available) &
                 bind(c, name='lbx_l2err_trace')
             import :: c_int, c_ptr, c_double, c_int64_t

@@ -6,6 +6,7 @@
 module moment_propagation
 
     use precision_kinds
+    use iso_c_binding, only: c_double, c_int64_t
     implicit none
 
     private
@@ -13,6 +14,17 @@ module moment_propagation
 
     real(dp), parameter :: eps = epsilon(1._dp)
     logical, save :: write_vacf = .true.
+
+    ! propagate() runs the propagation ahead in chunks of up to mp_chunk steps
+    ! (one lbx_mp_run each, stopping at the reference's convergence step) and
+    ! hands back one VACF per call from the buffer: VACF(buf_first + k - 1) is
+    ! buf(:, k), k = 1..buf_n; buf_conv: the last step held passed the
+    ! convergence test
+    integer, parameter :: mp_chunk = 256
+    real(c_double), save :: buf(3, mp_chunk)
+    integer(c_int64_t), save :: buf_first = 0, buf_n = 0
+    logical, save :: buf_conv = .false.
+    integer, save :: max_steps = 0  ! maximum_moment_propagation_steps (< 0: until converged)
 
 contains
 
@@ -59,6 +71,10 @@ contains
                        "lbx_mp_init_from_flow")
 
         print *, 0, real(vacf0, sp)
+        max_steps = getinput%int("maximum_moment_propagation_steps", 0)
+        buf_first = 0
+        buf_n = 0
+        buf_conv = .false.
         write_vacf = getinput%log("print_vacf", .true.)
         if (write_vacf) then
             open (99, file='output/vacf.dat')
@@ -69,21 +85,33 @@ contains
 
     ! one moment-propagation step (module_moment_propagation.f90:164-289):
     ! VACF(it) to stdout every 10000 steps and to output/vacf.dat, and the
-    ! convergence verdict of :284-288.
+    ! convergence verdict of :284-288.  The steps run on the device a chunk
+    ! at a time (drop_tracers calls this with it = 1, 2, ... until the verdict
+    ! or maximum_moment_propagation_steps); a chunk never runs past either.
     subroutine propagate(it, is_converged)
-        use iso_c_binding, only: c_double, c_int64_t, c_int32_t
+        use iso_c_binding, only: c_int32_t
         use lbx_bind
         implicit none
         integer(i2b), intent(in) :: it
         logical, intent(out) :: is_converged
         real(c_double) :: vacf(3)
-        integer(c_int64_t) :: done
+        integer(c_int64_t) :: n, k
         integer(c_int32_t) :: converged
 
-        call lbx_check(lbx_mp_run(lbx_lattice, 1_c_int64_t, 1_c_int32_t, vacf, done, converged), "lbx_mp_run")
+        if (it < buf_first .or. it >= buf_first + buf_n) then
+            n = mp_chunk
+            if (max_steps > 0) n = min(n, int(max_steps, c_int64_t) - it + 1)
+            n = max(n, 1_c_int64_t)
+            call lbx_check(lbx_mp_run(lbx_lattice, n, 1_c_int32_t, buf, buf_n, converged), "lbx_mp_run")
+            if (buf_n < 1) error stop "moment propagation called past its convergence"
+            buf_first = it
+            buf_conv = converged /= 0
+        end if
+        k = it - buf_first + 1
+        vacf = buf(:, k)
         if (modulo(it, 10000) == 0) print *, it, real(vacf, sp)
         if (write_vacf) write (99, *) it, vacf
-        is_converged = converged /= 0
+        is_converged = buf_conv .and. k == buf_n
     end subroutine propagate
 
     subroutine deallocate_propagated_quantity

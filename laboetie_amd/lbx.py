@@ -51,6 +51,10 @@ SIGNATURES = {
     "lbx_set_state": (ctypes.c_int, [_p] * 6),
     "lbx_equilibrate": (ctypes.c_int, [_p, _f64, _p, _i64, _p, _i64, _p, _p]),
     "lbx_get_state": (ctypes.c_int, [_p] * 6),
+    "lbx_equilibrate_begin": (ctypes.c_int, [_p, _f64, _p, _i64, _i32]),
+    "lbx_equilibrate_advance": (ctypes.c_int, [_p, _i64, _p, _p]),
+    "lbx_flux_sums": (ctypes.c_int, [_p, _i64, _i64, _p, _p]),
+    "lbx_flow": (ctypes.c_int, [_p] * 5),
     "lbx_l2err_trace": (ctypes.c_int, [_p, _i64, _i64, _p, _p]),
     "lbx_mp_init": (ctypes.c_int, [_p, _p, _f64, _f64, _f64, _p]),
     "lbx_mp_init_from_flow": (ctypes.c_int, [_p] * 6 + [_f64] * 3 + [_p]),
@@ -362,6 +366,36 @@ class Lattice:
         steps = sd.value
         nl2 = steps - 1 if st == ERR_NEG_POP else steps
         return dict(status=st, steps=steps, exit_reason=er.value, l2err=tr[:min(nl2, trace_cap)].copy())
+
+    def equilibrate_begin(self, fext, target_error=1e-10, max_steps=0, flux_sums=False):
+        """Start the equilibration loop; lbx_equilibrate_advance runs it on."""
+        fe = np.ascontiguousarray(fext, np.float64)
+        _check(lib().lbx_equilibrate_begin(self._h, float(target_error), _ptr(fe), int(max_steps),
+                                           1 if flux_sums else 0), "equilibrate_begin")
+
+    def equilibrate_advance(self, until_step=(1 << 63) - 1, check=True):
+        """Run the begun equilibration to step until_step or its end;
+        exit_reason 0 while it goes on."""
+        sd, er = _i64(), _i32()
+        st = lib().lbx_equilibrate_advance(self._h, int(until_step), ctypes.byref(sd), ctypes.byref(er))
+        if check and st != OK:
+            raise LbxError(st, "equilibrate_advance")
+        return dict(status=st, steps=sd.value, exit_reason=er.value)
+
+    def flux_sums(self):
+        """sum_r j(r, t) for every step taken, (steps, 3)."""
+        n = _i64()
+        _check(lib().lbx_flux_sums(self._h, 1, 0, None, ctypes.byref(n)), "flux_sums")
+        out = np.zeros(3 * max(n.value, 1))
+        if n.value:
+            _check(lib().lbx_flux_sums(self._h, 1, n.value, _ptr(out), None), "flux_sums")
+        return out[:3 * n.value].reshape(-1, 3)
+
+    def flow(self):
+        """(rho, j) after the last step taken (also during a paused run)."""
+        rho, jx, jy, jz = (np.zeros(self.N) for _ in range(4))
+        _check(lib().lbx_flow(self._h, _ptr(rho), _ptr(jx), _ptr(jy), _ptr(jz)), "flow")
+        return rho, np.stack([jx, jy, jz])
 
     def l2err_trace(self):
         """Every l2err of the last equilibrate call."""

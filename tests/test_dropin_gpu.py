@@ -127,3 +127,53 @@ def test_dropin_fp32_precision_keys(key):
         assert np.all((steps[3:] - 3) % 64 == 0) and np.all(np.diff(steps) > 0)
         rc_b, out_b = run_case(GPU_BIN, lb_in + "\ngpu_precision = fp16\n", str(g["geom_in"]), db)
         assert "gpu_precision must be fp64, fp32_shift or fp32_store" in out_b
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_BIN) and os.path.exists(GPU_BIN)),
+                    reason="drop-in or reference binary not built (needs the reference sources at build time)")
+@pytest.mark.parametrize("name", ["slit_conv_ads", "slit_conv"])
+def test_dropin_stops_where_the_reference_converges(name):
+    """maximum_moment_propagation_steps = -1: the drop-in's propagate()
+    reports convergence at the reference's step (module_moment_propagation.f90
+    :284-288), so both programs write the same number of vacf.dat lines."""
+    g = load(name)
+    print("drop-in convergence case", name, flush=True)
+    with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg:
+        rc_r, out_r = run_case(REF_BIN, str(g["lb_in"]), "", dr)
+        rc_g, out_g = run_case(GPU_BIN, str(g["lb_in"]), "", dg)
+        assert rc_r == 0, out_r
+        assert rc_g == 0, out_g
+        for f in TEXT_OUTPUTS:
+            assert open(os.path.join(dr, "output", f)).read() == open(os.path.join(dg, "output", f)).read(), f
+        vr = np.loadtxt(os.path.join(dr, "output", "vacf.dat"), comments="#")
+        vg = np.loadtxt(os.path.join(dg, "output", "vacf.dat"), comments="#")
+        assert vr.shape == vg.shape == (int(g["vacf_step"][-1]) + 1, 4)
+        assert np.array_equal(vr[:, 0], vg[:, 0])
+        assert np.all(np.abs(vr[:, 1:] - vg[:, 1:]) <= 1e-12 * np.abs(vr[0, 1:]).max())
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_BIN) and os.path.exists(GPU_BIN)),
+                    reason="drop-in or reference binary not built (needs the reference sources at build time)")
+@pytest.mark.parametrize("name", ["slit_4x4x34", "bcc12_ads"])
+def test_dropin_periodic_profiles_and_total_mass_flux(name):
+    """print_files_frequency = 50 and write_total_mass_flux = T: the profiles
+    at every 50th step (equilibration.f90:154-176) and output/
+    total_mass_flux.dat (:125-128,257-261) match the reference program's byte
+    for byte, as do the stdout step lines (print_frequency = 7)."""
+    g = load(name)
+    lb_in = str(g["lb_in"]).replace(
+        [ln for ln in str(g["lb_in"]).splitlines() if ln.startswith("maximum_moment_propagation_steps")][0],
+        "maximum_moment_propagation_steps = 20") + \
+        "print_files_frequency = 50\nwrite_total_mass_flux = T\nprint_frequency = 7\n"
+    print("drop-in outputs case", name, flush=True)
+    with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg:
+        rc_r, out_r = run_case(REF_BIN, lb_in, str(g["geom_in"]), dr)
+        rc_g, out_g = run_case(GPU_BIN, lb_in, str(g["geom_in"]), dg)
+        assert rc_r == 0, out_r
+        assert rc_g == 0, out_g
+        for f in TEXT_OUTPUTS + ["total_mass_flux.dat"]:
+            assert open(os.path.join(dr, "output", f)).read() == open(os.path.join(dg, "output", f)).read(), f
+        # several profile blocks were written
+        assert open(os.path.join(dg, "output", "mass-flux_profile_along_z.dat")).read().count("# timestep") > 2
+        step_lines = lambda out: [ln for ln in out.splitlines() if "(target" in ln]
+        assert step_lines(out_r) == step_lines(out_g) and len(step_lines(out_g)) > 3

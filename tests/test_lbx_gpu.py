@@ -270,3 +270,48 @@ def test_full_size_invariants_bcc128():
         P, Pa = L.mp_get()
         tot = (P + Pa).reshape(-1, 3).sum(0)
         assert np.allclose(tot, tot0, rtol=0, atol=1e-13 * np.abs(P0).sum())
+
+
+@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT])
+def test_equilibration_in_pieces_equals_one_run(prec):
+    """lbx_equilibrate_begin / _advance with pauses at steps 1, 2, 3, 4, 50,
+    the force switch and beyond give the one-shot run's bits (steps, exit,
+    l2err trace, state); lbx_flow at a pause is the oracle's state after
+    that many steps, and the flux sums are the sums of its flux."""
+    c = Case("bcc12_ads")
+    say("pieces", prec)
+    with lbx.Lattice(c.dims, c.nature, c.tau, precision=prec) as L:
+        L.init_rest(c.rho0)
+        one = L.equilibrate(c.fext, c.target)
+        n1, rho1, j1 = L.get_state()
+        L.init_rest(c.rho0)
+        L.equilibrate_begin(c.fext, c.target, flux_sums=True)
+        flows = {}
+        for stop in (1, 2, 3, 4, 50, 51, 333):
+            r = L.equilibrate_advance(stop)
+            if r["exit_reason"]:
+                break
+            assert r["steps"] == stop
+            flows[stop] = L.flow()
+            with pytest.raises(lbx.LbxError) as e:  # the state is extracted at the run's end only
+                L.get_state()
+            assert e.value.status == lbx.ERR_STATE
+        r = L.equilibrate_advance()
+        assert (r["steps"], r["exit_reason"]) == (one["steps"], one["exit_reason"])
+        assert np.array_equal(L.l2err_trace(), one["l2err"], equal_nan=True)
+        n2, rho2, j2 = L.get_state()
+        assert np.array_equal(n2, n1) and np.array_equal(rho2, rho1) and np.array_equal(j2, j1)
+        sums = L.flux_sums()
+        assert sums.shape == (one["steps"], 3)
+    # the flux sums are of the step's own flux: FP64, or FP32 in the
+    # FP32-shift form (within FP32 rounding of the FP64 flux extracted from
+    # its deviations)
+    rtol = 1e-12 if prec == lbx.PREC_FP64 else 1e-6
+    for stop, (rho, j) in flows.items():
+        if prec == lbx.PREC_FP64:
+            ref = lbref.equilibrate(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, max_steps=stop)
+        else:
+            ref = lbref.equilibrate_shift(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, max_steps=stop)
+        assert np.array_equal(rho, ref["rho"]) and np.array_equal(j, ref["j"]), stop
+        assert np.allclose(sums[stop - 1], ref["j"].sum(axis=1), rtol=rtol, atol=1e-15 * c.N)
+    assert np.allclose(sums[-1], j1.sum(axis=1), rtol=rtol, atol=1e-15 * c.N)
