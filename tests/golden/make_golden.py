@@ -62,6 +62,18 @@ FULL = {
                          geom_file=SHIPPED_GEOM),
 }
 
+# Moment propagation run until the reference's convergence test stops it
+# (maximum_moment_propagation_steps < 0, drop_tracers.f90:40; the test at
+# module_moment_propagation.f90:284-288): no-flow slits, whose VACF decays
+# below 1e-12 within a few thousand steps.
+CONV = {
+    "slit_conv_ads": dict(lb=dict(lx=4, ly=4, lz=8, geometryLabel=1, f_ext="0.0 0.0 0.0", target_error=1e-10,
+                                  maximum_moment_propagation_steps=-1, tracer_Db=0.1,
+                                  tracer_ka=0.02, tracer_kd=0.01)),
+    "slit_conv": dict(lb=dict(lx=5, ly=4, lz=7, geometryLabel=1, f_ext="0.0 0.0 0.0", target_error=1e-10,
+                              maximum_moment_propagation_steps=-1, tracer_Db=0.15)),
+}
+
 # Reference error stops (section "runtime assertions" of the reference).
 FAIL = {
     "fail_tau": dict(lx=3, ly=3, lz=6, geometryLabel=1, relaxation_time=0.4, f_ext="1e-5 0 0"),
@@ -207,7 +219,7 @@ def make_collide():
 
 def main(argv):
     want = set(argv[1:])
-    for name, spec in FULL.items():
+    for name, spec in list(FULL.items()) + list(CONV.items()):
         if not want or name in want:
             make_full(name, spec)
     if not want or "failures" in want:

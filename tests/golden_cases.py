@@ -7,6 +7,9 @@ from laboetie_amd.lbin import LbIn, read_geom_in
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FULL_CASES = ["slit_4x4x34", "bcc12_ads", "cyl9_tau08", "chromat_2disks", "random_3d", "shipped_geom"]
+# moment propagation until the reference's convergence test stops it
+# (maximum_moment_propagation_steps = -1)
+CONV_CASES = ["slit_conv_ads", "slit_conv"]
 
 
 def load(name):

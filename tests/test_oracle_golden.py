@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import lbref
-from golden_cases import FULL_CASES, Case, load
+from golden_cases import CONV_CASES, FULL_CASES, Case, load
 
 
 def test_tables_match_reference_dump():
@@ -95,6 +95,33 @@ def test_moment_propagation_matches_reference_bitwise(name):
     st, vacf = mp.run(c.mp_steps)
     assert st == 0
     assert np.array_equal(vacf, c.g["vacf"])
+
+
+@pytest.mark.parametrize("name", CONV_CASES)
+def test_moment_propagation_stops_where_the_reference_converges(name):
+    """maximum_moment_propagation_steps = -1: the restatement's convergence
+    test (module_moment_propagation.f90:284-288) first passes at the step
+    where the reference stopped, with the same VACF at every step."""
+    c = Case(name)
+    assert c.mp_steps < 0
+    r = lbref.equilibrate(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target)
+    assert r["steps"] == int(c.g["l2err_step"][-1])
+    assert np.array_equal(r["j"], c.j) and np.array_equal(r["rho"], c.g["rho"])
+    nt = lbref.tracer_population(c.dims, c.nature, c.g["rho"], c.j, c.fext)
+    mp = lbref.MP(c.dims, c.nature, c.interfacial, nt, c.g["rho"], c.Db, c.ka, c.kd)
+    vacf = [mp.vacf0]
+    while True:
+        st, v = mp.step()
+        assert st == 0
+        vacf.append(v)
+        if mp.converged(v):
+            break
+        assert mp.it < 10 * int(c.g["vacf_step"][-1]), "no convergence"
+    assert mp.it == int(c.g["vacf_step"][-1])
+    assert np.array_equal(np.array(vacf), c.g["vacf"])
+    # the test is the reference's: both limits, every component, and it > 2
+    last = np.abs(c.g["vacf"][-1])
+    assert np.all(last < 1e-12) and not np.all(np.abs(c.g["vacf"][-2]) < np.float32(1e-12))
 
 
 def test_failures_match_reference_error_stops():
