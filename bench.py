@@ -17,18 +17,21 @@ launch / measured mean launch duration, HIP events on the library's stream)
 and the reference program timed on this machine's host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-fp32-shift] [--no-large]
-                    [--no-batch]
+                    [--no-batch] [--no-serving]
 
 The same job also runs in the two FP32-storage precision forms (FP32-shift:
 FP32 collision; FP32-store: FP64 collision); their throughput, rooflines and
 D_eff error against the FP64 run go under "fp32_shift" and "fp32_store".  The
-128^3 job's working set (~0.25 GB) fits the 256 MB MALL, so "roofline_256"
-adds the two kernels' rooflines on the 256^3 random medium (3 GB of FP64
-populations), outside the timed region.  The other configurations run
-beside it, also outside the timed region: the 4x4x34 slit ("slit_4x4x34"),
-the 1024-instance batch ("batch_1024", FP64, FP32-shift and FP32-store), and, in
-"cpu_baseline", the reference program on a bounded sample of each of them
-and of two serving requests.
+128^3 job's LB working set (~0.24 GB) fits the 256 MB MALL, so "roofline_lb"
+is read on the 256^3 random medium (3 GB of FP64 populations), whose
+shortened precision run ("precision_256": the error of every output of the
+FP32 forms against FP64; "roofline_256": both kernels in all three forms)
+runs outside the timed region.  The other configurations run beside it,
+also outside the timed region: the 4x4x34 slit ("slit_4x4x34"), the
+1024-instance batch ("batch_1024", FP64, FP32-shift and FP32-store), a
+bounded serving stream ("serving": p50/p99 latency and MLUPS at 0.5x, 1x and
+2x of saturation, FP32-shift), and, in "cpu_baseline", the reference program
+on a bounded sample of the job and of each of them.
 """
 import argparse
 import json
@@ -126,52 +129,80 @@ def fp32_leg(w, nat, args, nf, nfi, prec=lbx.PREC_FP32_SHIFT):
 
 
 LARGE = dict(dims=(256, 256, 256), sigma=8.0, porosity=0.6, seed=20261202, tau=0.8, fext=(1e-6, 0.0, 0.0),
-             Db=0.01, kd=0.01, dF=-1.0, lb_steps=300, mp_steps=300)
+             Db=0.01, kd=0.01, dF=-1.0, lb_steps=2000, mp_steps=1000)
 
 
 def large_leg():
-    """The kernels on a working set far above the 256 MB MALL: the 256^3
-    Gaussian random medium (correlation length 8, porosity 0.6), tau 0.8,
-    F = 1e-6 along x; LARGE["lb_steps"] LB steps from rest (the force on
-    from t = 4) and LARGE["mp_steps"] MP steps of one tracer (dF = -1 kT),
-    FP64, FP32-shift and FP32-store, kernel times from HIP events."""
+    """The precision configuration, shortened: the 256^3 Gaussian random
+    medium (correlation length 8, porosity 0.6), tau 0.8, F = 1e-6 along x;
+    LARGE["lb_steps"] LB steps from rest (the force on from t = 4) and
+    LARGE["mp_steps"] MP steps of one tracer (dF = -1 kT), in FP64,
+    FP32-shift and FP32-store (laboetie_amd.precision.error_table): the error
+    of every output of the FP32 forms against FP64, and the two kernels'
+    rooflines on a working set far above the 256 MB MALL (kernel times from
+    HIP events)."""
+    from laboetie_amd import precision
+
     g = LARGE
     nat = lbx.geometry_random(*g["dims"], g["sigma"], g["porosity"], g["seed"])
     lx, ly, lz = g["dims"]
     itf = lbx.detect_interfacial(lx, ly, lz, nat)
     nfi = int(((nat == 0) & (itf == 1)).sum())
-    ka = lbx.ka_kd_from_dF(g["dF"], g["kd"])[0]
-    out = {"workload": f"Gaussian random medium 256^3 (sigma 8, porosity 0.6, seed {g['seed']}), tau 0.8, "
-                       f"F=(1e-6,0,0): {g['lb_steps']} LB steps from rest, {g['mp_steps']} MP steps of one "
-                       "tracer (dF=-1 kT)"}
-    for prec, name, lbb, coef in ((lbx.PREC_FP64, "fp64", LB_BYTES_PER_FLUID, MP_COEF_BYTES_PER_FLUID),
-                                  (lbx.PREC_FP32_SHIFT, "fp32_shift", LB32_BYTES_PER_FLUID,
-                                   MP32_COEF_BYTES_PER_FLUID),
-                                  (lbx.PREC_FP32_STORE, "fp32_store", LB32_BYTES_PER_FLUID,
-                                   MP32_COEF_BYTES_PER_FLUID)):
-        with lbx.Lattice(g["dims"], nat, g["tau"], precision=prec) as L:
-            if prec != lbx.PREC_FP64:
-                L.set_floor_window(0)
-            nf = L.fluid_nodes
-            L.init_rest(1.0)
-            L.timing(True)
-            L.equilibrate(g["fext"], 0.0, max_steps=g["lb_steps"], trace_cap=1)
-            L.mp_init(g["fext"], g["Db"], ka, g["kd"])
-            L.mp_run(g["mp_steps"])
-            L.sync()
-            lb_ms, lb_n = L.kernel_time(lbx.KERNEL_LB)
-            mp_ms, mp_n = L.kernel_time(lbx.KERNEL_MP)
-        lb_s, mp_s = lb_ms / lb_n / 1e3, mp_ms / mp_n / 1e3
+    tab = precision.error_table(lx, g["lb_steps"], g["mp_steps"], g["seed"], nat=nat, keep_runs=True)
+    runs = tab.pop("_runs")
+    out = {"workload": tab["workload"],
+           "error_table": {k: v for k, v in tab.items() if k.startswith(("errors_vs", "deff", "permeability"))}}
+    for name, lbb, coef in (("fp64", LB_BYTES_PER_FLUID, MP_COEF_BYTES_PER_FLUID),
+                            ("fp32_shift", LB32_BYTES_PER_FLUID, MP32_COEF_BYTES_PER_FLUID),
+                            ("fp32_store", LB32_BYTES_PER_FLUID, MP32_COEF_BYTES_PER_FLUID)):
+        r = runs[name]
+        nf = r["fluid"]
+        lb_s, mp_s = r["lb_kernel_us"] / 1e6, r["mp_kernel_us"] / 1e6
         mp_bytes = (coef + MP_P_BYTES_PER_FLUID) * nf + MP_BYTES_PER_ADS_INTERFACIAL * nfi
         out[name] = {
             "fluid_nodes": nf,
-            "roofline_lb": {"achieved": round(lbb * nf / lb_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(lbb * nf / lb_s / 1e9 / HBM_PEAK_GBS, 4), "avg_launch_us": round(lb_s * 1e6, 1)},
-            "roofline_mp": {"achieved": round(mp_bytes / mp_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(mp_bytes / mp_s / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_launch": round(mp_bytes),
+            "roofline_lb": {"bound": "hbm", "achieved": round(lbb * nf / lb_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(lbb * nf / lb_s / 1e9 / HBM_PEAK_GBS, 4),
+                            "kernel": "k_lb_step", "bytes_per_launch": lbb * nf, "avg_launch_us": round(lb_s * 1e6, 1)},
+            "roofline_mp": {"bound": "hbm", "achieved": round(mp_bytes / mp_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(mp_bytes / mp_s / 1e9 / HBM_PEAK_GBS, 4),
+                            "kernel": "k_mp_step", "bytes_per_launch": round(mp_bytes),
                             "avg_launch_us": round(mp_s * 1e6, 1)},
             "lb_mlups": round(lx * ly * lz / lb_s / 1e6, 1), "mp_mlups": round(lx * ly * lz / mp_s / 1e6, 1)}
     print("bench: 256^3 leg done", file=sys.stderr, flush=True)
+    return out
+
+
+def serving_leg(n_requests=16, seed=20261202):
+    """The serving configuration, bounded: n_requests requests per rate
+    (N^3 with N log-uniform in [32, 192]; slit, cylinder along z, BCC spheres
+    or Gaussian random medium of porosity U[0.4, 0.8]; F log-uniform in
+    [1e-7, 1e-5]; target 1e-9, LB capped at 10000 steps; 2000 adsorbing MP
+    steps, dF U[-3, 0] kT) in the FP32-shift form, continuous batching with
+    HBM-budget admission.  Saturation throughput from a closed batch of the
+    first half of the requests; then Poisson arrivals at 0.5x, 1x and 2x of
+    it, latency from arrival to result."""
+    from laboetie_amd import serving
+
+    rng = np.random.default_rng(seed)
+    reqs = [serving.serving_request(rng) for _ in range(n_requests)]
+    prec = lbx.PREC_FP32_SHIFT
+    cap_nodes, cap_box = 48 << 20, 64 << 20
+    sat = serving.serve(reqs[:n_requests // 2], np.zeros(n_requests // 2), prec, cap_nodes, cap_box)
+    rate0 = (n_requests // 2) / sat["wall_s"]
+    out = {"workload": f"{n_requests} requests per rate: N^3 (N log-uniform [32,192]), slit/cylinder/BCC/random, "
+                       "F log-uniform [1e-7,1e-5], target 1e-9 (cap 10000), 2000 MP steps, dF U[-3,0], FP32-shift, "
+                       "48 M-node HBM budget",
+           "saturation_req_per_s": round(rate0, 3), "saturation_mlups": round(sat["mlups"], 1)}
+    for f in (0.5, 1.0, 2.0):
+        rate = f * rate0
+        arr = np.cumsum(np.random.default_rng(seed + int(f * 1000)).exponential(1.0 / rate, n_requests))
+        s = serving.serve(reqs, arr, prec, cap_nodes, cap_box)
+        lat = s["latency"][np.isfinite(s["latency"])]
+        out[f"rate_{f:g}x"] = {"req_per_s": round(rate, 3), "p50_s": round(float(np.percentile(lat, 50)), 3),
+                               "p99_s": round(float(np.percentile(lat, 99)), 3), "mlups": round(s["mlups"], 1),
+                               "wall_s": round(s["wall_s"], 2), "failed": int((s["status"] != 0).sum())}
+        print(f"bench: serving rate {f:g}x done", file=sys.stderr, flush=True)
     return out
 
 
@@ -294,17 +325,18 @@ def _run_on_pty(exe, d, threads, stop, max_wait):
     return lines
 
 
-def _lb_steps_per_second(exe, d, threads, window):
+def _lb_steps_per_second(exe, d, threads, window, min_steps=0):
     """The equilibration prints `t l2err (target ...)` every step with
-    print_frequency = 1 (equilibration.f90:149); time those lines."""
+    print_frequency = 1 (equilibration.f90:149); time those lines over at
+    least `window` seconds and `min_steps` steps."""
     def steps(lines):
         return [(t, int(l.split()[0])) for t, l in lines if l.endswith(")") and "(target" in l]
 
     def stop(lines):
         s = steps(lines)
-        return len(s) >= 2 and s[-1][0] - s[0][0] >= window
+        return len(s) >= 2 and s[-1][0] - s[0][0] >= window and s[-1][1] - s[0][1] >= min_steps
 
-    s = steps(_run_on_pty(exe, d, threads, stop, max_wait=300))
+    s = steps(_run_on_pty(exe, d, threads, stop, max_wait=600))
     if len(s) < 2 or s[-1][1] == s[0][1]:
         return None
     return (s[-1][1] - s[0][1]) / (s[-1][0] - s[0][0])
@@ -399,24 +431,38 @@ def cpu_configs(exe, threads):
     return out
 
 
-def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0, mp_sample=30, other_configs=True):
+def host_threads():
+    """The host cores this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS where the environment sets it (one GPU's share of a
+    shared host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(w, lb_steps, mp_steps_total, window=5.0, lb_sample=200, mp_sample=200, other_configs=True):
     """The reference program (oracle/_ref/laboetie_fast: flang -O3 -fopenmp
     -funroll-loops, the reference Makefile's flags) on this machine's host
-    cores, timed on a bounded sample of the same workload: the LB step rate
-    from its per-step stdout lines over `window` seconds of the 128^3 BCC
-    run, the MP step rate from a run of mp_sample steps of the dF = -1 tracer
-    on a flow at rest (f_ext = 0, so equilibration ends at t = 4; the MP cost
-    does not depend on the flow values).  Whole-job MLUPS =
-    N (LB + MP steps) / (LB/rate + MP/rate)."""
+    cores (host_threads()), timed on a bounded sample of the same workload:
+    the LB step rate from its per-step stdout lines over at least lb_sample
+    steps of the 128^3 BCC run, the MP step rate from a run of mp_sample
+    steps of the dF = -1 tracer on a flow at rest (f_ext = 0, so
+    equilibration ends at t = 4; the MP cost does not depend on the flow
+    values).  Whole-job MLUPS = N (LB + MP steps) / (LB/rate + MP/rate)."""
     exe = os.path.join(HERE, "oracle", "_ref", "laboetie_fast")
     if not os.path.exists(exe):
         return None
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     N = int(np.prod(w["dims"]))
     print("cpu_baseline: timing the reference LB phase", file=sys.stderr, flush=True)
     with tempfile.TemporaryDirectory() as d:
         open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, w["fext"], 0, 0.0))
-        lb_rate = _lb_steps_per_second(exe, d, threads, window)
+        lb_rate = _lb_steps_per_second(exe, d, threads, window, lb_sample)
     print(f"cpu_baseline: LB {lb_rate} steps/s; timing the reference MP phase", file=sys.stderr, flush=True)
     with tempfile.TemporaryDirectory() as d:
         ka = w["kd"] * np.exp(1.0)
@@ -429,14 +475,14 @@ def cpu_baseline(w, lb_steps, mp_steps_total, window=12.0, mp_sample=30, other_c
     print("cpu_baseline: the reference's LB phase on one core", file=sys.stderr, flush=True)
     with tempfile.TemporaryDirectory() as d:
         open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, w["fext"], 0, 0.0))
-        lb_rate1 = _lb_steps_per_second(exe, d, 1, window)
+        lb_rate1 = _lb_steps_per_second(exe, d, 1, 12.0)
     configs = cpu_configs(exe, threads) if other_configs else None
     return {"value": round(N * (lb_steps + mp_steps_total) / t_job / 1e6, 4), "unit": "MLUPS",
             "cores": threads, "kind": "reference",
-            "sample": (f"reference laboetie (flang -O3 -fopenmp -funroll-loops), OMP_NUM_THREADS={threads}: "
-                       f"LB {lb_rate:.3f} steps/s over {window:.0f} s of the 128^3 BCC run, MP "
-                       f"{mp_rate:.3f} steps/s over {mp_sample} steps (dF=-1 tracer), combined over this "
-                       f"job's {lb_steps} LB + {mp_steps_total} MP steps"),
+            "sample": (f"reference laboetie (flang -O3 -fopenmp -funroll-loops), OMP_NUM_THREADS={threads} (the "
+                       f"host cores this process may use): LB {lb_rate:.3f} steps/s over >= {lb_sample} steps of "
+                       f"the 128^3 BCC run, MP {mp_rate:.3f} steps/s over {mp_sample} steps (dF=-1 tracer), "
+                       f"combined over this job's {lb_steps} LB + {mp_steps_total} MP steps"),
             "lb_mlups": round(N * lb_rate / 1e6, 4), "mp_mlups": round(N * mp_rate / 1e6, 4),
             "lb_mlups_1_core": round(N * lb_rate1 / 1e6, 4) if lb_rate1 else None,
             "other_configs": configs}
@@ -453,6 +499,7 @@ def main():
                     help="propagate the dF tracers one after the other instead of together")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 roofline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 1024-instance batch leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving (Poisson stream) leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain-copy bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -491,6 +538,7 @@ def main():
     large = None if args.no_large else large_leg()
     slit = slit_leg()
     batch = None if args.no_batch else batch_leg()
+    served = None if args.no_serving else serving_leg()
     copy_gbs = lbx.copy_bandwidth(2 << 30, 20)  # measured ceiling, outside the timed region
     lb_steps, mp_steps, deffs = results[-1]
     updates = N * sum(r[0] + r[1] for r in results)
@@ -534,7 +582,11 @@ def main():
                    "lattice": [lx, ly, lz], "fluid_nodes": nf, "lb_steps": lb_steps, "mp_steps": mp_steps,
                    "precision": "fp64"},
         "roofline": roof_lb if dominant == "lb_step" else roof_mp,
-        "roofline_lb": roof_lb, "roofline_mp": roof_mp,
+        # the LB kernel's roofline is read on the 256^3 medium: the 128^3 job's LB
+        # working set (~0.24 GB) sits in the 256 MB MALL
+        "roofline_lb": (dict(large["fp64"]["roofline_lb"], workload="256^3 random medium (FP64)")
+                        if large is not None else None),
+        "roofline_lb_bcc128_mall_resident": roof_lb, "roofline_mp": roof_mp,
         "copy_ceiling_gbs": round(copy_gbs, 1),
         "lb_mlups": round(N * lb_steps * args.steps / (lb_ms / 1e3) / 1e6, 1),
         "mp_mlups": round(N * mp_steps * args.steps / (mp_ms / 1e3) / 1e6, 1),
@@ -553,7 +605,10 @@ def main():
     if batch is not None:
         line["batch_1024"] = batch
     if large is not None:
+        line["precision_256"] = large.pop("error_table")
         line["roofline_256"] = large
+    if served is not None:
+        line["serving"] = served
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, lb_steps, mp_steps)
     print(json.dumps(line), flush=True)

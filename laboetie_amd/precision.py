@@ -70,24 +70,22 @@ def rel(a, b):
     return float(np.abs(a - b).max()) / s if s > 0 else float(np.abs(a - b).max())
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--flow-steps", type=int, default=10000)
-    ap.add_argument("--mp-steps", type=int, default=5000)
-    ap.add_argument("--seed", type=int, default=20261202)
-    args = ap.parse_args(argv)
-    L_ = args.size
+def error_table(size=256, flow_steps=10000, mp_steps=5000, seed=20261202, nat=None, keep_runs=False):
+    """Run the three forms on the Gaussian random medium of `size`^3 and
+    return the report (the JSON object main prints); with keep_runs, also
+    the per-form results under "_runs" (fluid node counts, kernel times)."""
+    L_ = size
     dims = (L_, L_, L_)
     tau, fext, Db, kd = 0.8, (1e-6, 0.0, 0.0), 0.01, lbx.KD_DEFAULT
     ka = lbx.ka_kd_from_dF(-1.0, kd)[0]
     t0 = time.perf_counter()
-    nat = lbx.geometry_random(*dims, 8.0, 0.6, args.seed)
+    if nat is None:
+        nat = lbx.geometry_random(*dims, 8.0, 0.6, seed)
     print(f"precision: medium {dims} built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     res = {}
     forms = (("fp64", lbx.PREC_FP64), ("fp32_shift", lbx.PREC_FP32_SHIFT), ("fp32_store", lbx.PREC_FP32_STORE))
     for name, prec in forms:
-        res[name] = run(prec, dims, nat, tau, fext, args.flow_steps, args.mp_steps, Db, ka, kd)
+        res[name] = run(prec, dims, nat, tau, fext, flow_steps, mp_steps, Db, ka, kd)
         print(f"precision: {name} done", file=sys.stderr, flush=True)
     a = res["fp64"]
     fluid = nat == 0
@@ -108,8 +106,8 @@ def main(argv=None):
         }
 
     out = {
-        "workload": f"Gaussian random medium {L_}^3 (sigma 8, porosity 0.6, seed {args.seed}), tau 0.8, "
-                    f"F=(1e-6,0,0), {args.flow_steps} flow + {args.mp_steps} MP steps, dF=-1 kT",
+        "workload": f"Gaussian random medium {L_}^3 (sigma 8, porosity 0.6, seed {seed}), tau 0.8, "
+                    f"F=(1e-6,0,0), {flow_steps} flow + {mp_steps} MP steps, dF=-1 kT",
         "fluid_nodes": a["fluid"],
         "error_scale": "max|x32 - x64| / max|x64| per output (D_eff: per component, relative to max |D|)",
         "deff_fp64": a["deff"].tolist(), "permeability_fp64": perm(a),
@@ -125,6 +123,20 @@ def main(argv=None):
                                                     "lb_roofline")}
     for name in ("fp32_shift", "fp32_store"):
         out[f"lb_speedup_{name}"] = round(res[name]["lb_mlups"] / a["lb_mlups"], 3)
+    if keep_runs:
+        out["_runs"] = {name: {k: res[name][k] for k in ("fluid", "lb_kernel_us", "mp_kernel_us", "lb_mlups",
+                                                         "mp_mlups")} for name, _ in forms}
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--flow-steps", type=int, default=10000)
+    ap.add_argument("--mp-steps", type=int, default=5000)
+    ap.add_argument("--seed", type=int, default=20261202)
+    args = ap.parse_args(argv)
+    out = error_table(args.size, args.flow_steps, args.mp_steps, args.seed)
     print(json.dumps(out), flush=True)
 
 

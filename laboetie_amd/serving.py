@@ -86,11 +86,15 @@ def _add(B, r):
 
 
 # ------------------------------------------------------------------ batch
-def run_batch(reqs, precision, lb_round=128, mp_round=128, autotune=True):
+def run_batch(reqs, precision, lb_round=128, mp_round=128, autotune=True, launch=None):
     """All requests admitted at once and advanced together; returns per
-    request D_eff, LB steps and exit reason, plus throughput."""
+    request D_eff, LB steps and exit reason, plus throughput.  launch (one of
+    lbx.LAUNCH_*) fixes the launch mode and turns the autotuner off."""
     cap, box = capacity_for(reqs)
     with lbx.Batch(cap, box, len(reqs), precision=precision) as B:
+        if launch is not None:
+            B.set_launch_mode(launch)
+            autotune = False
         ids = [_add(B, r) for r in reqs]
         assert None not in ids, "capacity"
         tuned = []
@@ -219,7 +223,11 @@ def main(argv=None):
                          "first served)")
     ap.add_argument("--capacity-nodes", type=int, default=48 << 20, help="serve: fluid-node budget of the batch")
     ap.add_argument("--capacity-box", type=int, default=64 << 20, help="serve: lattice-node budget of the batch")
+    ap.add_argument("--launch", choices=["auto", "graph", "stream", "persistent"], default="auto",
+                    help="batch: launch mode (auto: the autotuner picks)")
     args = ap.parse_args(argv)
+    launch = {"auto": None, "graph": lbx.LAUNCH_GRAPH, "stream": lbx.LAUNCH_STREAM,
+              "persistent": lbx.LAUNCH_PERSISTENT}[args.launch]
     if args.mode == "batch":
         reqs = chromatography_requests(args.instances, args.seed)
         out = {"workload": f"{args.instances} chromatography 2D instances (1x50x50 / 1x100x100, 1-8 disks r=5), "
@@ -228,10 +236,11 @@ def main(argv=None):
             args.precision, [args.precision])
         res = {}
         for p in precs:
-            r = run_batch(reqs, _prec(p))
+            r = run_batch(reqs, _prec(p), launch=launch)
             res[p] = r
             out[p] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in r.items()
-                      if k in ("wall_s", "rounds", "tuned", "mlups", "mflups", "lb_kernel_mlups", "mp_kernel_mlups")}
+                      if k in ("wall_s", "rounds", "tuned", "mlups", "mflups", "lb_kernel_mlups", "mp_kernel_mlups",
+                               "stats")}
             print(f"serving: batch {p} done in {r['wall_s']:.1f} s", file=sys.stderr, flush=True)
         for p in ("fp32_shift", "fp32_store"):
             if p in res and "fp64" in res:
