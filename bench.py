@@ -383,6 +383,40 @@ def _ref_rates(exe, threads, w, nat, lb_window, mp_sample):
     return lb_rate, mp_rate
 
 
+def dropin_leg(w, mp_steps=2000):
+    """The drop-in binary (laboetie_amd/fortran: the reference program with
+    its hot-path modules replaced by the lbx shims) on the bench lattice with
+    one tracer (dF = -1 kT): its moment-propagation rate, timed like the
+    reference's (from the `0 VACF(0)` line to the closing status line, on a
+    pseudo-terminal), beside lbx_mp_run's rate on the same lattice and
+    tracer in this process.  The drop-in writes vacf.dat every step, as the
+    reference does."""
+    exe = os.path.join(HERE, "laboetie_amd", "fortran", "_build", "laboetie_gpu")
+    if not os.path.exists(exe):
+        return None
+    ka = w["kd"] * float(np.exp(1.0))
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "lb.in"), "w").write(lb_in_text(w, w["fext"], mp_steps, ka, print_frequency=1000))
+        rate = _mp_steps_per_second(exe, d, 1, mp_steps)
+    lx, ly, lz = w["dims"]
+    nat = lbx.build_geometry(w["label"], lx, ly, lz)
+    with lbx.Lattice(w["dims"], nat, w["tau"]) as L:
+        L.init_rest(w["rho0"])
+        L.equilibrate(w["fext"], w["target"], trace_cap=1)
+        L.mp_init(w["fext"], w["Db"], ka, w["kd"])
+        L.mp_run(50)
+        L.sync()
+        t0 = time.perf_counter()
+        L.mp_run(mp_steps)
+        L.sync()
+        lib_rate = mp_steps / (time.perf_counter() - t0)
+    print("bench: drop-in leg done", file=sys.stderr, flush=True)
+    return {"workload": f"BCC 128^3, one tracer (dF=-1 kT), {mp_steps} MP steps",
+            "dropin_mp_steps_per_s": round(rate, 1) if rate else None,
+            "lbx_mp_run_steps_per_s": round(lib_rate, 1),
+            "dropin_vs_lbx_mp_run": round(rate / lib_rate, 3) if rate else None}
+
+
 def cpu_configs(exe, threads):
     """The reference on a bounded sample of each other configuration, on the
     same host cores: node-update rates of its LB and MP phases (lattice nodes
@@ -539,6 +573,7 @@ def main():
     slit = slit_leg()
     batch = None if args.no_batch else batch_leg()
     served = None if args.no_serving else serving_leg()
+    dropin = dropin_leg(w)
     copy_gbs = lbx.copy_bandwidth(2 << 30, 20)  # measured ceiling, outside the timed region
     lb_steps, mp_steps, deffs = results[-1]
     updates = N * sum(r[0] + r[1] for r in results)
@@ -609,6 +644,8 @@ def main():
         line["roofline_256"] = large
     if served is not None:
         line["serving"] = served
+    if dropin is not None:
+        line["dropin_mp"] = dropin
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, lb_steps, mp_steps)
     print(json.dumps(line), flush=True)

@@ -464,11 +464,14 @@ void launch_mp_step(lbx_ctx* c, int now, int64_t it, int64_t k, int32_t stop_on_
     a.Px = c->P[now ^ 1];
     a.Pan = c->any_ads ? c->Pa[now] : nullptr;
     a.Pax = c->any_ads ? c->Pa[now ^ 1] : nullptr;
-    // partial buffers alternate by step; without a convergence test the
-    // previous step's VACF is reduced by block 0 of this step's kernel
+    // partial buffers alternate by step; the previous step's VACF (and its
+    // convergence test) is reduced by block 0 of this step's kernel
     a.partials = c->part + (uint64_t)nv * gt * (it & 1);
-    a.prev_partials = (!stop_on_convergence && k > 0) ? c->part + (uint64_t)nv * gt * ((it - 1) & 1) : nullptr;
+    a.prev_partials = k > 0 ? c->part + (uint64_t)nv * gt * ((it - 1) & 1) : nullptr;
     a.prev_vtrace = c->vtrace + nv * (k - 1);
+    a.check_conv = stop_on_convergence;
+    a.lim1 = lim1;
+    a.lim2 = lim2;
     a.it = it;
     a.ctrl = c->ctrl;
     a.dlx = c->dlx;
@@ -483,9 +486,6 @@ void launch_mp_step(lbx_ctx* c, int now, int64_t it, int64_t k, int32_t stop_on_
         constexpr int T = TT;
         auto kern = c->wide ? k_mp_step<true, T, W> : k_mp_step<false, T, W>;
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
-        if (stop_on_convergence)
-            hipLaunchKernelGGL(k_mp_reduce<3 * T>, dim3(1), dim3(kBlock), 0, c->stream, a.partials, gt,
-                               c->vtrace + nv * k, c->ctrl, 1, it, 1, lim1, lim2);
     });
 }
 
@@ -1026,13 +1026,13 @@ int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* 
                 launch_mp_step<double>(c, now, it, k, stop_on_convergence, lim1, lim2);
         }
         timer_stop(c);
-        if (!stop_on_convergence) {  // the last step's VACF
+        {  // the last step's VACF (and convergence test)
             const int64_t it = c->mp_it + S;
             with_tracers(c->ntr, [&](auto TT) {
                 constexpr int T = TT;
                 hipLaunchKernelGGL(k_mp_reduce<3 * T>, dim3(1), dim3(kBlock), 0, c->stream,
                                    c->part + (uint64_t)nv * gt * (it & 1), gt, c->vtrace + nv * (S - 1), c->ctrl, 1,
-                                   it, 0, lim1, lim2);
+                                   it, stop_on_convergence, lim1, lim2);
             });
         }
         LBX_TRY(hipGetLastError());

@@ -829,6 +829,8 @@ struct MpStepArgs {
     double* __restrict__ Pax;
     double* partials;                // this step's per-block VACF partials
     const double* prev_partials;     // the previous step's, reduced by block 0 (NULL: none)
+    int32_t check_conv = 0;          // block 0 also applies the convergence test to VACF(it-1)
+    double lim1 = 0.0, lim2 = 0.0;   // its limits (module_moment_propagation.f90:284)
     double* prev_vtrace;             // where VACF(it-1) goes
     int64_t it;
     Ctrl* ctrl;
@@ -863,8 +865,13 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     // neighbour indices and the node's coefficients with the LDS window.
     const uint32_t e = valid ? ld(a.nodes, i) : 0u;
     if (block_flag(vload(&a.ctrl->mp_stop) != 0)) return;
+    // VACF(it-1) from the previous step's partials and, with check_conv, the
+    // convergence test: a stop set here ends the run at step it-1 (this
+    // launch's other blocks may have started step it already; they write only
+    // the other buffer of P, and every later launch returns at once)
     if (a.prev_partials && first_block)
-        mp_reduce_block<K, BLOCK>(a.prev_partials, nparts, a.prev_vtrace, a.ctrl, 1, a.it - 1, 0, 0.0, 0.0, lds);
+        mp_reduce_block<K, BLOCK>(a.prev_partials, nparts, a.prev_vtrace, a.ctrl, 1, a.it - 1, a.check_conv, a.lim1,
+                                  a.lim2, lds);
     const uint32_t Np = (uint32_t)a.Np;
     const View<Wide> Pn(a.Pn, K * a.Np);
     const View<Wide, W> win(a.win, 18 * a.Np), us(a.ustar, 3 * a.Np);
