@@ -248,8 +248,19 @@ def batch_leg(n=1024):
         deffs[name] = np.array([x["deff"] for x in r["results"]])
         if conv is None:  # instances whose FP64 run converged (the others stop at the step cap)
             conv = np.array([x["exit_reason"] == lbx.EXIT_CONVERGED for x in r["results"]])
+            steps64 = [x["lb_steps"] for x in r["results"]]
         out[name] = {"mlups": round(r["mlups"], 1), "mflups": round(r["mflups"], 1), "wall_s": round(r["wall_s"], 2),
                      "lb_kernel_mlups": round(r["lb_kernel_mlups"], 1), "mp_kernel_mlups": round(r["mp_kernel_mlups"], 1)}
+    # precision: each instance run in the FP32 forms for exactly the LB steps its FP64 run took
+    # (no stop test), so that the two differ by their arithmetic only and not by where each
+    # form's stop rule ends a still-settling flow
+    for prec, name in ((lbx.PREC_FP32_SHIFT, "fp32_shift"), (lbx.PREC_FP32_STORE, "fp32_store")):
+        r = serving.run_batch(reqs, prec, lb_steps=steps64)
+        ds = np.array([x["deff"] for x in r["results"]])
+        err = np.abs(ds - deffs["fp64"]).max(axis=1) / np.abs(deffs["fp64"]).max(axis=1)
+        out[f"deff_rel_error_{name}_same_lb_steps"] = {
+            "median": float(f"{np.median(err):.3e}"), "max": float(f"{err.max():.3e}"),
+            "count_above_1e-5": int((err > 1e-5).sum())}
     for name in ("fp32_shift", "fp32_store"):
         err = np.abs(deffs[name] - deffs["fp64"]).max(axis=1) / np.abs(deffs["fp64"]).max(axis=1)
         out[f"deff_rel_error_{name}"] = {"median": float(f"{np.median(err):.3e}"), "max": float(f"{err.max():.3e}"),

@@ -335,6 +335,18 @@ int lbx_set_check_interval(lbx_ctx* ctx, int32_t steps);
 int lbx_set_addressing(lbx_ctx* ctx, int32_t wide);
 int lbx_addressing(lbx_ctx* ctx);
 
+/* How the moment-propagation step gets the incoming scattering probabilities
+ * s(r+c_l, inv l) (module_moment_propagation.f90:218-229): LBX_MP_RECOMPUTED
+ * keeps each node's density and flow (32 bytes) and recomputes them in the
+ * step with the set-up's own arithmetic; LBX_MP_STORED keeps the 18
+ * probabilities of each node (144 bytes in FP64, 72 in the FP32 forms).
+ * Both give the same bits.  Default: recomputed in the FP64 form, stored in
+ * the FP32 forms (where the recomputation costs more than the 72 bytes).  Takes effect at the next lbx_mp_init*.  No
+ * counterpart in the reference. */
+#define LBX_MP_STORED     0
+#define LBX_MP_RECOMPUTED 1
+int lbx_set_mp_coefficients(lbx_ctx* ctx, int32_t form);
+
 /* Fluid-node count of the context (the MFLUPS denominator). */
 int64_t lbx_fluid_nodes(lbx_ctx* ctx);
 
@@ -403,6 +415,9 @@ int lbxb_set_block(lbxb* b, int32_t lb_block, int32_t mp_block);
 /* lbx_set_check_interval for every instance of a batch (set before the
  * instances run; default 64 in the FP32-storage forms). */
 int lbxb_set_check_interval(lbxb* b, int32_t steps);
+/* lbx_set_floor_window for the instances added to the batch from now on
+ * (default 1000 in the FP32-storage forms; FP64 batches accept only 0). */
+int lbxb_set_floor_window(lbxb* b, int64_t steps);
 /* How a round's steps are launched: one launch per step on the stream, the
  * same launches replayed from a captured HIP graph (default), or one
  * persistent cooperative launch per phase that loops over the steps with a

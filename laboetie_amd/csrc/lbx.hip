@@ -98,6 +98,11 @@ struct lbx_ctx {
     double* ustar = nullptr;
     float* win32 = nullptr;      // FP32-storage forms
     float* ustar32 = nullptr;
+    double* rv = nullptr;        // (rho, v) per node: the step recomputes the incoming probabilities from it
+    bool mp_recompute = true;    // the next MP set-up keeps rv (true) or the incoming probabilities win / win32
+                                 // (default: rv in FP64, where it saves 112 of 144 bytes per node; win32 in
+                                 // the FP32 forms, whose 72 bytes cost less than the recomputation)
+    bool mp_rv_on = false;       // the current MP state keeps rv
     double* rem = nullptr;
     double* P[2] = {nullptr, nullptr};
     double* Pa[2] = {nullptr, nullptr};
@@ -178,6 +183,7 @@ void dfree(T*& p) {
 
 void free_mp(lbx_ctx* c) {
     dfree(c->win);
+    dfree(c->rv);
     dfree(c->rem);
     dfree(c->ustar);
     dfree(c->win32);
@@ -421,7 +427,8 @@ void launch_mp_prep(lbx_ctx* c, const double fext[3], double eps) {
     a.j = c->j;
     a.nodes = c->nodes;
     a.map = c->map;
-    a.win = win_buf<W>(c);
+    a.win = c->mp_rv_on ? nullptr : win_buf<W>(c);
+    a.rv = c->mp_rv_on ? c->rv : nullptr;
     a.rem = c->rem;
     a.ustar = ustar_buf<W>(c);
     a.P0 = c->P[0];
@@ -472,6 +479,8 @@ void launch_mp_step(lbx_ctx* c, int now, int64_t it, int64_t k, int32_t stop_on_
     a.check_conv = stop_on_convergence;
     a.lim1 = lim1;
     a.lim2 = lim2;
+    a.rv = c->mp_rv_on ? c->rv : nullptr;
+    a.lambda = c->lambda;
     a.it = it;
     a.ctrl = c->ctrl;
     a.dlx = c->dlx;
@@ -484,7 +493,8 @@ void launch_mp_step(lbx_ctx* c, int now, int64_t it, int64_t k, int32_t stop_on_
     a.tile_order = c->mp_tile_order;
     with_tracers(c->ntr, [&](auto TT) {
         constexpr int T = TT;
-        auto kern = c->wide ? k_mp_step<true, T, W> : k_mp_step<false, T, W>;
+        auto kern = c->mp_rv_on ? (c->wide ? k_mp_step<true, T, W, true> : k_mp_step<false, T, W, true>)
+                                    : (c->wide ? k_mp_step<true, T, W, false> : k_mp_step<false, T, W, false>);
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
     });
 }
@@ -562,6 +572,7 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     c->dly.set((uint32_t)ly);
     c->prec = precision;
     c->floor_window = f32_storage(precision) ? 1000 : 0;
+    c->mp_recompute = !f32_storage(precision);
     c->check_every = f32_storage(precision) ? kCheckEvery : 1;
     c->tau = tau;
     relaxation_coeffs(tau, &c->A, &c->B, &c->C);
@@ -935,10 +946,12 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, int32_t ntr, co
     const uint64_t np3 = 3 * (uint64_t)ntr * c->Np;
     int st = LBX_OK;
     const bool shift = f32_storage(c->prec);
-    if (!st && !shift) st = dalloc(&c->win, 18 * c->Np);
+    c->mp_rv_on = c->mp_recompute;
+    if (!st && !shift && !c->mp_recompute) st = dalloc(&c->win, 18 * c->Np);
     if (!st && !shift) st = dalloc(&c->ustar, 3 * c->Np);
-    if (!st && shift) st = dalloc(&c->win32, 18 * c->Np);
+    if (!st && shift && !c->mp_recompute) st = dalloc(&c->win32, 18 * c->Np);
     if (!st && shift) st = dalloc(&c->ustar32, 3 * c->Np);
+    if (!st && c->mp_recompute) st = dalloc(&c->rv, 4 * c->Np);
     if (!st) st = dalloc(&c->rem, c->Np);
     if (!st) st = dalloc(&c->P[0], np3);
     if (!st) st = dalloc(&c->P[1], np3);
@@ -1292,6 +1305,12 @@ int lbx_set_addressing(lbx_ctx* c, int32_t wide) {
 int lbx_addressing(lbx_ctx* c) { return c ? (c->wide ? 1 : 0) : -1; }
 
 int64_t lbx_fluid_nodes(lbx_ctx* c) { return c ? (int64_t)c->Nf : -1; }
+
+int lbx_set_mp_coefficients(lbx_ctx* c, int32_t form) {
+    if (!c || (form != LBX_MP_STORED && form != LBX_MP_RECOMPUTED)) return LBX_ERR_ARG;
+    c->mp_recompute = form == LBX_MP_RECOMPUTED;
+    return LBX_OK;
+}
 
 int lbx_copy_bandwidth(int64_t bytes, int32_t iters, double* gb_per_s) {
     if (bytes < (1 << 20) || iters <= 0 || !gb_per_s) return LBX_ERR_ARG;

@@ -376,6 +376,7 @@ struct lbxb {
     bool persistent = false;  // a round's launches of each phase as one cooperative launch
     int32_t num_cu = 0;
     int32_t check_every = kCheckEvery;  // FP32 storage: stage-2 steps between l2err measurements
+    int64_t floor_window = 1000;        // FP32 storage: l2err floor test of the instances added from now on
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     // device arrays
@@ -931,7 +932,7 @@ int lbxb_add(lbxb* b, int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, 
     in.check_inv = 1.0;
     in.rho0 = rho0;
     in.target = target_error;
-    in.floor_window = shift(b) ? 1000 : 0;
+    in.floor_window = shift(b) ? b->floor_window : 0;
     // nodes (global packed indices of the instance's fluid nodes) and its map
     std::vector<uint32_t> gnodes(h.nodes_h);
     std::vector<uint32_t> map(N, kSolidSlot);
@@ -1204,6 +1205,13 @@ int lbxb_set_check_interval(lbxb* b, int32_t steps) {
     if (!b || steps < 1 || steps > 4096 || (steps & (steps - 1)) != 0) return LBX_ERR_ARG;
     if (steps != 1 && !shift(b)) return LBX_ERR_ARG;
     b->check_every = steps;
+    return LBX_OK;
+}
+
+int lbxb_set_floor_window(lbxb* b, int64_t steps) {
+    if (!b || steps < 0) return LBX_ERR_ARG;
+    if (steps > 0 && !shift(b)) return LBX_ERR_ARG;
+    b->floor_window = steps;
     return LBX_OK;
 }
 

@@ -714,13 +714,27 @@ __device__ __forceinline__ double as_coef(double v) {
     return (double)(W)v;
 }
 
+// The scattering probability s(r', l') = n_l'/rho - w_l' + lambda w_l' / 2
+// (module_moment_propagation.f90:218-228,331-346) of a node r' with density
+// rho and flow v = j + f, from its tracer population n_l' = a0 rho + a1 c.v
+// (drop_tracers.f90:97-100), rounded as the coefficients are kept (W).  The
+// set-up and the recomputing step kernel call this one function, so they get
+// the same bits.
+template <int L, typename W>
+__device__ __forceinline__ double scatter_prob(double rho, double vx, double vy, double vz, double lambda) {
+    constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
+    const double nt = a0(L) * rho + a1(L) * dot3<cx, cy, cz>(vx, vy, vz);
+    return as_coef<W>(nt / rho - a0(L) + lambda * a0(L) * 0.5);
+}
+
 template <typename W>
 struct MpPrepArgs {
     const double* __restrict__ rho;
     const double* __restrict__ j;
     const uint32_t* __restrict__ nodes;
     const uint32_t* __restrict__ map;
-    W* __restrict__ win;
+    W* __restrict__ win = nullptr;       // incoming probabilities (NULL: not kept, the step recomputes them)
+    double* __restrict__ rv = nullptr;   // (rho, v) of each node, 4 doubles (NULL: not kept)
     double* __restrict__ rem;
     W* __restrict__ ustar;
     double* __restrict__ P0;   // [T][3][Np]
@@ -758,6 +772,12 @@ __device__ __forceinline__ void mp_prep_tile(const MpPrepArgs<W>& a, uint32_t ti
         nb.load(a.map, e & kIndexMask, a.dlx, a.dly, a.lz);
         const double rr = a.rho[i];
         const double vx = a.j[i] + a.fx, vy = a.j[a.Np + i] + a.fy, vz = a.j[2 * a.Np + i] + a.fz;
+        if (a.rv) {
+            a.rv[4 * (uint64_t)i] = rr;
+            a.rv[4 * (uint64_t)i + 1] = vx;
+            a.rv[4 * (uint64_t)i + 2] = vy;
+            a.rv[4 * (uint64_t)i + 3] = vz;
+        }
         double us[3] = {0.0, 0.0, 0.0};
         double p1[3 * T];
 #pragma unroll
@@ -769,8 +789,7 @@ __device__ __forceinline__ void mp_prep_tile(const MpPrepArgs<W>& a, uint32_t ti
             constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
             const uint32_t q = nb.q[L];
             if (q != kSolidSlot) {
-                const double nt = a0(L) * rr + a1(L) * dot3<cx, cy, cz>(vx, vy, vz);
-                const double s = as_coef<W>(nt / rr - a0(L) + a.lambda * a0(L) * 0.5);
+                const double s = scatter_prob<L, W>(rr, vx, vy, vz, a.lambda);
                 remv = remv - s;
                 if constexpr (cx != 0) us[0] = us[0] + sgn<cx>(s);
                 if constexpr (cy != 0) us[1] = us[1] + sgn<cy>(s);
@@ -783,16 +802,15 @@ __device__ __forceinline__ void mp_prep_tile(const MpPrepArgs<W>& a, uint32_t ti
                 }
                 const double rn = a.rho[q];
                 const double wx = a.j[q] + a.fx, wy = a.j[a.Np + q] + a.fy, wz = a.j[2 * a.Np + q] + a.fz;
-                const double ntn = a0(Li) * rn + a1(Li) * dot3<-cx, -cy, -cz>(wx, wy, wz);
-                const double sp = as_coef<W>(ntn / rn - a0(Li) + a.lambda * a0(Li) * 0.5);
-                a.win[comp_index(18, L - 1, i, a.Np)] = (W)sp;
+                const double sp = scatter_prob<Li, W>(rn, wx, wy, wz, a.lambda);
+                if (a.win) a.win[comp_index(18, L - 1, i, a.Np)] = (W)sp;
 #pragma unroll
                 for (int t = 0; t < T; ++t) {
                     if constexpr (cx != 0) p1[3 * t + 0] = p1[3 * t + 0] + sgn<-cx>(sp) * a.tp.bw[t];
                     if constexpr (cy != 0) p1[3 * t + 1] = p1[3 * t + 1] + sgn<-cy>(sp) * a.tp.bw[t];
                     if constexpr (cz != 0) p1[3 * t + 2] = p1[3 * t + 2] + sgn<-cz>(sp) * a.tp.bw[t];
                 }
-            } else {
+            } else if (a.win) {
                 a.win[comp_index(18, L - 1, i, a.Np)] = W(0);
             }
         });
@@ -831,6 +849,10 @@ struct MpStepArgs {
     const double* prev_partials;     // the previous step's, reduced by block 0 (NULL: none)
     int32_t check_conv = 0;          // block 0 also applies the convergence test to VACF(it-1)
     double lim1 = 0.0, lim2 = 0.0;   // its limits (module_moment_propagation.f90:284)
+    // RV steps: (rho, v) of each node, 4 doubles, and lambda, from which the
+    // incoming probabilities are recomputed instead of read from win
+    const double* rv = nullptr;
+    double lambda = 0.0;
     double* prev_vtrace;             // where VACF(it-1) goes
     int64_t it;
     Ctrl* ctrl;
@@ -850,13 +872,19 @@ struct MpStepArgs {
 // mp_step_tile: one block's share; nodes tile*BLOCK + lane (valid below a.Nf),
 // VACF partial to a.partials[K*tile ...]; first_block also reduces the
 // previous step's nparts partials (a.prev_partials) into VACF(it-1).
-template <int BLOCK, bool Wide, int T, typename W>
+//
+// RV: the incoming probabilities s(r+c_l, inv l) are recomputed from the
+// neighbours' (rho, v) (scatter_prob, the set-up's own arithmetic: the same
+// bits) instead of read from win: 32 bytes per node kept and staged in the
+// LDS window beside P, in place of the 18 probabilities of each node.
+template <int BLOCK, bool Wide, int T, typename W, bool RV = false>
 __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t tile, bool first_block,
                                              uint32_t nparts) {
     constexpr int K = 3 * T;  // moment components of the batch
     constexpr int WIN = BLOCK + 2 * kMpHalo;
     __shared__ double lds[BLOCK / 64 * K];
     __shared__ double pwin[K * WIN];  // P(now) of packed nodes [w0, w0 + WIN), comp_index layout
+    __shared__ double rvwin[RV ? 4 * WIN : 2];  // RV: (rho, v) of the same nodes
     const uint32_t i = tile * BLOCK + threadIdx.x;
     const bool valid = i < a.Nf;
     // Each wave's work is a chain of dependent memory round trips (stop flag,
@@ -888,12 +916,17 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     // along y where a row holds fewer fluid nodes than that, are read from LDS
     const int64_t w0 = (int64_t)tile * BLOCK - kMpHalo;
     const LdsView pw{pwin};
+    const View<Wide> rvv(a.rv, 4 * a.Np);
     for (int n = threadIdx.x; n < WIN; n += BLOCK) {
         const int64_t g = w0 + n;
         if (g >= 0 && g < (int64_t)Np) {
             double v[K];
             load_comps<K>(Pn, Np, (uint32_t)g, v);
             store_comps<K>(pw, WIN, (uint32_t)n, v);
+            if constexpr (RV) {
+                ((Vec2<double>*)rvwin)[2 * n] = rvv.ld2(2 * (uint32_t)g);
+                ((Vec2<double>*)rvwin)[2 * n + 1] = rvv.ld2(2 * (uint32_t)g + 1);
+            }
         }
     }
     __syncthreads();
@@ -909,16 +942,31 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
         Vec2<W> wp;  // the probabilities of directions L, L+1 (L odd) from one paired load
         static_for<1, kQ>([&](auto I) {
             constexpr int L = I;
-            if constexpr (L & 1) wp = win.ld2((uint32_t)((L - 1) / 2) * Np + i);
+            if constexpr (!RV && (L & 1)) wp = win.ld2((uint32_t)((L - 1) / 2) * Np + i);
             const uint32_t q = nb.q[L];
             if (q != kSolidSlot) {
-                const double w = (double)wp[(L - 1) & 1];
                 double pq[K];
+                double w;
                 const uint32_t d = (uint32_t)((int64_t)q - w0);
-                if (d < (uint32_t)WIN)
-                    load_comps<K>(pw, WIN, d, pq);
-                else
-                    load_comps<K>(Pn, Np, q, pq);
+                if constexpr (RV) {
+                    Vec2<double> r0, r1;  // (rho, vx), (vy, vz) of the neighbour
+                    if (d < (uint32_t)WIN) {
+                        load_comps<K>(pw, WIN, d, pq);
+                        r0 = ((const Vec2<double>*)rvwin)[2 * d];
+                        r1 = ((const Vec2<double>*)rvwin)[2 * d + 1];
+                    } else {
+                        load_comps<K>(Pn, Np, q, pq);
+                        r0 = rvv.ld2(2 * q);
+                        r1 = rvv.ld2(2 * q + 1);
+                    }
+                    w = scatter_prob<kInv[L], W>(r0[0], r0[1], r1[0], r1[1], a.lambda);
+                } else {
+                    w = (double)wp[(L - 1) & 1];
+                    if (d < (uint32_t)WIN)
+                        load_comps<K>(pw, WIN, d, pq);
+                    else
+                        load_comps<K>(Pn, Np, q, pq);
+                }
 #pragma unroll
                 for (int k = 0; k < K; ++k) acc[k] = acc[k] + pq[k] * w;
             }
@@ -956,10 +1004,10 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     block_partial<K, BLOCK>(vp, a.partials + (uint64_t)K * tile, lds);
 }
 
-template <bool Wide, int T, typename W>
+template <bool Wide, int T, typename W, bool RV>
 __global__ __launch_bounds__(kBlock, LBX_MP_WAVES) void k_mp_step(MpStepArgs<W> a) {
-    mp_step_tile<kBlock, Wide, T, W>(a, a.tile_order ? a.tile_order[xcd_tile()] : xcd_tile(), blockIdx.x == 0,
-                                     gridDim.x);
+    mp_step_tile<kBlock, Wide, T, W, RV>(a, a.tile_order ? a.tile_order[xcd_tile()] : xcd_tile(), blockIdx.x == 0,
+                                         gridDim.x);
 }
 
 // Plain streaming copy (16 B per lane per iteration, grid-stride): the

@@ -21,6 +21,7 @@ NOALLOC_MAX_NODES = 320000
 PREC_FP64, PREC_FP32_SHIFT, PREC_FP32_STORE = 0, 1, 2
 EXIT_CONVERGED, EXIT_CAP, EXIT_FLOOR = 1, 2, 3
 KERNEL_LB, KERNEL_MP = 0, 1
+MP_STORED, MP_RECOMPUTED = 0, 1
 
 _p = ctypes.c_void_p
 _i32, _i64, _f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -74,6 +75,7 @@ SIGNATURES = {
     "lbx_set_check_interval": (ctypes.c_int, [_p, _i32]),
     "lbx_set_addressing": (ctypes.c_int, [_p, _i32]),
     "lbx_addressing": (ctypes.c_int, [_p]),
+    "lbx_set_mp_coefficients": (ctypes.c_int, [_p, _i32]),
     "lbx_fluid_nodes": (_i64, [_p]),
     "lbx_sync": (ctypes.c_int, [_p]),
     "lbxb_footprint_bytes": (_i64, [_i32, _i64, _i64, _i32]),
@@ -86,6 +88,7 @@ SIGNATURES = {
     "lbxb_result": (ctypes.c_int, [_p, _i32, _p, _p, _i64, _p, _p, _p, _p]),
     "lbxb_set_block": (ctypes.c_int, [_p, _i32, _i32]),
     "lbxb_set_check_interval": (ctypes.c_int, [_p, _i32]),
+    "lbxb_set_floor_window": (ctypes.c_int, [_p, _i64]),
     "lbxb_autotune": (ctypes.c_int, [_p, _i32, _p, _p, _p]),
     "lbxb_set_launch_mode": (ctypes.c_int, [_p, _i32]),
     "lbxb_set_addressing": (ctypes.c_int, [_p, _i32]),
@@ -344,6 +347,13 @@ class Lattice:
     def addressing(self):
         return int(lib().lbx_addressing(self._h))
 
+    def set_mp_coefficients(self, form):
+        """MP_RECOMPUTED (default in FP64): the MP step recomputes the incoming
+        scattering probabilities from each node's density and flow;
+        MP_STORED (default in the FP32 forms): keeps them (18 per node).  Same bits; from the next
+        mp_init."""
+        _check(lib().lbx_set_mp_coefficients(self._h, int(form)), "set_mp_coefficients")
+
     @property
     def fluid_nodes(self):
         return int(lib().lbx_fluid_nodes(self._h))
@@ -573,6 +583,10 @@ class Batch:
 
     def set_block(self, lb_block, mp_block):
         _check(lib().lbxb_set_block(self._h, int(lb_block), int(mp_block)), "lbxb_set_block")
+
+    def set_floor_window(self, steps):
+        """lbx_set_floor_window for the instances added from now on."""
+        _check(lib().lbxb_set_floor_window(self._h, int(steps)), "lbxb_set_floor_window")
 
     def set_check_interval(self, steps):
         """lbxb_set_check_interval: the FP32-storage forms' stage-2 l2err interval."""

@@ -86,15 +86,22 @@ def _add(B, r):
 
 
 # ------------------------------------------------------------------ batch
-def run_batch(reqs, precision, lb_round=128, mp_round=128, autotune=True, launch=None):
+def run_batch(reqs, precision, lb_round=128, mp_round=128, autotune=True, launch=None, lb_steps=None):
     """All requests admitted at once and advanced together; returns per
     request D_eff, LB steps and exit reason, plus throughput.  launch (one of
-    lbx.LAUNCH_*) fixes the launch mode and turns the autotuner off."""
+    lbx.LAUNCH_*) fixes the launch mode and turns the autotuner off.
+    lb_steps (one count per request): each request takes exactly that many
+    LB steps (no stop test: target 0, no floor test), so that another form's
+    run of the same requests can be compared step for step."""
     cap, box = capacity_for(reqs)
     with lbx.Batch(cap, box, len(reqs), precision=precision) as B:
         if launch is not None:
             B.set_launch_mode(launch)
             autotune = False
+        if lb_steps is not None:
+            if precision != lbx.PREC_FP64:
+                B.set_floor_window(0)
+            reqs = [dict(r, target=0.0, max_lb=int(n)) for r, n in zip(reqs, lb_steps)]
         ids = [_add(B, r) for r in reqs]
         assert None not in ids, "capacity"
         tuned = []
@@ -244,7 +251,15 @@ def main(argv=None):
             print(f"serving: batch {p} done in {r['wall_s']:.1f} s", file=sys.stderr, flush=True)
         for p in ("fp32_shift", "fp32_store"):
             if p in res and "fp64" in res:
+                # precision: the same requests, each with the FP64 run's LB step count
+                steps64 = [x["lb_steps"] for x in res["fp64"]["results"]]
+                same = run_batch(reqs, _prec(p), launch=launch, lb_steps=steps64)
                 d64 = np.array([x["deff"] for x in res["fp64"]["results"]])
+                ds = np.array([x["deff"] for x in same["results"]])
+                rel_s = np.abs(ds - d64).max(axis=1) / np.abs(d64).max(axis=1)
+                out[f"deff_rel_error_{p}_same_lb_steps"] = {
+                    "max": float(f"{rel_s.max():.3e}"), "median": float(f"{np.median(rel_s):.3e}"),
+                    "count_above_1e-5": int((rel_s > 1e-5).sum())}
                 d32 = np.array([x["deff"] for x in res[p]["results"]])
                 rel = np.abs(d32 - d64).max(axis=1) / np.abs(d64).max(axis=1)
                 out[f"deff_rel_error_{p}"] = {"max": float(f"{rel.max():.3e}"), "median": float(f"{np.median(rel):.3e}"),
