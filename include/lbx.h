@@ -409,7 +409,9 @@ int lbxb_info(lbxb* b, int32_t id, int32_t* phase, int32_t* status, int64_t* lb_
 int lbxb_result(lbxb* b, int32_t id, double deff[3], double* vacf, int64_t vacf_rows, double* rho, double* jx,
                 double* jy, double* jz);
 
-/* Block size (128 or 256 nodes) of the LB and MP launches. */
+/* Block size (64, 128 or 256 nodes) of the LB and MP launches.  LB results
+ * do not depend on it; VACF and D_eff sum per block, so they agree across
+ * block sizes to rounding. */
 int lbxb_set_block(lbxb* b, int32_t lb_block, int32_t mp_block);
 
 /* lbx_set_check_interval for every instance of a batch (set before the
@@ -432,10 +434,14 @@ int lbxb_launch_mode(lbxb* b);
  * (the capacity decides by itself; wide = 1 forces the 64-bit form). */
 int lbxb_set_addressing(lbxb* b, int32_t wide);
 int lbxb_addressing(lbxb* b);
-/* Autotuner: times 128- and 256-node blocks per phase on the current state
- * without changing it and keeps the faster, then keeps the persistent mode
- * if it beats per-step launches in every phase; times[6] = mean ms per step
- * of LB-128, LB-256, MP-128, MP-256, LB-persistent, MP-persistent. */
+/* Autotuner: on the current state, without changing it, times 64-, 128-
+ * and 256-node blocks per phase with one launch per step and keeps the
+ * fastest, then times those launches replayed from a captured graph and as
+ * one persistent cooperative launch per phase, and keeps the launch mode
+ * with the least LB + MP time.  times[LBXB_AUTOTUNE_TIMES] (may be NULL) =
+ * mean ms per step of LB-64, LB-128, LB-256, MP-64, MP-128, MP-256,
+ * LB-graph, MP-graph, LB-persistent, MP-persistent (0: phase not timed). */
+#define LBXB_AUTOTUNE_TIMES 10
 int lbxb_autotune(lbxb* b, int32_t launches, int32_t* lb_block, int32_t* mp_block, double* times);
 /* Device time (HIP events around each phase of each round) and work done. */
 int lbxb_stats(lbxb* b, double* lb_ms, int64_t* lb_box_updates, int64_t* lb_fluid_updates, double* mp_ms,

@@ -789,7 +789,6 @@ int lbx_equilibrate_advance(lbx_ctx* c, int64_t until_step, int64_t* steps_done,
     if (!q.active) return q.began ? q.status : LBX_ERR_STATE;
     const double zero[3] = {0.0, 0.0, 0.0};
     const uint32_t gt = c->ntiles;
-    const bool shift = f32_storage(c->prec);
     for (;;) {
         if (q.max_steps > 0 && q.t >= q.max_steps) {
             // a convergence at the capped step is still reported as such

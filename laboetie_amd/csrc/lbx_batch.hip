@@ -529,43 +529,54 @@ BatchMpView<W> mp_view(lbxb* b) {
     return v;
 }
 
+// f(std::integral_constant<int, B>{}) for the block size B of a launch (64, 128 or 256 nodes)
+template <typename F>
+void with_block(int block, F&& f) {
+    if (block == 64)
+        f(std::integral_constant<int, 64>{});
+    else if (block == 128)
+        f(std::integral_constant<int, 128>{});
+    else
+        f(std::integral_constant<int, 256>{});
+}
+
 // launch LB step k of a round over every allocated tile
 void launch_lb(lbxb* b, int64_t k, int block) {
     const uint32_t grid = grid_of(b, block);
     if (!grid) return;
-    if (b->prec == LBX_PREC_FP32_SHIFT) {
-        auto v = lb_view<float>(b);
-        auto kern = block == 128 ? (b->wide ? kb_lb_step<128, true, float> : kb_lb_step<128, false, float>)
-                                 : (b->wide ? kb_lb_step<256, true, float> : kb_lb_step<256, false, float>);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, v, k);
-    } else if (b->prec == LBX_PREC_FP32_STORE) {
-        auto v = lb_view<float, double>(b);
-        auto kern = block == 128
-                        ? (b->wide ? kb_lb_step<128, true, float, double> : kb_lb_step<128, false, float, double>)
-                        : (b->wide ? kb_lb_step<256, true, float, double> : kb_lb_step<256, false, float, double>);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, v, k);
-    } else {
-        auto v = lb_view<double>(b);
-        auto kern = block == 128 ? (b->wide ? kb_lb_step<128, true, double> : kb_lb_step<128, false, double>)
-                                 : (b->wide ? kb_lb_step<256, true, double> : kb_lb_step<256, false, double>);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, v, k);
-    }
+    with_block(block, [&](auto BT) {
+        constexpr int BL = decltype(BT)::value;
+        if (b->prec == LBX_PREC_FP32_SHIFT) {
+            auto v = lb_view<float>(b);
+            auto kern = b->wide ? kb_lb_step<BL, true, float> : kb_lb_step<BL, false, float>;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k);
+        } else if (b->prec == LBX_PREC_FP32_STORE) {
+            auto v = lb_view<float, double>(b);
+            auto kern = b->wide ? kb_lb_step<BL, true, float, double> : kb_lb_step<BL, false, float, double>;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k);
+        } else {
+            auto v = lb_view<double>(b);
+            auto kern = b->wide ? kb_lb_step<BL, true, double> : kb_lb_step<BL, false, double>;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k);
+        }
+    });
 }
 
 void launch_mp(lbxb* b, int64_t k, int block, int reduce_prev) {
     const uint32_t grid = grid_of(b, block);
     if (!grid) return;
-    if (shift(b)) {
-        auto v = mp_view<float>(b);
-        auto kern = block == 128 ? (b->wide ? kb_mp_step<128, true, float> : kb_mp_step<128, false, float>)
-                                 : (b->wide ? kb_mp_step<256, true, float> : kb_mp_step<256, false, float>);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, v, k, reduce_prev);
-    } else {
-        auto v = mp_view<double>(b);
-        auto kern = block == 128 ? (b->wide ? kb_mp_step<128, true, double> : kb_mp_step<128, false, double>)
-                                 : (b->wide ? kb_mp_step<256, true, double> : kb_mp_step<256, false, double>);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, v, k, reduce_prev);
-    }
+    with_block(block, [&](auto BT) {
+        constexpr int BL = decltype(BT)::value;
+        if (shift(b)) {
+            auto v = mp_view<float>(b);
+            auto kern = b->wide ? kb_mp_step<BL, true, float> : kb_mp_step<BL, false, float>;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k, reduce_prev);
+        } else {
+            auto v = mp_view<double>(b);
+            auto kern = b->wide ? kb_mp_step<BL, true, double> : kb_mp_step<BL, false, double>;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k, reduce_prev);
+        }
+    });
 }
 
 void launch_lb_finalize(lbxb* b) {
@@ -574,9 +585,11 @@ void launch_lb_finalize(lbxb* b) {
 }
 
 void launch_mp_reduce_last(lbxb* b, int block) {
-    auto kern = block == 128 ? kb_mp_reduce_last<128> : kb_mp_reduce_last<256>;
-    hipLaunchKernelGGL(kern, dim3(b->cap_inst), dim3(kBlock), 0, b->stream, b->insts, b->part[0], b->part[1],
-                       b->vtrace, b->cap_inst, b->ctrl);
+    with_block(block, [&](auto BT) {
+        constexpr int BL = decltype(BT)::value;
+        hipLaunchKernelGGL(kb_mp_reduce_last<BL>, dim3(b->cap_inst), dim3(kBlock), 0, b->stream, b->insts,
+                           b->part[0], b->part[1], b->vtrace, b->cap_inst, b->ctrl);
+    });
 }
 
 // the S launches of one phase of a round (+ its closing reduction), replayed
@@ -654,20 +667,18 @@ int run_phase_persistent(lbxb* b, int phase, int64_t S, int32_t redo_only) {
         void* args[] = {&lv, &mv, &S_lb, &S_mp, &nt, &ni, &ro};
         return hip_status(hipLaunchCooperativeKernel((const void*)kern, dim3(grid), dim3(BL), args, 0, b->stream));
     };
-    using I128 = std::integral_constant<int, 128>;
-    using I256 = std::integral_constant<int, 256>;
     using T_ = std::true_type;
     using F_ = std::false_type;
-    if (b->prec == LBX_PREC_FP32_SHIFT) {
-        if (block == 128) return b->wide ? go(float{}, float{}, I128{}, T_{}) : go(float{}, float{}, I128{}, F_{});
-        return b->wide ? go(float{}, float{}, I256{}, T_{}) : go(float{}, float{}, I256{}, F_{});
-    }
-    if (b->prec == LBX_PREC_FP32_STORE) {
-        if (block == 128) return b->wide ? go(float{}, double{}, I128{}, T_{}) : go(float{}, double{}, I128{}, F_{});
-        return b->wide ? go(float{}, double{}, I256{}, T_{}) : go(float{}, double{}, I256{}, F_{});
-    }
-    if (block == 128) return b->wide ? go(double{}, double{}, I128{}, T_{}) : go(double{}, double{}, I128{}, F_{});
-    return b->wide ? go(double{}, double{}, I256{}, T_{}) : go(double{}, double{}, I256{}, F_{});
+    int st = LBX_OK;
+    with_block(block, [&](auto BT) {
+        if (b->prec == LBX_PREC_FP32_SHIFT)
+            st = b->wide ? go(float{}, float{}, BT, T_{}) : go(float{}, float{}, BT, F_{});
+        else if (b->prec == LBX_PREC_FP32_STORE)
+            st = b->wide ? go(float{}, double{}, BT, T_{}) : go(float{}, double{}, BT, F_{});
+        else
+            st = b->wide ? go(double{}, double{}, BT, T_{}) : go(double{}, double{}, BT, F_{});
+    });
+    return st;
 }
 
 template <typename T>
@@ -815,7 +826,7 @@ int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
         if (!st && sh) st = dalloc(&b->g[k], kQ * Np);
         if (!st) st = dalloc(&b->P[k], 3 * Np);
         if (!st) st = dalloc(&b->Pa[k], 3 * Np);
-        if (!st) st = dalloc(&b->part[k], (Np / 128) * 3);
+        if (!st) st = dalloc(&b->part[k], (Np / 64) * 3);  // one VACF partial per 64-node block
     }
     if (!st && precision == LBX_PREC_FP32_SHIFT) st = dalloc(&b->j32, 3 * Np);
     if (!st) st = dalloc(&b->j, 3 * Np);
@@ -1195,7 +1206,8 @@ int lbxb_result(lbxb* b, int32_t id, double deff[3], double* vacf, int64_t vacf_
 }
 
 int lbxb_set_block(lbxb* b, int32_t lb_block, int32_t mp_block) {
-    if (!b || (lb_block != 128 && lb_block != 256) || (mp_block != 128 && mp_block != 256)) return LBX_ERR_ARG;
+    auto ok = [](int32_t x) { return x == 64 || x == 128 || x == 256; };
+    if (!b || !ok(lb_block) || !ok(mp_block)) return LBX_ERR_ARG;
     b->lb_block = lb_block;
     b->mp_block = mp_block;
     return LBX_OK;
@@ -1237,20 +1249,26 @@ int lbxb_launch_mode(lbxb* b) {
     return b->persistent ? LBXB_LAUNCH_PERSISTENT : (b->graphs ? LBXB_LAUNCH_GRAPH : LBXB_LAUNCH_STREAM);
 }
 
-// Times `launches` launches of each block size on the current state without
-// changing it: LB launches redo each LB instance's last completed step (same
-// inputs, same outputs), MP launches recompute its last MP step without the
-// VACF reduction.  Picks the faster size per phase, then times the same
-// repeated steps in one cooperative launch per phase and keeps that
-// (persistent) mode when it is faster in every phase, else HIP graphs.
-// times[6] (may be NULL) receives mean ms per step of LB-128, LB-256,
-// MP-128, MP-256, LB-persistent, MP-persistent (0 when a phase had no
-// instance to time).
+// The autotuner's search space: the block size of each phase (64, 128 or
+// 256 nodes, i.e. 1, 2 or 4 waves per block and 4, 2 or 1 blocks per
+// 256-node tile) and the launch mode of the rounds (one launch per step, the
+// same launches replayed from a captured graph, or one cooperative
+// persistent launch per phase).  Every point is timed on the current state
+// without changing it: LB launches redo each LB instance's last completed step
+// (same inputs, same outputs), MP launches recompute its last MP step without
+// the VACF reduction.  The fastest block size per phase is kept, then the
+// mode with the least LB + MP time at those sizes (the persistent launch
+// runs both phases with the LB block size).  times[LBXB_AUTOTUNE_TIMES] (may be
+// NULL): mean ms per step of LB-64, LB-128, LB-256, MP-64, MP-128, MP-256
+// (per-step launches), LB-graph, MP-graph, LB-persistent, MP-persistent (0
+// when a phase had no instance to time).
 int lbxb_autotune(lbxb* b, int32_t launches, int32_t* lb_block, int32_t* mp_block, double* times) {
     if (!b || launches <= 0) return LBX_ERR_ARG;
     const uint32_t ni = b->cap_inst;
-    double tm[4] = {0, 0, 0, 0};
-    bool any_lb = false, any_mp = false;
+    constexpr int kSizes = 3;
+    const int sizes[kSizes] = {64, 128, 256};
+    double tm[2][kSizes] = {{0, 0, 0}, {0, 0, 0}};
+    bool any[2] = {false, false};
     for (uint32_t id = 0; id < ni; ++id) {
         BInst& in = b->tab[id];
         const HostInst& h = b->hi[id];
@@ -1266,32 +1284,66 @@ int lbxb_autotune(lbxb* b, int32_t launches, int32_t* lb_block, int32_t* mp_bloc
                 in.fm[a] = (on ? h.F[a] : 0.0) / 2.0;
                 in.fc[a] = h.stage == 2 ? h.F[a] : 0.0;
             }
-            any_lb = true;
+            any[0] = true;
         } else if (h.phase == kMp && h.mp_it >= 1) {
             in.mp_on = 1;
             in.mp_it0 = h.mp_it - 1;  // recompute step mp_it
             in.mp_n = 1;
-            any_mp = true;
+            any[1] = true;
         }
     }
     LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
-    const int sizes[2] = {128, 256};
-    for (int s = 0; s < 2; ++s) {
+    auto launch = [&](int phase, int block) {
+        if (phase == 0) launch_lb(b, 0, block); else launch_mp(b, 0, block, 0);
+    };
+    auto elapsed = [&](double* out) -> int {
+        LBX_TRY(hipEventRecord(b->ev[1], b->stream));
+        LBX_TRY(hipEventSynchronize(b->ev[1]));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+        *out = ms / launches;
+        return LBX_OK;
+    };
+    // per-step launches of every block size
+    for (int s = 0; s < kSizes; ++s)
         for (int phase = 0; phase < 2; ++phase) {
-            if ((phase == 0 && !any_lb) || (phase == 1 && !any_mp)) continue;
-            if (phase == 0) launch_lb(b, 0, sizes[s]); else launch_mp(b, 0, sizes[s], 0);  // warm
+            if (!any[phase]) continue;
+            launch(phase, sizes[s]);  // warm
             LBX_TRY(hipEventRecord(b->ev[0], b->stream));
-            for (int32_t k = 0; k < launches; ++k)
-                if (phase == 0) launch_lb(b, 0, sizes[s]); else launch_mp(b, 0, sizes[s], 0);
-            LBX_TRY(hipEventRecord(b->ev[1], b->stream));
-            LBX_TRY(hipEventSynchronize(b->ev[1]));
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
-            tm[2 * phase + s] = ms / launches;
+            for (int32_t k = 0; k < launches; ++k) launch(phase, sizes[s]);
+            LBX_TRY(elapsed(&tm[phase][s]));
         }
+    int best[2] = {b->lb_block, b->mp_block};
+    for (int phase = 0; phase < 2; ++phase) {
+        if (!any[phase]) continue;
+        int k = 0;
+        for (int s = 1; s < kSizes; ++s)
+            if (tm[phase][s] < tm[phase][k]) k = s;
+        best[phase] = sizes[k];
     }
-    if (any_lb) b->lb_block = tm[0] < tm[1] ? 128 : 256;
-    if (any_mp) b->mp_block = tm[2] < tm[3] ? 128 : 256;
+    b->lb_block = best[0];
+    b->mp_block = best[1];
+    // the same launches at those sizes replayed from a captured graph
+    double tg[2] = {0, 0};
+    for (int phase = 0; phase < 2; ++phase) {
+        if (!any[phase]) continue;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        LBX_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int32_t k = 0; k < launches; ++k) launch(phase, best[phase]);
+        const hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+        if (ce != hipSuccess) return hip_status(ce);
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return hip_status(ie);
+        int st = hip_status(hipGraphLaunch(exec, b->stream));  // warm
+        if (!st) st = hip_status(hipEventRecord(b->ev[0], b->stream));
+        if (!st) st = hip_status(hipGraphLaunch(exec, b->stream));
+        if (!st) st = elapsed(&tg[phase]);
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipGraphExecDestroy(exec);
+        LBX_TRY(st);
+    }
     // one cooperative launch of `launches` repeated steps per phase, with the
     // LB block size (kb_round runs both phases with one size)
     double tp[2] = {0, 0};
@@ -1303,35 +1355,40 @@ int lbxb_autotune(lbxb* b, int32_t launches, int32_t* lb_block, int32_t* mp_bloc
     LBX_TRY(hipStreamSynchronize(b->stream));
     LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
     for (int phase = 0; phase < 2; ++phase) {
-        if ((phase == 0 && !any_lb) || (phase == 1 && !any_mp)) continue;
+        if (!any[phase]) continue;
         LBX_TRY(run_phase_persistent(b, phase, 1, 1));  // warm
         LBX_TRY(hipEventRecord(b->ev[0], b->stream));
         LBX_TRY(run_phase_persistent(b, phase, launches, 1));
-        LBX_TRY(hipEventRecord(b->ev[1], b->stream));
-        LBX_TRY(hipEventSynchronize(b->ev[1]));
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
-        tp[phase] = ms / launches;
+        LBX_TRY(elapsed(&tp[phase]));
     }
     for (uint32_t id = 0; id < ni; ++id) {
         BInst& in = b->tab[id];
         in.lb_on = in.mp_on = in.lb_redo = in.mp_repeat = 0;
     }
     LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
-    // persistent when it beats the per-step launches in every phase that was timed
-    const double per_lb = any_lb ? std::min(tm[0], tm[1]) : 0.0, per_mp = any_mp ? std::min(tm[2], tm[3]) : 0.0;
-    if (any_lb || any_mp) {
-        const bool pers = (!any_lb || tp[0] < per_lb) && (!any_mp || tp[1] < per_mp);
-        b->persistent = pers;
-        b->graphs = !pers;
+    // the launch mode with the least time over the timed phases
+    if (any[0] || any[1]) {
+        double t_stream = 0, t_graph = 0, t_pers = 0;
+        for (int phase = 0; phase < 2; ++phase) {
+            if (!any[phase]) continue;
+            for (int s = 0; s < kSizes; ++s)
+                if (sizes[s] == best[phase]) t_stream += tm[phase][s];
+            t_graph += tg[phase];
+            t_pers += tp[phase];
+        }
+        b->persistent = t_pers < t_graph && t_pers < t_stream;
+        b->graphs = !b->persistent && t_graph <= t_stream;
     }
     clear_graphs(b);
     if (lb_block) *lb_block = b->lb_block;
     if (mp_block) *mp_block = b->mp_block;
     if (times) {
-        std::memcpy(times, tm, sizeof(tm));
-        times[4] = tp[0];
-        times[5] = tp[1];
+        for (int phase = 0; phase < 2; ++phase)
+            for (int s = 0; s < kSizes; ++s) times[phase * kSizes + s] = tm[phase][s];
+        times[6] = tg[0];
+        times[7] = tg[1];
+        times[8] = tp[0];
+        times[9] = tp[1];
     }
     return hip_status(hipStreamSynchronize(b->stream));
 }

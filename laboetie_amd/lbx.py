@@ -22,6 +22,7 @@ PREC_FP64, PREC_FP32_SHIFT, PREC_FP32_STORE = 0, 1, 2
 EXIT_CONVERGED, EXIT_CAP, EXIT_FLOOR = 1, 2, 3
 KERNEL_LB, KERNEL_MP = 0, 1
 MP_STORED, MP_RECOMPUTED = 0, 1
+AUTOTUNE_TIMES = 10  # LBXB_AUTOTUNE_TIMES
 
 _p = ctypes.c_void_p
 _i32, _i64, _f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -593,13 +594,17 @@ class Batch:
         _check(lib().lbxb_set_check_interval(self._h, int(steps)), "lbxb_set_check_interval")
 
     def autotune(self, launches=10):
+        """lbxb_autotune: block size per phase (64/128/256 nodes) and launch
+        mode (per-step, graph, persistent); the chosen point and every timing
+        (mean ms per step)."""
         lb, mp = _i32(), _i32()
-        t = np.zeros(6)
+        t = np.zeros(AUTOTUNE_TIMES)
         _check(lib().lbxb_autotune(self._h, int(launches), ctypes.byref(lb), ctypes.byref(mp), _ptr(t)),
                "lbxb_autotune")
+        keys = ("lb64", "lb128", "lb256", "mp64", "mp128", "mp256", "lb_graph", "mp_graph", "lb_persistent",
+                "mp_persistent")
         return dict(lb_block=lb.value, mp_block=mp.value, mode=self.launch_mode,
-                    ms={"lb128": t[0], "lb256": t[1], "mp128": t[2], "mp256": t[3], "lb_persistent": t[4],
-                        "mp_persistent": t[5]})
+                    ms={k: float(v) for k, v in zip(keys, t)})
 
     def set_launch_mode(self, mode):
         _check(lib().lbxb_set_launch_mode(self._h, int(mode)), "lbxb_set_launch_mode")

@@ -94,3 +94,9 @@ def test_footprint_grows_with_size_and_precision():
     c = lbx.footprint_bytes(128, 128, 128, lbx.PREC_FP32_SHIFT, True)
     assert 7.5 < b / a < 8.5
     assert c < b
+
+
+def test_autotune_times_length_matches_header():
+    text = open(os.path.join(REPO, "include", "lbx.h")).read()
+    m = re.search(r"#define LBXB_AUTOTUNE_TIMES (\d+)", text)
+    assert m and int(m.group(1)) == lbx.AUTOTUNE_TIMES

@@ -130,8 +130,10 @@ def test_batch_admission_autotune_and_slot_reuse():
         B.round(16, 0)  # force switch, then stage 2
         assert B.info(a)["phase"] == lbx.PHASE_LB
         t = B.autotune(3)
-        assert t["lb_block"] in (128, 256) and t["ms"]["lb128"] > 0 and t["ms"]["lb256"] > 0
-        assert t["ms"]["lb_persistent"] > 0 and t["mode"] in (lbx.LAUNCH_GRAPH, lbx.LAUNCH_PERSISTENT)
+        assert t["lb_block"] in (64, 128, 256)
+        assert all(t["ms"][k] > 0 for k in ("lb64", "lb128", "lb256", "lb_graph", "lb_persistent"))
+        assert t["ms"]["mp64"] == 0 and t["ms"]["mp_graph"] == 0  # no MP phase yet
+        assert t["mode"] in (lbx.LAUNCH_STREAM, lbx.LAUNCH_GRAPH, lbx.LAUNCH_PERSISTENT)
         while B.round(64, 64) > 0:
             pass
         ra = B.result(a, vacf_rows=81, flow=True)
