@@ -545,6 +545,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 roofline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 1024-instance batch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (Poisson stream) leg")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the Fortran drop-in's MP timing (a child process: profiled runs leave it out)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain-copy bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -584,7 +586,7 @@ def main():
     slit = slit_leg()
     batch = None if args.no_batch else batch_leg()
     served = None if args.no_serving else serving_leg()
-    dropin = dropin_leg(w)
+    dropin = None if args.no_dropin else dropin_leg(w)
     copy_gbs = lbx.copy_bandwidth(2 << 30, 20)  # measured ceiling, outside the timed region
     lb_steps, mp_steps, deffs = results[-1]
     updates = N * sum(r[0] + r[1] for r in results)
