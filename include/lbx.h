@@ -256,6 +256,16 @@ int lbx_mp_tracers(lbx_ctx* ctx, int32_t* ntracers);
 int lbx_mp_run(lbx_ctx* ctx, int64_t nsteps, int32_t stop_on_convergence, double* vacf_trace,
                int64_t* steps_done, int32_t* converged);
 
+/* lbx_mp_run in two halves, so that the caller's host work (writing the
+ * previous steps' vacf.dat lines, module_moment_propagation.f90:269) runs
+ * while the GPU steps: lbx_mp_run_start launches 1 <= nsteps <= 512 steps
+ * and returns; lbx_mp_run_wait waits for them and returns what lbx_mp_run
+ * would have.  One begun chunk at a time; until it is waited for, the other
+ * lbx_mp_* calls return LBX_ERR_STATE (lbx_mp_free and lbx_mp_init* drop it).
+ * The same steps and bits as lbx_mp_run.  No counterpart in the reference. */
+int lbx_mp_run_start(lbx_ctx* ctx, int64_t nsteps, int32_t stop_on_convergence);
+int lbx_mp_run_wait(lbx_ctx* ctx, double* vacf_trace, int64_t* steps_done, int32_t* converged);
+
 /* Current P(now) and, with adsorption, Pa(now), (3,lx,ly,lz), of tracer 0;
  * either may be NULL. */
 int lbx_mp_get(lbx_ctx* ctx, double* P, double* Pa);

@@ -114,6 +114,13 @@ struct lbx_ctx {
     TracerParams tp{};
     bool any_ads = false;
     bool mp_stopped = false;
+    // a chunk of moment-propagation steps begun by lbx_mp_run_start and not yet waited for
+    struct {
+        bool active = false;
+        int64_t S = 0;               // steps launched
+        double* vt_h = nullptr;      // pinned: its VACF trace (3 kMaxTracers kMpChunk doubles)
+        Ctrl* ctrl_h = nullptr;      // pinned: the control block after it
+    } mpp;
     // timing
     bool timing = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -182,6 +189,10 @@ void dfree(T*& p) {
 }
 
 void free_mp(lbx_ctx* c) {
+    if (c->mpp.active) {  // a begun chunk is dropped once its kernels are done with the buffers
+        (void)hipStreamSynchronize(c->stream);
+        c->mpp.active = false;
+    }
     dfree(c->win);
     dfree(c->rv);
     dfree(c->rem);
@@ -226,6 +237,10 @@ int band_tile_order(lbx_ctx* c, int32_t band_rows, uint32_t** out) {
 
 void free_all(lbx_ctx* c) {
     free_mp(c);
+    if (c->mpp.vt_h) (void)hipHostFree(c->mpp.vt_h);
+    if (c->mpp.ctrl_h) (void)hipHostFree(c->mpp.ctrl_h);
+    c->mpp.vt_h = nullptr;
+    c->mpp.ctrl_h = nullptr;
     dfree(c->f[0]);
     dfree(c->f[1]);
     dfree(c->g[0]);
@@ -1009,61 +1024,116 @@ int lbx_mp_init_from_flow(lbx_ctx* c, const double* rho, const double* jx, const
     return mp_setup(c, fext, Db, 1, &ka, &kd, vacf0);
 }
 
+// The convergence test's limits (module_moment_propagation.f90:284): 1/(2 N/Db)
+// and a default-real literal
+static void mp_limits(const lbx_ctx* c, double* lim1, double* lim2) {
+    *lim1 = 1.0 / (2.0 * (double)c->lx * (double)c->ly * (double)c->lz / c->Db);
+    *lim2 = (double)1.e-12f;
+}
+
+// Launches S <= kMpChunk steps from the context's current MP step, the last
+// step's VACF reduction (and convergence test), and the copies of the chunk's
+// VACF trace (3T S doubles) and of the control block into vt_h, ctrl_h.
+static int mp_chunk_launch(lbx_ctx* c, int64_t S, int32_t stop, double* vt_h, Ctrl* ctrl_h) {
+    const int nv = 3 * c->ntr;
+    double lim1, lim2;
+    mp_limits(c, &lim1, &lim2);
+    timer_begin(c);
+    for (int64_t k = 0; k < S; ++k) {
+        const int now = c->mp_now ^ (int)(k & 1);
+        const int64_t it = c->mp_it + k + 1;
+        if (f32_storage(c->prec))
+            launch_mp_step<float>(c, now, it, k, stop, lim1, lim2);
+        else
+            launch_mp_step<double>(c, now, it, k, stop, lim1, lim2);
+    }
+    timer_stop(c);
+    {  // the last step's VACF (and convergence test)
+        const int64_t it = c->mp_it + S;
+        with_tracers(c->ntr, [&](auto TT) {
+            constexpr int T = TT;
+            hipLaunchKernelGGL(k_mp_reduce<3 * T>, dim3(1), dim3(kBlock), 0, c->stream,
+                               c->part + (uint64_t)nv * c->ntiles * (it & 1), c->ntiles, c->vtrace + nv * (S - 1),
+                               c->ctrl, 1, it, stop, lim1, lim2);
+        });
+    }
+    LBX_TRY(hipGetLastError());
+    LBX_TRY(hipMemcpyAsync(vt_h, c->vtrace, nv * S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return hip_status(hipMemcpyAsync(ctrl_h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+}
+
+// After a launched chunk of S steps has completed (h: the control block after
+// it): the number of its steps that ran (all, or up to the convergence step),
+// and the context's MP state moved past them.
+static int64_t mp_chunk_done(lbx_ctx* c, int64_t S, const Ctrl& h) {
+    int64_t ran = S;
+    if (h.mp_stop) ran = std::min<int64_t>(S, h.mp_conv_step - c->mp_it);
+    timer_end(c, LBX_KERNEL_MP, ran);
+    c->mp_it += ran;
+    if (ran & 1) c->mp_now ^= 1;
+    if (h.mp_stop) c->mp_stopped = true;
+    return ran;
+}
+
 int lbx_mp_run(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence, double* vacf_trace, int64_t* steps_done,
                int32_t* converged) {
     if (!c || nsteps < 0) return LBX_ERR_ARG;
     if (steps_done) *steps_done = 0;
     if (converged) *converged = 0;
-    if (!c->mp_ready) return LBX_ERR_STATE;
+    if (!c->mp_ready || c->mpp.active) return LBX_ERR_STATE;
     if (stop_on_convergence && c->ntr != 1) return LBX_ERR_ARG;  // the test is per tracer
     Ctrl h;
     LBX_TRY(read_ctrl(c, &h));
     if (h.restpart) return LBX_ERR_RESTPART;  // the reference stops in the first propagate call
     if (c->mp_stopped) return LBX_OK;
-    const uint32_t gt = c->ntiles;
     const int nv = 3 * c->ntr;
-    const double lim1 = 1.0 / (2.0 * (double)c->lx * (double)c->ly * (double)c->lz / c->Db);  // :284
-    const double lim2 = (double)1.e-12f;  // a default-real literal in the reference
     int64_t done = 0;
     std::vector<double> vt((size_t)nv * kMpChunk);
     while (done < nsteps) {
         const int64_t S = std::min<int64_t>(kMpChunk, nsteps - done);
-        timer_begin(c);
-        for (int64_t k = 0; k < S; ++k) {
-            const int now = c->mp_now ^ (int)(k & 1);
-            const int64_t it = c->mp_it + k + 1;
-            if (f32_storage(c->prec))
-                launch_mp_step<float>(c, now, it, k, stop_on_convergence, lim1, lim2);
-            else
-                launch_mp_step<double>(c, now, it, k, stop_on_convergence, lim1, lim2);
-        }
-        timer_stop(c);
-        {  // the last step's VACF (and convergence test)
-            const int64_t it = c->mp_it + S;
-            with_tracers(c->ntr, [&](auto TT) {
-                constexpr int T = TT;
-                hipLaunchKernelGGL(k_mp_reduce<3 * T>, dim3(1), dim3(kBlock), 0, c->stream,
-                                   c->part + (uint64_t)nv * gt * (it & 1), gt, c->vtrace + nv * (S - 1), c->ctrl, 1,
-                                   it, stop_on_convergence, lim1, lim2);
-            });
-        }
-        LBX_TRY(hipGetLastError());
-        LBX_TRY(hipMemcpyAsync(vt.data(), c->vtrace, nv * S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        LBX_TRY(read_ctrl(c, &h));
-        int64_t ran = S;
-        if (h.mp_stop) ran = std::min<int64_t>(S, h.mp_conv_step - c->mp_it);
-        timer_end(c, LBX_KERNEL_MP, ran);
+        LBX_TRY(mp_chunk_launch(c, S, stop_on_convergence, vt.data(), &h));
+        LBX_TRY(hip_status(hipStreamSynchronize(c->stream)));
+        const int64_t ran = mp_chunk_done(c, S, h);
         if (vacf_trace) std::memcpy(vacf_trace + nv * done, vt.data(), nv * ran * sizeof(double));
-        c->mp_it += ran;
-        if (ran & 1) c->mp_now ^= 1;
         done += ran;
         if (h.mp_stop) {
-            c->mp_stopped = true;
             if (converged) *converged = 1;
             break;
         }
     }
     if (steps_done) *steps_done = done;
+    return LBX_OK;
+}
+
+int lbx_mp_run_start(lbx_ctx* c, int64_t nsteps, int32_t stop_on_convergence) {
+    if (!c || nsteps < 1 || nsteps > kMpChunk) return LBX_ERR_ARG;
+    if (!c->mp_ready || c->mpp.active) return LBX_ERR_STATE;
+    if (stop_on_convergence && c->ntr != 1) return LBX_ERR_ARG;
+    Ctrl h;
+    LBX_TRY(read_ctrl(c, &h));
+    if (h.restpart) return LBX_ERR_RESTPART;
+    if (!c->mpp.vt_h)
+        LBX_TRY(hip_status(hipHostMalloc((void**)&c->mpp.vt_h, 3 * kMaxTracers * kMpChunk * sizeof(double))));
+    if (!c->mpp.ctrl_h) LBX_TRY(hip_status(hipHostMalloc((void**)&c->mpp.ctrl_h, sizeof(Ctrl))));
+    c->mpp.S = c->mp_stopped ? 0 : nsteps;  // after the convergence step nothing more runs
+    if (c->mpp.S) LBX_TRY(mp_chunk_launch(c, c->mpp.S, stop_on_convergence, c->mpp.vt_h, c->mpp.ctrl_h));
+    c->mpp.active = true;
+    return LBX_OK;
+}
+
+int lbx_mp_run_wait(lbx_ctx* c, double* vacf_trace, int64_t* steps_done, int32_t* converged) {
+    if (!c) return LBX_ERR_ARG;
+    if (steps_done) *steps_done = 0;
+    if (converged) *converged = 0;
+    if (!c->mpp.active) return LBX_ERR_STATE;
+    c->mpp.active = false;
+    if (!c->mpp.S) return LBX_OK;
+    LBX_TRY(hip_status(hipStreamSynchronize(c->stream)));
+    const Ctrl& h = *c->mpp.ctrl_h;
+    const int64_t ran = mp_chunk_done(c, c->mpp.S, h);
+    if (vacf_trace) std::memcpy(vacf_trace, c->mpp.vt_h, 3 * c->ntr * ran * sizeof(double));
+    if (steps_done) *steps_done = ran;
+    if (converged && h.mp_stop) *converged = 1;
     return LBX_OK;
 }
 
@@ -1075,7 +1145,7 @@ int lbx_mp_tracers(lbx_ctx* c, int32_t* ntracers) {
 
 int lbx_mp_get_tracer(lbx_ctx* c, int32_t tracer, double* P, double* Pa) {
     if (!c) return LBX_ERR_ARG;
-    if (!c->mp_ready) return LBX_ERR_STATE;
+    if (!c->mp_ready || c->mpp.active) return LBX_ERR_STATE;  // a begun chunk is waited for first
     if (tracer < 0 || tracer >= c->ntr) return LBX_ERR_ARG;
     const int K = 3 * c->ntr;
     std::vector<double> packed((size_t)K * c->Np);
@@ -1105,7 +1175,7 @@ int lbx_mp_get(lbx_ctx* c, double* P, double* Pa) { return lbx_mp_get_tracer(c, 
 
 int lbx_mp_deff_batch(lbx_ctx* c, double* deff) {
     if (!c || !deff) return LBX_ERR_ARG;
-    if (!c->mp_ready) return LBX_ERR_STATE;
+    if (!c->mp_ready || c->mpp.active) return LBX_ERR_STATE;  // a begun chunk is waited for first
     Ctrl h;
     LBX_TRY(read_ctrl(c, &h));
     for (int a = 0; a < 3 * c->ntr; ++a) deff[a] = h.deff_s[a] + h.deff_c[a];
@@ -1114,7 +1184,7 @@ int lbx_mp_deff_batch(lbx_ctx* c, double* deff) {
 
 int lbx_mp_deff(lbx_ctx* c, double deff[3]) {
     if (!c || !deff) return LBX_ERR_ARG;
-    if (!c->mp_ready) return LBX_ERR_STATE;
+    if (!c->mp_ready || c->mpp.active) return LBX_ERR_STATE;  // a begun chunk is waited for first
     Ctrl h;
     LBX_TRY(read_ctrl(c, &h));
     for (int a = 0; a < 3; ++a) deff[a] = h.deff_s[a] + h.deff_c[a];

@@ -121,6 +121,21 @@ module lbx_bind
             integer(c_int32_t), intent(out) :: converged
         end function
 
+        integer(c_int) function lbx_mp_run_start(ctx, nsteps, stop_on_convergence) bind(c, name='lbx_mp_run_start')
+            import :: c_int, c_ptr, c_int64_t, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: nsteps
+            integer(c_int32_t), value :: stop_on_convergence
+        end function
+
+        integer(c_int) function lbx_mp_run_wait(ctx, vacf_trace, steps_done, converged) bind(c, name='lbx_mp_run_wait')
+            import :: c_int, c_ptr, c_double, c_int64_t, c_int32_t
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: vacf_trace(*)
+            integer(c_int64_t), intent(out) :: steps_done
+            integer(c_int32_t), intent(out) :: converged
+        end function
+
         integer(c_int) function lbx_mp_free(ctx) bind(c, name='lbx_mp_free')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx

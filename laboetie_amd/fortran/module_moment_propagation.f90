@@ -16,14 +16,16 @@ module moment_propagation
     logical, save :: write_vacf = .true.
 
     ! propagate() runs the propagation ahead in chunks of up to mp_chunk steps
-    ! (one lbx_mp_run each, stopping at the reference's convergence step) and
-    ! hands back one VACF per call from the buffer: VACF(buf_first + k - 1) is
-    ! buf(:, k), k = 1..buf_n; buf_conv: the last step held passed the
-    ! convergence test
+    ! (each stopping at the reference's convergence step) and hands back one
+    ! VACF per call from the buffer: VACF(buf_first + k - 1) is buf(:, k),
+    ! k = 1..buf_n; buf_conv: the last step held passed the convergence test.
+    ! The next chunk is begun (lbx_mp_run_start) as soon as a chunk is in the
+    ! buffer, so the GPU steps it while this one's lines are written; it starts
+    ! at step pend_first.
     integer, parameter :: mp_chunk = 256
     real(c_double), save :: buf(3, mp_chunk)
-    integer(c_int64_t), save :: buf_first = 0, buf_n = 0
-    logical, save :: buf_conv = .false.
+    integer(c_int64_t), save :: buf_first = 0, buf_n = 0, pend_first = 0
+    logical, save :: buf_conv = .false., pending = .false.
     integer, save :: max_steps = 0  ! maximum_moment_propagation_steps (< 0: until converged)
 
 contains
@@ -75,6 +77,7 @@ contains
         buf_first = 0
         buf_n = 0
         buf_conv = .false.
+        pending = .false.
         write_vacf = getinput%log("print_vacf", .true.)
         if (write_vacf) then
             open (99, file='output/vacf.dat')
@@ -95,17 +98,19 @@ contains
         integer(i2b), intent(in) :: it
         logical, intent(out) :: is_converged
         real(c_double) :: vacf(3)
-        integer(c_int64_t) :: n, k
+        integer(c_int64_t) :: k
         integer(c_int32_t) :: converged
 
         if (it < buf_first .or. it >= buf_first + buf_n) then
-            n = mp_chunk
-            if (max_steps > 0) n = min(n, int(max_steps, c_int64_t) - it + 1)
-            n = max(n, 1_c_int64_t)
-            call lbx_check(lbx_mp_run(lbx_lattice, n, 1_c_int32_t, buf, buf_n, converged), "lbx_mp_run")
+            if (.not. pending) call start_chunk(int(it, c_int64_t))
+            ! drop_tracers calls it = 1, 2, ...: the begun chunk starts at this step
+            if (.not. pending .or. pend_first /= it) error stop "moment propagation called out of step order"
+            call lbx_check(lbx_mp_run_wait(lbx_lattice, buf, buf_n, converged), "lbx_mp_run_wait")
+            pending = .false.
             if (buf_n < 1) error stop "moment propagation called past its convergence"
             buf_first = it
             buf_conv = converged /= 0
+            if (.not. buf_conv) call start_chunk(buf_first + buf_n)
         end if
         k = it - buf_first + 1
         vacf = buf(:, k)
@@ -113,6 +118,21 @@ contains
         if (write_vacf) write (99, *) it, vacf
         is_converged = buf_conv .and. k == buf_n
     end subroutine propagate
+
+    ! begin the chunk of steps first.. (up to mp_chunk, not past
+    ! maximum_moment_propagation_steps); nothing when no step remains
+    subroutine start_chunk(first)
+        use iso_c_binding, only: c_int32_t
+        use lbx_bind
+        integer(c_int64_t), intent(in) :: first
+        integer(c_int64_t) :: n
+        n = mp_chunk
+        if (max_steps > 0) n = min(n, int(max_steps, c_int64_t) - first + 1)
+        if (n < 1) return
+        call lbx_check(lbx_mp_run_start(lbx_lattice, n, 1_c_int32_t), "lbx_mp_run_start")
+        pending = .true.
+        pend_first = first
+    end subroutine start_chunk
 
     subroutine deallocate_propagated_quantity
         use lbx_bind

@@ -63,6 +63,8 @@ SIGNATURES = {
     "lbx_mp_init_batch": (ctypes.c_int, [_p, _p, _f64, _i32, _p, _p, _p]),
     "lbx_mp_tracers": (ctypes.c_int, [_p, _p]),
     "lbx_mp_run": (ctypes.c_int, [_p, _i64, _i32, _p, _p, _p]),
+    "lbx_mp_run_start": (ctypes.c_int, [_p, _i64, _i32]),
+    "lbx_mp_run_wait": (ctypes.c_int, [_p, _p, _p, _p]),
     "lbx_mp_get": (ctypes.c_int, [_p, _p, _p]),
     "lbx_mp_get_tracer": (ctypes.c_int, [_p, _i32, _p, _p]),
     "lbx_mp_deff": (ctypes.c_int, [_p, _p]),
@@ -471,6 +473,20 @@ class Lattice:
         v = tr[:3 * nt * sd.value]
         v = v.reshape(-1, nt, 3) if self._batch else v.reshape(-1, 3)
         return dict(status=st, steps=sd.value, converged=bool(cv.value), vacf=v)
+
+    def mp_run_start(self, nsteps, stop_on_convergence=False):
+        """lbx_mp_run_start: launch 1..512 steps and return at once."""
+        _check(lib().lbx_mp_run_start(self._h, int(nsteps), 1 if stop_on_convergence else 0), "mp_run_start")
+
+    def mp_run_wait(self):
+        """lbx_mp_run_wait: the begun steps' result, as mp_run returns it."""
+        nt = max(self.mp_tracers, 1)
+        tr = np.zeros(3 * nt * 512)
+        sd, cv = _i64(), _i32()
+        _check(lib().lbx_mp_run_wait(self._h, _ptr(tr), ctypes.byref(sd), ctypes.byref(cv)), "mp_run_wait")
+        v = tr[:3 * nt * sd.value]
+        v = v.reshape(-1, nt, 3) if self._batch else v.reshape(-1, 3)
+        return dict(status=OK, steps=sd.value, converged=bool(cv.value), vacf=v)
 
     def mp_get(self, tracer=0):
         P, Pa = np.zeros(3 * self.N), np.zeros(3 * self.N)

@@ -350,3 +350,47 @@ def test_mp_coefficient_forms_give_the_same_bits(geom, prec):
     assert np.array_equal(a0, b0) and np.array_equal(va, vb) and np.array_equal(da, db)
     for (Pa_, Paa), (Pb, Pab) in zip(fa, fb):
         assert np.array_equal(Pa_, Pb) and np.array_equal(Paa, Pab)
+
+
+@pytest.mark.parametrize("name", ["bcc32", "slit_conv"])
+def test_mp_run_start_wait_equals_mp_run(name):
+    """lbx_mp_run_start / lbx_mp_run_wait in chunks of 256, 256 and the rest
+    (the drop-in's propagate pattern): the same VACF, P, Pa and D_eff bits as
+    one lbx_mp_run, the same convergence step with the test on, and the other
+    MP calls refused while a chunk is in flight."""
+    if name == "bcc32":
+        dims, nat, fext, tau, conv = (32, 32, 32), lbx.build_geometry(3, 32, 32, 32), [0.0, 0.0, 1e-6], 1.0, False
+        Db, ka, kd, steps = 0.01, lbx.ka_kd_from_dF(-1.0)[0], lbx.KD_DEFAULT, 600
+        rho0, target = 1.0, 1e-10
+    else:
+        c = Case(name)
+        dims, nat, fext, tau, conv = c.dims, c.nature, c.fext, c.tau, True
+        Db, ka, kd, steps, rho0, target = c.Db, c.ka, c.kd, 10 * int(c.g["vacf_step"][-1]), c.rho0, c.target
+    say("mp start/wait", name, dims)
+    with lbx.Lattice(dims, nat, tau) as L:
+        L.init_rest(rho0)
+        L.equilibrate(fext, target, max_steps=2000)
+        L.mp_init(fext, Db, ka, kd)
+        ref = L.mp_run(steps, stop_on_convergence=conv)
+        P0, Pa0 = L.mp_get()
+        d0 = L.mp_deff()
+        L.mp_init(fext, Db, ka, kd)
+        got, done, converged = [], 0, False
+        L.mp_run_start(min(256, steps), stop_on_convergence=conv)
+        with pytest.raises(lbx.LbxError):  # a chunk is in flight
+            L.mp_deff()
+        while True:
+            r = L.mp_run_wait()
+            got.append(r["vacf"])
+            done += r["steps"]
+            converged = r["converged"]
+            if converged or done >= steps:
+                break
+            L.mp_run_start(min(256, steps - done), stop_on_convergence=conv)
+        assert done == ref["steps"] and converged == ref["converged"]
+        assert np.array_equal(np.concatenate(got), ref["vacf"])
+        P1, Pa1 = L.mp_get()
+        assert np.array_equal(P0, P1) and np.array_equal(Pa0, Pa1) and np.array_equal(d0, L.mp_deff())
+        if conv:  # after the convergence step a begun chunk runs nothing
+            L.mp_run_start(5, stop_on_convergence=True)
+            assert L.mp_run_wait()["steps"] == 0
