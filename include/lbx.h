@@ -349,12 +349,18 @@ int lbx_addressing(lbx_ctx* ctx);
  * s(r+c_l, inv l) (module_moment_propagation.f90:218-229): LBX_MP_RECOMPUTED
  * keeps each node's density and flow (32 bytes) and recomputes them in the
  * step with the set-up's own arithmetic; LBX_MP_STORED keeps the 18
- * probabilities of each node (144 bytes in FP64, 72 in the FP32 forms).
- * Both give the same bits.  Default: recomputed in the FP64 form, stored in
- * the FP32 forms (where the recomputation costs more than the 72 bytes).  Takes effect at the next lbx_mp_init*.  No
- * counterpart in the reference. */
+ * probabilities of each node (144 bytes in FP64, 72 in the FP32 forms);
+ * LBX_MP_MIXED keeps the 10 of the directions with c_z != 0 (whose neighbours
+ * lie a plane away) and recomputes the 8 others from the density and flow.
+ * All give the same bits.  LBX_MP_AUTO (default): recomputed for two or more
+ * FP64 tracers (their coefficients are shared, and 112 bytes less per node
+ * pays), stored for one tracer and in the FP32 forms (where the recomputation
+ * costs more than the bytes it saves).  Takes effect at the next
+ * lbx_mp_init*.  No counterpart in the reference. */
+#define LBX_MP_AUTO       (-1)
 #define LBX_MP_STORED     0
 #define LBX_MP_RECOMPUTED 1
+#define LBX_MP_MIXED      2
 int lbx_set_mp_coefficients(lbx_ctx* ctx, int32_t form);
 
 /* Fluid-node count of the context (the MFLUPS denominator). */

@@ -99,10 +99,8 @@ struct lbx_ctx {
     float* win32 = nullptr;      // FP32-storage forms
     float* ustar32 = nullptr;
     double* rv = nullptr;        // (rho, v) per node: the step recomputes the incoming probabilities from it
-    bool mp_recompute = true;    // the next MP set-up keeps rv (true) or the incoming probabilities win / win32
-                                 // (default: rv in FP64, where it saves 112 of 144 bytes per node; win32 in
-                                 // the FP32 forms, whose 72 bytes cost less than the recomputation)
-    bool mp_rv_on = false;       // the current MP state keeps rv
+    int32_t mp_coef = LBX_MP_AUTO;        // where the next MP set-up's step takes its incoming probabilities
+    int32_t mp_coef_on = LBX_MP_STORED;   // ... and the current MP state's (kCoef* in the kernels)
     double* rem = nullptr;
     double* P[2] = {nullptr, nullptr};
     double* Pa[2] = {nullptr, nullptr};
@@ -442,8 +440,9 @@ void launch_mp_prep(lbx_ctx* c, const double fext[3], double eps) {
     a.j = c->j;
     a.nodes = c->nodes;
     a.map = c->map;
-    a.win = c->mp_rv_on ? nullptr : win_buf<W>(c);
-    a.rv = c->mp_rv_on ? c->rv : nullptr;
+    a.win = c->mp_coef_on == LBX_MP_RECOMPUTED ? nullptr : win_buf<W>(c);
+    a.win_z = c->mp_coef_on == LBX_MP_MIXED ? 1 : 0;
+    a.rv = c->mp_coef_on == LBX_MP_STORED ? nullptr : c->rv;
     a.rem = c->rem;
     a.ustar = ustar_buf<W>(c);
     a.P0 = c->P[0];
@@ -494,7 +493,8 @@ void launch_mp_step(lbx_ctx* c, int now, int64_t it, int64_t k, int32_t stop_on_
     a.check_conv = stop_on_convergence;
     a.lim1 = lim1;
     a.lim2 = lim2;
-    a.rv = c->mp_rv_on ? c->rv : nullptr;
+    a.win = c->mp_coef_on == LBX_MP_RECOMPUTED ? nullptr : win_buf<W>(c);
+    a.rv = c->mp_coef_on == LBX_MP_STORED ? nullptr : c->rv;
     a.lambda = c->lambda;
     a.it = it;
     a.ctrl = c->ctrl;
@@ -508,8 +508,13 @@ void launch_mp_step(lbx_ctx* c, int now, int64_t it, int64_t k, int32_t stop_on_
     a.tile_order = c->mp_tile_order;
     with_tracers(c->ntr, [&](auto TT) {
         constexpr int T = TT;
-        auto kern = c->mp_rv_on ? (c->wide ? k_mp_step<true, T, W, true> : k_mp_step<false, T, W, true>)
-                                    : (c->wide ? k_mp_step<true, T, W, false> : k_mp_step<false, T, W, false>);
+        auto pick = [&](auto CFT) {
+            constexpr int CF = decltype(CFT)::value;
+            return c->wide ? k_mp_step<true, T, W, CF> : k_mp_step<false, T, W, CF>;
+        };
+        auto kern = c->mp_coef_on == LBX_MP_RECOMPUTED ? pick(std::integral_constant<int, kCoefRecomputed>{})
+                    : c->mp_coef_on == LBX_MP_MIXED   ? pick(std::integral_constant<int, kCoefMixed>{})
+                                                      : pick(std::integral_constant<int, kCoefStored>{});
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
     });
 }
@@ -587,7 +592,6 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     c->dly.set((uint32_t)ly);
     c->prec = precision;
     c->floor_window = f32_storage(precision) ? 1000 : 0;
-    c->mp_recompute = !f32_storage(precision);
     c->check_every = f32_storage(precision) ? kCheckEvery : 1;
     c->tau = tau;
     relaxation_coeffs(tau, &c->A, &c->B, &c->C);
@@ -960,12 +964,17 @@ static int mp_setup(lbx_ctx* c, const double fext[3], double Db, int32_t ntr, co
     const uint64_t np3 = 3 * (uint64_t)ntr * c->Np;
     int st = LBX_OK;
     const bool shift = f32_storage(c->prec);
-    c->mp_rv_on = c->mp_recompute;
-    if (!st && !shift && !c->mp_recompute) st = dalloc(&c->win, 18 * c->Np);
+    // the automatic form: the recomputation pays for several FP64 tracers (measured at 128^3 and
+    // 256^3); for one tracer, and in the FP32 forms, the stored probabilities are as fast or faster
+    c->mp_coef_on = c->mp_coef != LBX_MP_AUTO ? c->mp_coef
+                                             : (!shift && ntr >= 2 ? LBX_MP_RECOMPUTED : LBX_MP_STORED);
+    // stored probabilities: 18 per node, or the kZDirs with c_z != 0 in the mixed form
+    const uint64_t nwin = c->mp_coef_on == LBX_MP_STORED ? 18 : (c->mp_coef_on == LBX_MP_MIXED ? kZDirs : 0);
+    if (!st && !shift && nwin) st = dalloc(&c->win, nwin * c->Np);
     if (!st && !shift) st = dalloc(&c->ustar, 3 * c->Np);
-    if (!st && shift && !c->mp_recompute) st = dalloc(&c->win32, 18 * c->Np);
+    if (!st && shift && nwin) st = dalloc(&c->win32, nwin * c->Np);
     if (!st && shift) st = dalloc(&c->ustar32, 3 * c->Np);
-    if (!st && c->mp_recompute) st = dalloc(&c->rv, 4 * c->Np);
+    if (!st && c->mp_coef_on != LBX_MP_STORED) st = dalloc(&c->rv, 4 * c->Np);
     if (!st) st = dalloc(&c->rem, c->Np);
     if (!st) st = dalloc(&c->P[0], np3);
     if (!st) st = dalloc(&c->P[1], np3);
@@ -1376,8 +1385,8 @@ int lbx_addressing(lbx_ctx* c) { return c ? (c->wide ? 1 : 0) : -1; }
 int64_t lbx_fluid_nodes(lbx_ctx* c) { return c ? (int64_t)c->Nf : -1; }
 
 int lbx_set_mp_coefficients(lbx_ctx* c, int32_t form) {
-    if (!c || (form != LBX_MP_STORED && form != LBX_MP_RECOMPUTED)) return LBX_ERR_ARG;
-    c->mp_recompute = form == LBX_MP_RECOMPUTED;
+    if (!c || form < LBX_MP_AUTO || form > LBX_MP_MIXED) return LBX_ERR_ARG;
+    c->mp_coef = form;
     return LBX_OK;
 }
 

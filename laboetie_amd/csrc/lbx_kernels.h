@@ -727,6 +727,17 @@ __device__ __forceinline__ double scatter_prob(double rho, double vx, double vy,
     return as_coef<W>(nt / rho - a0(L) + lambda * a0(L) * 0.5);
 }
 
+// Where the MP step takes the incoming probabilities s(r+c_l, inv l) from
+// (the C-ABI's LBX_MP_STORED / _RECOMPUTED / _MIXED): the 18 stored per node,
+// all 18 recomputed from the neighbours' (rho, v), or the 10 with c_z != 0
+// stored (those neighbours are a plane away, outside the LDS window, where
+// reading (rho, v) costs more than a stored value) and the 8 others
+// recomputed from the window.
+constexpr int kCoefStored = 0, kCoefRecomputed = 1, kCoefMixed = 2;
+constexpr int kZDirs = 10;  // directions with c_z != 0: 5, 6, 11..18
+// index of direction L (c_z != 0) among them, and of its pair (L, L+1 for odd L)
+constexpr int zdir(int L) { return L <= 6 ? L - 5 : L - 9; }
+
 template <typename W>
 struct MpPrepArgs {
     const double* __restrict__ rho;
@@ -734,6 +745,7 @@ struct MpPrepArgs {
     const uint32_t* __restrict__ nodes;
     const uint32_t* __restrict__ map;
     W* __restrict__ win = nullptr;       // incoming probabilities (NULL: not kept, the step recomputes them)
+    int32_t win_z = 0;                   // win holds only the kZDirs directions with c_z != 0
     double* __restrict__ rv = nullptr;   // (rho, v) of each node, 4 doubles (NULL: not kept)
     double* __restrict__ rem;
     W* __restrict__ ustar;
@@ -803,7 +815,12 @@ __device__ __forceinline__ void mp_prep_tile(const MpPrepArgs<W>& a, uint32_t ti
                 const double rn = a.rho[q];
                 const double wx = a.j[q] + a.fx, wy = a.j[a.Np + q] + a.fy, wz = a.j[2 * a.Np + q] + a.fz;
                 const double sp = scatter_prob<Li, W>(rn, wx, wy, wz, a.lambda);
-                if (a.win) a.win[comp_index(18, L - 1, i, a.Np)] = (W)sp;
+                if (a.win) {
+                    if (!a.win_z)
+                        a.win[comp_index(18, L - 1, i, a.Np)] = (W)sp;
+                    else if constexpr (cz != 0)
+                        a.win[comp_index(kZDirs, zdir(L), i, a.Np)] = (W)sp;
+                }
 #pragma unroll
                 for (int t = 0; t < T; ++t) {
                     if constexpr (cx != 0) p1[3 * t + 0] = p1[3 * t + 0] + sgn<-cx>(sp) * a.tp.bw[t];
@@ -811,7 +828,10 @@ __device__ __forceinline__ void mp_prep_tile(const MpPrepArgs<W>& a, uint32_t ti
                     if constexpr (cz != 0) p1[3 * t + 2] = p1[3 * t + 2] + sgn<-cz>(sp) * a.tp.bw[t];
                 }
             } else if (a.win) {
-                a.win[comp_index(18, L - 1, i, a.Np)] = W(0);
+                if (!a.win_z)
+                    a.win[comp_index(18, L - 1, i, a.Np)] = W(0);
+                else if constexpr (cz != 0)
+                    a.win[comp_index(kZDirs, zdir(L), i, a.Np)] = W(0);
             }
         });
         a.rem[i] = remv;
@@ -873,13 +893,14 @@ struct MpStepArgs {
 // VACF partial to a.partials[K*tile ...]; first_block also reduces the
 // previous step's nparts partials (a.prev_partials) into VACF(it-1).
 //
-// RV: the incoming probabilities s(r+c_l, inv l) are recomputed from the
-// neighbours' (rho, v) (scatter_prob, the set-up's own arithmetic: the same
-// bits) instead of read from win: 32 bytes per node kept and staged in the
-// LDS window beside P, in place of the 18 probabilities of each node.
-template <int BLOCK, bool Wide, int T, typename W, bool RV = false>
+// CF (kCoef*): recomputed probabilities come from the neighbours' (rho, v)
+// (scatter_prob, the set-up's own arithmetic: the same bits): 32 bytes per
+// node kept and staged in the LDS window beside P; stored ones from win (18 or,
+// mixed, kZDirs per node, read in pairs).
+template <int BLOCK, bool Wide, int T, typename W, int CF = kCoefStored>
 __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t tile, bool first_block,
                                              uint32_t nparts) {
+    constexpr bool RV = CF != kCoefStored;  // (rho, v) staged in the window
     constexpr int K = 3 * T;  // moment components of the batch
     constexpr int WIN = BLOCK + 2 * kMpHalo;
     __shared__ double lds[BLOCK / 64 * K];
@@ -902,7 +923,7 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
                                   a.lim2, lds);
     const uint32_t Np = (uint32_t)a.Np;
     const View<Wide> Pn(a.Pn, K * a.Np);
-    const View<Wide, W> win(a.win, 18 * a.Np), us(a.ustar, 3 * a.Np);
+    const View<Wide, W> win(a.win, (CF == kCoefMixed ? kZDirs : 18) * a.Np), us(a.ustar, 3 * a.Np);
     const View<Wide> rem(a.rem, a.Np);
     Neighbours nb;
     W uw[3] = {W(0), W(0), W(0)};
@@ -942,13 +963,15 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
         Vec2<W> wp;  // the probabilities of directions L, L+1 (L odd) from one paired load
         static_for<1, kQ>([&](auto I) {
             constexpr int L = I;
-            if constexpr (!RV && (L & 1)) wp = win.ld2((uint32_t)((L - 1) / 2) * Np + i);
+            constexpr bool stored = CF == kCoefStored || (CF == kCoefMixed && kCZ[L] != 0);
+            if constexpr (CF == kCoefStored && (L & 1)) wp = win.ld2((uint32_t)((L - 1) / 2) * Np + i);
+            if constexpr (CF == kCoefMixed && stored && (L & 1)) wp = win.ld2((uint32_t)(zdir(L) / 2) * Np + i);
             const uint32_t q = nb.q[L];
             if (q != kSolidSlot) {
                 double pq[K];
                 double w;
                 const uint32_t d = (uint32_t)((int64_t)q - w0);
-                if constexpr (RV) {
+                if constexpr (!stored) {
                     Vec2<double> r0, r1;  // (rho, vx), (vy, vz) of the neighbour
                     if (d < (uint32_t)WIN) {
                         load_comps<K>(pw, WIN, d, pq);
@@ -1004,9 +1027,9 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     block_partial<K, BLOCK>(vp, a.partials + (uint64_t)K * tile, lds);
 }
 
-template <bool Wide, int T, typename W, bool RV>
+template <bool Wide, int T, typename W, int CF>
 __global__ __launch_bounds__(kBlock, LBX_MP_WAVES) void k_mp_step(MpStepArgs<W> a) {
-    mp_step_tile<kBlock, Wide, T, W, RV>(a, a.tile_order ? a.tile_order[xcd_tile()] : xcd_tile(), blockIdx.x == 0,
+    mp_step_tile<kBlock, Wide, T, W, CF>(a, a.tile_order ? a.tile_order[xcd_tile()] : xcd_tile(), blockIdx.x == 0,
                                          gridDim.x);
 }
 

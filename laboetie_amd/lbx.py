@@ -21,7 +21,7 @@ NOALLOC_MAX_NODES = 320000
 PREC_FP64, PREC_FP32_SHIFT, PREC_FP32_STORE = 0, 1, 2
 EXIT_CONVERGED, EXIT_CAP, EXIT_FLOOR = 1, 2, 3
 KERNEL_LB, KERNEL_MP = 0, 1
-MP_STORED, MP_RECOMPUTED = 0, 1
+MP_AUTO, MP_STORED, MP_RECOMPUTED, MP_MIXED = -1, 0, 1, 2
 AUTOTUNE_TIMES = 10  # LBXB_AUTOTUNE_TIMES
 
 _p = ctypes.c_void_p
@@ -351,10 +351,11 @@ class Lattice:
         return int(lib().lbx_addressing(self._h))
 
     def set_mp_coefficients(self, form):
-        """MP_RECOMPUTED (default in FP64): the MP step recomputes the incoming
-        scattering probabilities from each node's density and flow;
-        MP_STORED (default in the FP32 forms): keeps them (18 per node).  Same bits; from the next
-        mp_init."""
+        """MP_RECOMPUTED: the MP step recomputes the incoming scattering
+        probabilities from each node's density and flow; MP_STORED: keeps them
+        (18 per node); MP_MIXED: keeps the 10 with c_z != 0 and recomputes the
+        others; MP_AUTO (default): recomputed for several FP64 tracers, stored
+        otherwise.  Same bits; from the next mp_init."""
         _check(lib().lbx_set_mp_coefficients(self._h, int(form)), "set_mp_coefficients")
 
     @property

@@ -322,8 +322,8 @@ def test_equilibration_in_pieces_equals_one_run(prec):
 def test_mp_coefficient_forms_give_the_same_bits(geom, prec):
     """The moment-propagation step with the incoming scattering
     probabilities recomputed from the neighbours' density and flow (the FP64
-    default) and with them read from the 18 stored per node (the FP32
-    default): three adsorbing
+    default), read from the 18 stored per node (the FP32 default), and the
+    10 of the z-directions stored with the others recomputed: three adsorbing
     tracers on a developing flow, VACF, P, Pa and D_eff bit-identical."""
     if geom == "bcc64":
         dims, nat, fext, tau = (64, 64, 64), lbx.build_geometry(3, 64, 64, 64), [0.0, 0.0, 1e-6], 1.0
@@ -339,17 +339,19 @@ def test_mp_coefficient_forms_give_the_same_bits(geom, prec):
     with lbx.Lattice(dims, nat, tau, precision=prec) as L:
         L.init_rest(1.0)
         L.equilibrate(fext, 1e-10, max_steps=300)
-        for form in (lbx.MP_STORED, lbx.MP_RECOMPUTED):
+        forms = (lbx.MP_STORED, lbx.MP_RECOMPUTED, lbx.MP_MIXED)
+        for k, form in enumerate(forms):
             L.set_mp_coefficients(form)
             v0 = L.mp_init_batch(fext, 0.01, ka, kd)
-            L.set_mp_coefficients(lbx.MP_STORED if form else lbx.MP_RECOMPUTED)  # from the next init only
+            L.set_mp_coefficients(forms[(k + 1) % 3])  # from the next init only
             v = L.mp_run(150)["vacf"]
             fields = [L.mp_get(t) for t in range(3)]
             out.append((v0, v, fields, L.mp_deff()))
-    (a0, va, fa, da), (b0, vb, fb, db) = out
-    assert np.array_equal(a0, b0) and np.array_equal(va, vb) and np.array_equal(da, db)
-    for (Pa_, Paa), (Pb, Pab) in zip(fa, fb):
-        assert np.array_equal(Pa_, Pb) and np.array_equal(Paa, Pab)
+    (a0, va, fa, da) = out[0]
+    for (b0, vb, fb, db) in out[1:]:
+        assert np.array_equal(a0, b0) and np.array_equal(va, vb) and np.array_equal(da, db)
+        for (Pa_, Paa), (Pb, Pab) in zip(fa, fb):
+            assert np.array_equal(Pa_, Pb) and np.array_equal(Paa, Pab)
 
 
 @pytest.mark.parametrize("name", ["bcc32", "slit_conv"])
