@@ -92,6 +92,42 @@ def test_batch_block128_no_graphs_and_mixed_rounds(prec):
 
 
 @pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT, lbx.PREC_FP32_STORE])
+@pytest.mark.parametrize("mode", ["graph", "persistent"])
+def test_batch_block64(prec, mode):
+    """64-node blocks (one wave, four blocks per 256-node tile; a point of the
+    autotuner's search), in graph-replayed and persistent launches: the LB
+    fields and exit steps bit-identical to 256-node blocks, VACF within 1e-12
+    (its partials sum per block)."""
+    cases = [Case(n) for n in ["bcc12_ads", "random_3d", "cyl9_tau08"]]
+    mp_steps = 100
+    ref, _ = run_batch(cases, prec, mp_steps)
+    if mode == "graph":
+        got, _ = run_batch(cases, prec, mp_steps, block=64)
+    else:
+        cap = sum(((int((c.nature == 0).sum()) + 255) // 256) * 256 for c in cases) + 4096
+        box = sum(int(np.prod(c.dims)) for c in cases) + 4096
+        with lbx.Batch(cap, box, len(cases) + 2, precision=prec) as B:
+            B.set_block(64, 64)
+            B.set_launch_mode(lbx.LAUNCH_PERSISTENT)
+            ids = [B.add(c.dims, c.nature, c.tau, c.rho0, c.fext, c.target, c.Db, c.ka, c.kd, mp_steps,
+                         max_lb_steps=20000) for c in cases]
+            rounds = 0
+            while B.round(64, 64) > 0:
+                rounds += 1
+                assert rounds < 2000
+            got = []
+            for iid in ids:
+                g = B.result(iid, vacf_rows=mp_steps + 1, flow=True)
+                g.update(B.info(iid))
+                got.append(g)
+    for c, r, g in zip(cases, ref, got):
+        assert g["lb_steps"] == r["lb_steps"]
+        assert np.array_equal(g["rho"], r["rho"]) and np.array_equal(g["j"], r["j"])
+        scale = np.abs(r["vacf"][0]).max()
+        assert np.all(np.abs(g["vacf"] - r["vacf"]) <= 1e-12 * scale)
+
+
+@pytest.mark.parametrize("prec", [lbx.PREC_FP64, lbx.PREC_FP32_SHIFT, lbx.PREC_FP32_STORE])
 def test_batch_rounds_of_many_lengths_same_bits(prec):
     """Graph-replayed rounds of 1, 2, ..., 90 steps (more distinct round
     lengths than the captured-graph cache keeps, so it starts over) give the
