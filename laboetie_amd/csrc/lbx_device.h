@@ -306,6 +306,19 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;  // lane 0 holds the sum, fixed tree
 }
 
+// Lane k of the wave receives lane k+1's value (S = +1, DPP wave_shl:1) or
+// lane k-1's (S = -1, wave_shr:1); lane 63 (S = +1) and lane 0 (S = -1), and
+// a lane whose source lane is inactive, receive 0.  Register moves only.
+template <int S>
+__device__ __forceinline__ double lane_shift(double v) {
+    static_assert(S == 1 || S == -1, "one lane up or down");
+    constexpr int ctrl = S > 0 ? 0x130 : 0x138;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, ctrl, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), ctrl, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Deterministic block sum of 3 values; the result is valid in thread 0.
 template <int BLOCK>
 __device__ __forceinline__ void block_sum3(double v[3], double* lds /* [BLOCK/64*3] */) {

@@ -901,13 +901,32 @@ template <int BLOCK, bool Wide, int T, typename W, int CF = kCoefStored>
 __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t tile, bool first_block,
                                              uint32_t nparts) {
     constexpr bool RV = CF != kCoefStored;  // (rho, v) staged in the window
+    constexpr bool RVZ = CF == kCoefRecomputed;  // the directions with c_z != 0 recomputed too
     constexpr int K = 3 * T;  // moment components of the batch
-    constexpr int WIN = BLOCK + 2 * kMpHalo;
-    __shared__ double lds[BLOCK / 64 * K];
+    // three tracers with (rho, v) in the window: 4 fewer halo nodes a side
+    // leave room for the edge-lane records within a third of the CU's LDS
+    // Edge-lane records and the early z-neighbour gathers below: where the
+    // z-directions read stored probabilities (measured: with (rho, v)
+    // recomputed for them the added loads before the barrier cost more than
+    // the records save)
+    constexpr bool XCH = !RVZ;
+    constexpr int HALO = (XCH && RV && T == 3) ? kMpHalo - 4 : kMpHalo;
+    constexpr int WIN = BLOCK + 2 * HALO;
+    constexpr int NW = BLOCK / 64;  // waves per block
+    // Edge-lane records, [wave boundary][side][plane][XR]: P of a wave's first
+    // (side 0) or last (side 1) lane's neighbour along 5 (plane 0) or 6
+    // (plane 1) and, RVZ, the probability its neighbour lane in the adjacent
+    // wave takes from it.  The array holds the VACF partial's wave sums after
+    // the step's last read of a record.
+    constexpr int XR = K + (RVZ ? 1 : 0);
+    constexpr int XN = (XCH && NW > 1) ? (NW - 1) * 4 * XR : 0;
+    constexpr int LN = NW * K > XN ? NW * K : XN;
+    __shared__ double lds[LN];
     __shared__ double pwin[K * WIN];  // P(now) of packed nodes [w0, w0 + WIN), comp_index layout
     __shared__ double rvwin[RV ? 4 * WIN : 2];  // RV: (rho, v) of the same nodes
     const uint32_t i = tile * BLOCK + threadIdx.x;
     const bool valid = i < a.Nf;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     // Each wave's work is a chain of dependent memory round trips (stop flag,
     // node word, neighbour indices, neighbour values), so independent
     // requests are issued together: the node word with the stop flag, the
@@ -918,9 +937,11 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     // convergence test: a stop set here ends the run at step it-1 (this
     // launch's other blocks may have started step it already; they write only
     // the other buffer of P, and every later launch returns at once)
-    if (a.prev_partials && first_block)
+    if (a.prev_partials && first_block) {
         mp_reduce_block<K, BLOCK>(a.prev_partials, nparts, a.prev_vtrace, a.ctrl, 1, a.it - 1, a.check_conv, a.lim1,
                                   a.lim2, lds);
+        if constexpr (XCH && NW > 1) __syncthreads();  // its last read of lds before the records are written there
+    }
     const uint32_t Np = (uint32_t)a.Np;
     const View<Wide> Pn(a.Pn, K * a.Np);
     const View<Wide, W> win(a.win, (CF == kCoefMixed ? kZDirs : 18) * a.Np), us(a.ustar, 3 * a.Np);
@@ -933,21 +954,91 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
         load_comps<3>(us, Np, i, uw);
         rm = rem.ld(i);
     }
-    // The block's nodes and kMpHalo on either side: neighbours along x, and
+    // The block's nodes and HALO on either side: neighbours along x, and
     // along y where a row holds fewer fluid nodes than that, are read from LDS
-    const int64_t w0 = (int64_t)tile * BLOCK - kMpHalo;
+    const int64_t w0 = (int64_t)tile * BLOCK - HALO;
     const LdsView pw{pwin};
     const View<Wide> rvv(a.rv, 4 * a.Np);
-    for (int n = threadIdx.x; n < WIN; n += BLOCK) {
-        const int64_t g = w0 + n;
-        if (g >= 0 && g < (int64_t)Np) {
-            double v[K];
-            load_comps<K>(Pn, Np, (uint32_t)g, v);
-            store_comps<K>(pw, WIN, (uint32_t)n, v);
-            if constexpr (RV) {
-                ((Vec2<double>*)rvwin)[2 * n] = rvv.ld2(2 * (uint32_t)g);
-                ((Vec2<double>*)rvwin)[2 * n + 1] = rvv.ld2(2 * (uint32_t)g + 1);
+    // The node's neighbours along 5 (0,0,1) and 6 (0,0,-1), whose values the
+    // directions 11..14 reuse (below): gathered here, with the window, where
+    // they lie outside it or the lane is a wave's first or last.
+    double zc[2][K];         // their P (0 if solid)
+    Vec2<double> zr[2][2];   // RVZ: their (rho, v)
+    bool zearly[2] = {false, false};
+    const bool edge_lane = XCH && NW > 1 && (lane == 0u || lane == 63u);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) zc[s][k] = 0.0;
+        zr[s][0] = Vec2<double>{0.0, 0.0};
+        zr[s][1] = Vec2<double>{0.0, 0.0};
+        if (valid) {
+            const uint32_t q = nb.q[5 + s];
+            zearly[s] = XCH && q != kSolidSlot && ((uint32_t)((int64_t)q - w0) >= (uint32_t)WIN || edge_lane);
+            if (zearly[s]) {
+                load_comps<K>(Pn, Np, q, zc[s]);
+                if constexpr (RVZ) {
+                    zr[s][0] = rvv.ld2(2 * q);
+                    zr[s][1] = rvv.ld2(2 * q + 1);
+                }
             }
+        }
+    }
+    {
+        // every slot's loads go out before the first LDS write waits on them
+        constexpr int NIT = (WIN + BLOCK - 1) / BLOCK;
+        double fv[NIT][K];
+        Vec2<double> fr[NIT][2];
+        bool in[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int n = (int)threadIdx.x + it * BLOCK;
+            const int64_t g = w0 + n;
+            in[it] = n < WIN && g >= 0 && g < (int64_t)Np;
+            if (in[it]) {
+                load_comps<K>(Pn, Np, (uint32_t)g, fv[it]);
+                if constexpr (RV) {
+                    fr[it][0] = rvv.ld2(2 * (uint32_t)g);
+                    fr[it][1] = rvv.ld2(2 * (uint32_t)g + 1);
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int n = (int)threadIdx.x + it * BLOCK;
+            if (in[it]) {
+                store_comps<K>(pw, WIN, (uint32_t)n, fv[it]);
+                if constexpr (RV) {
+                    ((Vec2<double>*)rvwin)[2 * n] = fr[it][0];
+                    ((Vec2<double>*)rvwin)[2 * n + 1] = fr[it][1];
+                }
+            }
+        }
+    }
+    // records of the lanes at the block's inner wave boundaries: lane 0 of
+    // wave w serves lane 63 of wave w-1 (directions 11, 13), lane 63 of wave w
+    // serves lane 0 of wave w+1 (12, 14)
+    if constexpr (XCH && NW > 1) {
+        if (valid && ((lane == 0u && wave > 0u) || (lane == 63u && wave + 1u < (uint32_t)NW))) {
+            const uint32_t side = lane == 0u ? 0u : 1u, bnd = lane == 0u ? wave - 1u : wave;
+            static_for<0, 2>([&](auto S) {
+                constexpr int s = S;
+                double* r = lds + ((bnd * 2u + side) * 2u + (uint32_t)s) * XR;
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = zc[s][k];
+                if constexpr (RVZ) {
+                    double w = 0.0;
+                    if (zearly[s]) {
+                        if (side == 0u)
+                            w = scatter_prob<kInv[11 + 2 * s], W>(zr[s][0][0], zr[s][0][1], zr[s][1][0],
+                                                                  zr[s][1][1], a.lambda);
+                        else
+                            w = scatter_prob<kInv[12 + 2 * s], W>(zr[s][0][0], zr[s][0][1], zr[s][1][0],
+                                                                  zr[s][1][1], a.lambda);
+                    }
+                    r[K] = w;
+                }
+            });
         }
     }
     __syncthreads();
@@ -957,38 +1048,105 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     if (valid) {
         const View<Wide> Px(a.Px, K * a.Np);
         double p[K], acc[K];
-        load_comps<K>(pw, WIN, (uint32_t)(kMpHalo + threadIdx.x), p);
+        load_comps<K>(pw, WIN, (uint32_t)(HALO + threadIdx.x), p);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] = 0.0;
+        // The node's x-neighbour in its row is packed node i+1 (i-1) whenever
+        // that neighbour is fluid and not across the periodic wrap: held by the
+        // next (previous) lane of the wave, or by the first (last) lane of the
+        // next (previous) wave of the block.  Its neighbour one plane up (down)
+        // is then the node's neighbour along (+-1, 0, +-1): directions 11..14
+        // take P (and the probability from (rho, v)) of that lane's neighbour
+        // along 5 or 6, moved across the wave or from its edge-lane record,
+        // instead of gathering them again; the sum keeps its order and bits.
+        const bool adj_r = nb.q[1] == i + 1u;
+        const bool adj_l = i > 0u && nb.q[2] == i - 1u;
         Vec2<W> wp;  // the probabilities of directions L, L+1 (L odd) from one paired load
         static_for<1, kQ>([&](auto I) {
             constexpr int L = I;
             constexpr bool stored = CF == kCoefStored || (CF == kCoefMixed && kCZ[L] != 0);
+            // 11, 12 from 5 of lane +-1; 13, 14 from 6 of lane +-1; 5, 6 kept
+            constexpr int zsrc = (L >= 11 && L <= 14) ? (kCZ[L] > 0 ? 0 : 1) : -1;
+            constexpr int zkeep = L == 5 ? 0 : (L == 6 ? 1 : -1);
             if constexpr (CF == kCoefStored && (L & 1)) wp = win.ld2((uint32_t)((L - 1) / 2) * Np + i);
             if constexpr (CF == kCoefMixed && stored && (L & 1)) wp = win.ld2((uint32_t)(zdir(L) / 2) * Np + i);
+            double sh[K];          // moved across the wave while every valid lane is active
+            Vec2<double> sr[2];
+            if constexpr (zsrc >= 0) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) sh[k] = lane_shift<kCX[L]>(zc[zsrc][k]);
+                if constexpr (!stored)
+                    for (int h = 0; h < 2; ++h)
+                        sr[h] = Vec2<double>{lane_shift<kCX[L]>(zr[zsrc][h][0]), lane_shift<kCX[L]>(zr[zsrc][h][1])};
+            }
             const uint32_t q = nb.q[L];
             if (q != kSolidSlot) {
                 double pq[K];
-                double w;
-                const uint32_t d = (uint32_t)((int64_t)q - w0);
-                if constexpr (!stored) {
-                    Vec2<double> r0, r1;  // (rho, vx), (vy, vz) of the neighbour
+                double w = 0.0;
+                Vec2<double> r0, r1;  // (rho, vx), (vy, vz) of the neighbour (RV directions)
+                int how = 0;          // 0: gather, 1: from the lane beside, 2: from an edge-lane record
+                if constexpr (zsrc >= 0) {
+                    constexpr bool right = kCX[L] > 0;
+                    if (right ? adj_r : adj_l) {
+                        if (!(right ? lane == 63u : lane == 0u))
+                            how = 1;
+                        else if (XCH && (right ? wave + 1u < (uint32_t)NW : wave > 0u))
+                            how = 2;
+                    }
+                    if (how == 1) {
+#pragma unroll
+                        for (int k = 0; k < K; ++k) pq[k] = sh[k];
+                        if constexpr (!stored) {
+                            r0 = sr[0];
+                            r1 = sr[1];
+                        }
+                    } else if (how == 2) {
+                        const double* r = lds + (((right ? wave : wave - 1u) * 2u + (right ? 0u : 1u)) * 2u +
+                                                 (uint32_t)zsrc) * XR;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) pq[k] = r[k];
+                        if constexpr (!stored) w = r[K];
+                    }
+                }
+                if constexpr (zkeep >= 0) {
+                    if (zearly[zkeep]) {
+#pragma unroll
+                        for (int k = 0; k < K; ++k) pq[k] = zc[zkeep][k];
+                        if constexpr (!stored) {
+                            r0 = zr[zkeep][0];
+                            r1 = zr[zkeep][1];
+                        }
+                        how = 1;
+                    }
+                }
+                if (how == 0) {
+                    const uint32_t d = (uint32_t)((int64_t)q - w0);
                     if (d < (uint32_t)WIN) {
                         load_comps<K>(pw, WIN, d, pq);
-                        r0 = ((const Vec2<double>*)rvwin)[2 * d];
-                        r1 = ((const Vec2<double>*)rvwin)[2 * d + 1];
+                        if constexpr (!stored) {
+                            r0 = ((const Vec2<double>*)rvwin)[2 * d];
+                            r1 = ((const Vec2<double>*)rvwin)[2 * d + 1];
+                        }
                     } else {
                         load_comps<K>(Pn, Np, q, pq);
-                        r0 = rvv.ld2(2 * q);
-                        r1 = rvv.ld2(2 * q + 1);
+                        if constexpr (!stored) {
+                            r0 = rvv.ld2(2 * q);
+                            r1 = rvv.ld2(2 * q + 1);
+                        }
                     }
-                    w = scatter_prob<kInv[L], W>(r0[0], r0[1], r1[0], r1[1], a.lambda);
+                }
+                if constexpr (zkeep >= 0) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) zc[zkeep][k] = pq[k];
+                    if constexpr (!stored) {
+                        zr[zkeep][0] = r0;
+                        zr[zkeep][1] = r1;
+                    }
+                }
+                if constexpr (!stored) {
+                    if (how != 2) w = scatter_prob<kInv[L], W>(r0[0], r0[1], r1[0], r1[1], a.lambda);
                 } else {
                     w = (double)wp[(L - 1) & 1];
-                    if (d < (uint32_t)WIN)
-                        load_comps<K>(pw, WIN, d, pq);
-                    else
-                        load_comps<K>(Pn, Np, q, pq);
                 }
 #pragma unroll
                 for (int k = 0; k < K; ++k) acc[k] = acc[k] + pq[k] * w;
@@ -1024,6 +1182,7 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
             store_comps<K>(Pax, Np, i, pax);
         }
     }
+    if constexpr (XCH && NW > 1) __syncthreads();  // every wave's last edge-lane record read is over
     block_partial<K, BLOCK>(vp, a.partials + (uint64_t)K * tile, lds);
 }
 
