@@ -14,7 +14,11 @@ metric: lattice-node updates per second over LB and MP steps together
 (lx*ly*lz per step, the usual MLUPS convention), whole job.  The JSON line
 also carries the roofline of the dominant kernel (algorithmic HBM bytes per
 launch / measured mean launch duration, HIP events on the library's stream)
-and the reference program timed on this machine's host cores.
+and the reference program timed on this machine's host cores.  Two measured
+ceilings go beside it: the plain streaming copy ("copy_ceiling_gbs") and the
+LB step's own streams without its gathers ("soa_planes_ceiling": 22 FP64
+planes of the 256^3 medium's fluid nodes in and out), which "roofline_lb"
+reads its traffic against as "frac_of_soa_planes_ceiling".
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-fp32-shift] [--no-large]
                     [--no-batch] [--no-serving]
@@ -50,6 +54,7 @@ from laboetie_amd import lbx  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
 LB_BYTES_PER_FLUID = 19 * 8 * 2                 # read + write of 19 FP64 populations
+LB_FLUX_BYTES_PER_FLUID = 3 * 8 * 2             # read + write of the flux kept for the FP64 stop test
 MP_COEF_BYTES_PER_FLUID = 19 * 8 + 8            # coefficient source (n, rho), shared by the tracers of a batch
 MP_P_BYTES_PER_FLUID = 2 * 3 * 8                # P read + write, per tracer
 MP_BYTES_PER_ADS_INTERFACIAL = 2 * 3 * 8        # Pa read + write, per adsorbing tracer
@@ -588,6 +593,10 @@ def main():
     served = None if args.no_serving else serving_leg()
     dropin = None if args.no_dropin else dropin_leg(w)
     copy_gbs = lbx.copy_bandwidth(2 << 30, 20)  # measured ceiling, outside the timed region
+    # the ceiling of the LB step's own streams (19 population planes and 3 flux planes in and
+    # out, no gathers) at the 256^3 medium's fluid-node count
+    planes_nodes = large["fp64"]["fluid_nodes"] if large is not None else 10066330
+    planes_gbs = lbx.planes_bandwidth(22, planes_nodes, 20)
     lb_steps, mp_steps, deffs = results[-1]
     updates = N * sum(r[0] + r[1] for r in results)
     value = updates / dt / 1e6
@@ -632,10 +641,16 @@ def main():
         "roofline": roof_lb if dominant == "lb_step" else roof_mp,
         # the LB kernel's roofline is read on the 256^3 medium: the 128^3 job's LB
         # working set (~0.24 GB) sits in the 256 MB MALL
-        "roofline_lb": (dict(large["fp64"]["roofline_lb"], workload="256^3 random medium (FP64)")
+        "roofline_lb": (dict(large["fp64"]["roofline_lb"], workload="256^3 random medium (FP64)",
+                             # populations and the flux kept for the stop test (304 + 48 B per
+                             # fluid node) against the same streams without gathers
+                             frac_of_soa_planes_ceiling=round(
+                                 (LB_BYTES_PER_FLUID + LB_FLUX_BYTES_PER_FLUID) * planes_nodes
+                                 / (large["fp64"]["roofline_lb"]["avg_launch_us"] * 1e-6) / 1e9 / planes_gbs, 4))
                         if large is not None else None),
         "roofline_lb_bcc128_mall_resident": roof_lb, "roofline_mp": roof_mp,
         "copy_ceiling_gbs": round(copy_gbs, 1),
+        "soa_planes_ceiling": {"planes": 22, "nodes": planes_nodes, "gbs": round(planes_gbs, 1)},
         "lb_mlups": round(N * lb_steps * args.steps / (lb_ms / 1e3) / 1e6, 1),
         "mp_mlups": round(N * mp_steps * args.steps / (mp_ms / 1e3) / 1e6, 1),
         "mp_tracers_together": per_launch,

@@ -373,6 +373,14 @@ int64_t lbx_fluid_nodes(lbx_ctx* ctx);
  * kernel that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters. */
 int lbx_copy_bandwidth(int64_t bytes, int32_t iters, double* gb_per_s);
 
+/* Measurement helper with no counterpart in the reference: read+write
+ * bandwidth (GB/s) of streaming `planes` (19: the populations; 22: with the
+ * flux) FP64 planes of `nodes` values in and out, one node per lane, all
+ * loads before the stores: the LB step's streams without its gathers, the
+ * ceiling of the structure-of-arrays population layout (nodes in
+ * [1024, 2^31 - 1]). */
+int lbx_planes_bandwidth(int32_t planes, int64_t nodes, int32_t iters, double* gb_per_s);
+
 /* Block until the context's stream is idle. */
 int lbx_sync(lbx_ctx* ctx);
 

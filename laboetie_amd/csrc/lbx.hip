@@ -1384,6 +1384,47 @@ int lbx_addressing(lbx_ctx* c) { return c ? (c->wide ? 1 : 0) : -1; }
 
 int64_t lbx_fluid_nodes(lbx_ctx* c) { return c ? (int64_t)c->Nf : -1; }
 
+int lbx_planes_bandwidth(int32_t planes, int64_t nodes, int32_t iters, double* gb_per_s) {
+    if ((planes != 19 && planes != 22) || nodes < 1024 || nodes > (int64_t)INT32_MAX || iters <= 0 || !gb_per_s)
+        return LBX_ERR_ARG;
+    const uint64_t n = (uint64_t)nodes, count = (uint64_t)planes * n;
+    double *a = nullptr, *b = nullptr;
+    int st = dalloc(&a, count);
+    if (!st) st = dalloc(&b, count);
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st) st = hip_status(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (!st) st = hip_status(hipEventCreate(&e0));
+    if (!st) st = hip_status(hipEventCreate(&e1));
+    if (!st) st = hip_status(hipMemsetAsync(a, 0, count * sizeof(double), s));
+    if (!st) st = hip_status(hipMemsetAsync(b, 0, count * sizeof(double), s));
+    const uint32_t grid = (uint32_t)((n + kBlock - 1) / kBlock);
+    auto launch = [&]() {
+        if (planes == 19)
+            hipLaunchKernelGGL(k_planes<19>, dim3(grid), dim3(kBlock), 0, s, a, b, (uint32_t)n);
+        else
+            hipLaunchKernelGGL(k_planes<22>, dim3(grid), dim3(kBlock), 0, s, a, b, (uint32_t)n);
+    };
+    if (!st) {
+        launch();  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int32_t k = 0; k < iters; ++k) launch();
+        (void)hipEventRecord(e1, s);
+        st = hip_status(hipEventSynchronize(e1));
+    }
+    if (!st) {
+        float ms = 0.f;
+        st = hip_status(hipEventElapsedTime(&ms, e0, e1));
+        *gb_per_s = 2.0 * (double)(count * sizeof(double)) * iters / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    dfree(a);
+    dfree(b);
+    return st;
+}
+
 int lbx_set_mp_coefficients(lbx_ctx* c, int32_t form) {
     if (!c || form < LBX_MP_AUTO || form > LBX_MP_MIXED) return LBX_ERR_ARG;
     c->mp_coef = form;

@@ -1200,4 +1200,19 @@ __global__ __launch_bounds__(kBlock) void k_copy(const double2* __restrict__ a, 
         b[i] = a[i];
 }
 
+// The LB step's streams without its gathers: S FP64 planes of n nodes read
+// and S written, one node per lane, every load issued before the first store
+// (the order of the step's pulls and writes).  Its rate is the ceiling of the
+// structure-of-arrays layout itself, below the plain copy's.
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_planes(const double* __restrict__ in, double* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double v[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) v[s] = in[(uint64_t)s * n + i];
+#pragma unroll
+    for (int s = 0; s < S; ++s) out[(uint64_t)s * n + i] = v[s];
+}
+
 }  // namespace

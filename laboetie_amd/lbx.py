@@ -100,6 +100,7 @@ SIGNATURES = {
     "lbxb_stats": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
     "lbxb_free_tiles": (_i64, [_p]),
     "lbx_copy_bandwidth": (ctypes.c_int, [_i64, _i32, _p]),
+    "lbx_planes_bandwidth": (ctypes.c_int, [_i32, _i64, _i32, _p]),
 }
 
 
@@ -282,6 +283,14 @@ def copy_bandwidth(bytes_=2 << 30, iters=20):
     """Read+write GB/s of a plain streaming copy on the device."""
     g = _f64()
     _check(lib().lbx_copy_bandwidth(int(bytes_), int(iters), ctypes.byref(g)), "copy_bandwidth")
+    return g.value
+
+
+def planes_bandwidth(planes=19, nodes=10066330, iters=20):
+    """Read+write GB/s of streaming `planes` FP64 planes of `nodes` values
+    (the LB step's structure-of-arrays streams without its gathers)."""
+    g = _f64()
+    _check(lib().lbx_planes_bandwidth(int(planes), int(nodes), int(iters), ctypes.byref(g)), "planes_bandwidth")
     return g.value
 
 

@@ -100,3 +100,14 @@ def test_autotune_times_length_matches_header():
     text = open(os.path.join(REPO, "include", "lbx.h")).read()
     m = re.search(r"#define LBXB_AUTOTUNE_TIMES (\d+)", text)
     assert m and int(m.group(1)) == lbx.AUTOTUNE_TIMES
+
+
+def test_bandwidth_helpers_refuse_bad_arguments():
+    """Argument checks that return before any device call."""
+    import ctypes
+    g = ctypes.c_double(0.0)
+    L = lbx.lib()
+    assert L.lbx_planes_bandwidth(5, 1 << 20, 1, ctypes.byref(g)) == lbx.ERR_ARG
+    assert L.lbx_planes_bandwidth(19, 100, 1, ctypes.byref(g)) == lbx.ERR_ARG
+    assert L.lbx_planes_bandwidth(22, 1 << 20, 0, ctypes.byref(g)) == lbx.ERR_ARG
+    assert L.lbx_copy_bandwidth(1000, 1, ctypes.byref(g)) == lbx.ERR_ARG

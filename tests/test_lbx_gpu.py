@@ -396,3 +396,11 @@ def test_mp_run_start_wait_equals_mp_run(name):
         if conv:  # after the convergence step a begun chunk runs nothing
             L.mp_run_start(5, stop_on_convergence=True)
             assert L.mp_run_wait()["steps"] == 0
+
+
+@pytest.mark.gpu
+def test_bandwidth_helpers_measure():
+    """The two measured ceilings come back as plausible rates (GB/s)."""
+    for g in (lbx.copy_bandwidth(64 << 20, 2), lbx.planes_bandwidth(19, 1 << 20, 2),
+              lbx.planes_bandwidth(22, 1 << 20, 2)):
+        assert 10.0 < g < 20000.0
