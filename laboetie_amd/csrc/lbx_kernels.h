@@ -903,22 +903,22 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     constexpr bool RV = CF != kCoefStored;  // (rho, v) staged in the window
     constexpr bool RVZ = CF == kCoefRecomputed;  // the directions with c_z != 0 recomputed too
     constexpr int K = 3 * T;  // moment components of the batch
-    // three tracers with (rho, v) in the window: 4 fewer halo nodes a side
-    // leave room for the edge-lane records within a third of the CU's LDS
-    // Edge-lane records and the early z-neighbour gathers below: where the
-    // z-directions read stored probabilities (measured: with (rho, v)
-    // recomputed for them the added loads before the barrier cost more than
-    // the records save)
+    // The edge-lane records and the early gathers of the neighbours along 5
+    // and 6 (below) serve the forms that store the z-directions'
+    // probabilities.  With those recomputed (RVZ) the added loads and (rho, v)
+    // work before the window barrier cost more than the records save
+    // (measured), so RVZ leaves both out.
     constexpr bool XCH = !RVZ;
+    // the mixed form with three tracers: 4 fewer halo nodes a side keep the
+    // records within a third of the CU's LDS (three blocks per CU)
     constexpr int HALO = (XCH && RV && T == 3) ? kMpHalo - 4 : kMpHalo;
     constexpr int WIN = BLOCK + 2 * HALO;
     constexpr int NW = BLOCK / 64;  // waves per block
-    // Edge-lane records, [wave boundary][side][plane][XR]: P of a wave's first
+    // Edge-lane records, [wave boundary][side][plane][K]: P of a wave's first
     // (side 0) or last (side 1) lane's neighbour along 5 (plane 0) or 6
-    // (plane 1) and, RVZ, the probability its neighbour lane in the adjacent
-    // wave takes from it.  The array holds the VACF partial's wave sums after
-    // the step's last read of a record.
-    constexpr int XR = K + (RVZ ? 1 : 0);
+    // (plane 1), for the lane beside it in the adjacent wave.  The array
+    // holds the VACF partial's wave sums after the step's last record read.
+    constexpr int XR = K;
     constexpr int XN = (XCH && NW > 1) ? (NW - 1) * 4 * XR : 0;
     constexpr int LN = NW * K > XN ? NW * K : XN;
     __shared__ double lds[LN];
@@ -963,7 +963,7 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     // directions 11..14 reuse (below): gathered here, with the window, where
     // they lie outside it or the lane is a wave's first or last.
     double zc[2][K];         // their P (0 if solid)
-    Vec2<double> zr[2][2];   // RVZ: their (rho, v)
+    Vec2<double> zr[2][2];   // RVZ: their (rho, v), kept in the direction loop
     bool zearly[2] = {false, false};
     const bool edge_lane = XCH && NW > 1 && (lane == 0u || lane == 63u);
 #pragma unroll
@@ -975,13 +975,7 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
         if (valid) {
             const uint32_t q = nb.q[5 + s];
             zearly[s] = XCH && q != kSolidSlot && ((uint32_t)((int64_t)q - w0) >= (uint32_t)WIN || edge_lane);
-            if (zearly[s]) {
-                load_comps<K>(Pn, Np, q, zc[s]);
-                if constexpr (RVZ) {
-                    zr[s][0] = rvv.ld2(2 * q);
-                    zr[s][1] = rvv.ld2(2 * q + 1);
-                }
-            }
+            if (zearly[s]) load_comps<K>(Pn, Np, q, zc[s]);
         }
     }
     {
@@ -1021,24 +1015,12 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     if constexpr (XCH && NW > 1) {
         if (valid && ((lane == 0u && wave > 0u) || (lane == 63u && wave + 1u < (uint32_t)NW))) {
             const uint32_t side = lane == 0u ? 0u : 1u, bnd = lane == 0u ? wave - 1u : wave;
-            static_for<0, 2>([&](auto S) {
-                constexpr int s = S;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
                 double* r = lds + ((bnd * 2u + side) * 2u + (uint32_t)s) * XR;
 #pragma unroll
                 for (int k = 0; k < K; ++k) r[k] = zc[s][k];
-                if constexpr (RVZ) {
-                    double w = 0.0;
-                    if (zearly[s]) {
-                        if (side == 0u)
-                            w = scatter_prob<kInv[11 + 2 * s], W>(zr[s][0][0], zr[s][0][1], zr[s][1][0],
-                                                                  zr[s][1][1], a.lambda);
-                        else
-                            w = scatter_prob<kInv[12 + 2 * s], W>(zr[s][0][0], zr[s][0][1], zr[s][1][0],
-                                                                  zr[s][1][1], a.lambda);
-                    }
-                    r[K] = w;
-                }
-            });
+            }
         }
     }
     __syncthreads();
@@ -1100,12 +1082,11 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
                             r0 = sr[0];
                             r1 = sr[1];
                         }
-                    } else if (how == 2) {
+                    } else if (how == 2) {  // XCH: these directions' probabilities are stored
                         const double* r = lds + (((right ? wave : wave - 1u) * 2u + (right ? 0u : 1u)) * 2u +
                                                  (uint32_t)zsrc) * XR;
 #pragma unroll
                         for (int k = 0; k < K; ++k) pq[k] = r[k];
-                        if constexpr (!stored) w = r[K];
                     }
                 }
                 if constexpr (zkeep >= 0) {
@@ -1143,11 +1124,10 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
                         zr[zkeep][1] = r1;
                     }
                 }
-                if constexpr (!stored) {
-                    if (how != 2) w = scatter_prob<kInv[L], W>(r0[0], r0[1], r1[0], r1[1], a.lambda);
-                } else {
+                if constexpr (!stored)
+                    w = scatter_prob<kInv[L], W>(r0[0], r0[1], r1[0], r1[1], a.lambda);
+                else
                     w = (double)wp[(L - 1) & 1];
-                }
 #pragma unroll
                 for (int k = 0; k < K; ++k) acc[k] = acc[k] + pq[k] * w;
             }
