@@ -175,7 +175,7 @@ struct BatchMpView {
 
 // MP launch k of a round on one BLOCK-node tile (mp_repeat: every launch
 // recomputes step mp_it0 + 1, for timing)
-template <int BLOCK, bool Wide, typename W>
+template <int BLOCK, bool Wide, typename W, bool XR>
 __device__ __forceinline__ void kb_mp_tile(const BatchMpView<W>& v, uint32_t tile, int64_t k, int32_t reduce_prev) {
     const uint32_t id = v.tile_inst[tile / (kTile / BLOCK)];
     if (id == kNoInst) return;
@@ -207,12 +207,14 @@ __device__ __forceinline__ void kb_mp_tile(const BatchMpView<W>& v, uint32_t til
     a.Np = v.Np;
     a.any_ads = in.any_ads;
     a.tp = in.tp;
-    mp_step_tile<BLOCK, Wide, 1, W>(a, tile, tile == first, in.ntiles * (kTile / BLOCK));
+    mp_step_tile<BLOCK, Wide, 1, W, kCoefStored, XR>(a, tile, tile == first, in.ntiles * (kTile / BLOCK));
 }
 
-template <int BLOCK, bool Wide, typename W>
+// XR: some live instance is more than one node wide along x (the MP step's
+// reuse of the lanes' z-neighbours, compiled out for batches of 2-D instances)
+template <int BLOCK, bool Wide, typename W, bool XR>
 __global__ __launch_bounds__(BLOCK) void kb_mp_step(BatchMpView<W> v, int64_t k, int32_t reduce_prev) {
-    kb_mp_tile<BLOCK, Wide, W>(v, xcd_tile(), k, reduce_prev);
+    kb_mp_tile<BLOCK, Wide, W, XR>(v, xcd_tile(), k, reduce_prev);
 }
 
 // VACF of an MP instance's last step of the round: partials of TILE-node
@@ -245,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void kb_mp_reduce_last(const BInst* insts, 
 // launches, so the results are the same bits.  Every block runs every
 // barrier: the loops have grid-uniform bounds and the per-tile functions
 // only return from the tile, never from the kernel.
-template <int BLOCK, bool Wide, typename E, typename R, typename W>
+template <int BLOCK, bool Wide, typename E, typename R, typename W, bool XR>
 __global__ __launch_bounds__(BLOCK) void kb_round(BatchView<E, R> lv, BatchMpView<W> mv, int64_t S_lb, int64_t S_mp,
                                                   uint32_t ntiles, uint32_t ninst, int32_t lb_redo_only) {
     cooperative_groups::grid_group grid = cooperative_groups::this_grid();
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(BLOCK) void kb_round(BatchView<E, R> lv, BatchMpVie
         grid.sync();
     }
     for (int64_t k = 0; k < S_mp; ++k) {
-        for (uint32_t tile = t0 + local; tile < t1; tile += nblk) kb_mp_tile<BLOCK, Wide, W>(mv, tile, k, 1);
+        for (uint32_t tile = t0 + local; tile < t1; tile += nblk) kb_mp_tile<BLOCK, Wide, W, XR>(mv, tile, k, 1);
         grid.sync();
     }
     if (S_mp > 0)
@@ -368,6 +370,7 @@ struct HostInst {
 struct lbxb {
     int32_t prec = LBX_PREC_FP64;
     bool wide = false;       // 64-bit addressing of the population buffers
+    uint32_t n_xrows = 0;    // live instances more than one node wide along x
     bool wide_auto = false;  // what the capacity alone selects
     uint64_t Np = 0, cap_box = 0;
     uint32_t cap_inst = 0, ntiles_cap = 0;
@@ -413,6 +416,7 @@ struct lbxb {
         hipGraphExec_t exec = nullptr;
         int32_t block = 0;
         uint32_t grid_tiles = 0;
+        bool xr = false;  // the MP launches' kernel reuses the lanes' z-neighbours
     };
     std::map<std::pair<int, int64_t>, Graph> graph_cache;
     // statistics
@@ -565,15 +569,18 @@ void launch_lb(lbxb* b, int64_t k, int block) {
 void launch_mp(lbxb* b, int64_t k, int block, int reduce_prev) {
     const uint32_t grid = grid_of(b, block);
     if (!grid) return;
+    const bool xr = b->n_xrows > 0;
     with_block(block, [&](auto BT) {
         constexpr int BL = decltype(BT)::value;
         if (shift(b)) {
             auto v = mp_view<float>(b);
-            auto kern = b->wide ? kb_mp_step<BL, true, float> : kb_mp_step<BL, false, float>;
+            auto kern = b->wide ? (xr ? kb_mp_step<BL, true, float, true> : kb_mp_step<BL, true, float, false>)
+                                : (xr ? kb_mp_step<BL, false, float, true> : kb_mp_step<BL, false, float, false>);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k, reduce_prev);
         } else {
             auto v = mp_view<double>(b);
-            auto kern = b->wide ? kb_mp_step<BL, true, double> : kb_mp_step<BL, false, double>;
+            auto kern = b->wide ? (xr ? kb_mp_step<BL, true, double, true> : kb_mp_step<BL, true, double, false>)
+                                : (xr ? kb_mp_step<BL, false, double, true> : kb_mp_step<BL, false, double, false>);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k, reduce_prev);
         }
     });
@@ -615,7 +622,9 @@ int run_phase(lbxb* b, int phase /*0 LB, 1 MP*/, int64_t S) {
     }
     auto key = std::make_pair(phase, S);
     auto it = b->graph_cache.find(key);
-    if (it != b->graph_cache.end() && (it->second.block != block || it->second.grid_tiles != b->hw_tiles)) {
+    const bool xr = b->n_xrows > 0;
+    if (it != b->graph_cache.end() &&
+        (it->second.block != block || it->second.grid_tiles != b->hw_tiles || it->second.xr != xr)) {
         (void)hipGraphExecDestroy(it->second.exec);
         b->graph_cache.erase(it);
         it = b->graph_cache.end();
@@ -636,6 +645,7 @@ int run_phase(lbxb* b, int phase /*0 LB, 1 MP*/, int64_t S) {
         if (ie != hipSuccess) return hip_status(ie);
         gr.block = block;
         gr.grid_tiles = b->hw_tiles;
+        gr.xr = xr;
         it = b->graph_cache.emplace(key, gr).first;
     }
     return hip_status(hipGraphLaunch(it->second.exec, b->stream));
@@ -648,13 +658,14 @@ int run_phase_persistent(lbxb* b, int phase, int64_t S, int32_t redo_only) {
     const int block = b->lb_block;
     const uint32_t ntiles = grid_of(b, block);
     if (!ntiles) return LBX_OK;
+    const bool xr = b->n_xrows > 0;
     auto go = [&](auto etag, auto rtag, auto btag, auto wide) -> int {
         using E = decltype(etag);  // population storage
         using R = decltype(rtag);  // LB arithmetic
         using W = E;               // MP probabilities are stored like the populations
         constexpr int BL = decltype(btag)::value;
         constexpr bool WD = decltype(wide)::value;
-        auto kern = kb_round<BL, WD, E, R, W>;
+        auto kern = xr ? kb_round<BL, WD, E, R, W, true> : kb_round<BL, WD, E, R, W, false>;
         int per_cu = 0;
         LBX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BL, 0));
         if (per_cu <= 0) return LBX_ERR_DEVICE;
@@ -990,6 +1001,7 @@ int lbxb_add(lbxb* b, int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, 
     b->tab[id] = in;
     b->hi[id] = std::move(h);
     b->hw_tiles = std::max<uint32_t>(b->hw_tiles, (uint32_t)(tile0 + ntiles));
+    if (lx > 1) ++b->n_xrows;
     *id_out = (int32_t)id;
     return LBX_OK;
 }
@@ -1003,6 +1015,7 @@ int lbxb_remove(lbxb* b, int32_t id) {
                            in.ntiles * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
     b->node_alloc.give(in.tile_first, in.ntiles);
     b->map_alloc.give(in.map_off, b->hi[id].map_n);
+    if (b->hi[id].lx > 1) --b->n_xrows;
     b->hi[id] = HostInst{};
     in = BInst{};
     uint32_t hw = 0;

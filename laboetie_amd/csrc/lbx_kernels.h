@@ -893,11 +893,12 @@ struct MpStepArgs {
 // VACF partial to a.partials[K*tile ...]; first_block also reduces the
 // previous step's nparts partials (a.prev_partials) into VACF(it-1).
 //
+// XR: directions 11..14 may reuse the z-neighbours of the lanes beside (below).
 // CF (kCoef*): recomputed probabilities come from the neighbours' (rho, v)
 // (scatter_prob, the set-up's own arithmetic: the same bits): 32 bytes per
 // node kept and staged in the LDS window beside P; stored ones from win (18 or,
 // mixed, kZDirs per node, read in pairs).
-template <int BLOCK, bool Wide, int T, typename W, int CF = kCoefStored>
+template <int BLOCK, bool Wide, int T, typename W, int CF = kCoefStored, bool XR = true>
 __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t tile, bool first_block,
                                              uint32_t nparts) {
     constexpr bool RV = CF != kCoefStored;  // (rho, v) staged in the window
@@ -908,18 +909,18 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     // probabilities.  With those recomputed (RVZ) the added loads and (rho, v)
     // work before the window barrier cost more than the records save
     // (measured), so RVZ leaves both out.
-    constexpr bool XCH = !RVZ;
+    constexpr bool XCH = !RVZ && XR;
     // the mixed form with three tracers: 4 fewer halo nodes a side keep the
     // records within a third of the CU's LDS (three blocks per CU)
     constexpr int HALO = (XCH && RV && T == 3) ? kMpHalo - 4 : kMpHalo;
     constexpr int WIN = BLOCK + 2 * HALO;
     constexpr int NW = BLOCK / 64;  // waves per block
-    // Edge-lane records, [wave boundary][side][plane][K]: P of a wave's first
+    // Edge-lane records, [wave boundary][side][plane][RW]: P of a wave's first
     // (side 0) or last (side 1) lane's neighbour along 5 (plane 0) or 6
     // (plane 1), for the lane beside it in the adjacent wave.  The array
     // holds the VACF partial's wave sums after the step's last record read.
-    constexpr int XR = K;
-    constexpr int XN = (XCH && NW > 1) ? (NW - 1) * 4 * XR : 0;
+    constexpr int RW = K;  // doubles per record
+    constexpr int XN = (XCH && NW > 1) ? (NW - 1) * 4 * RW : 0;
     constexpr int LN = NW * K > XN ? NW * K : XN;
     __shared__ double lds[LN];
     __shared__ double pwin[K * WIN];  // P(now) of packed nodes [w0, w0 + WIN), comp_index layout
@@ -962,10 +963,16 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     // The node's neighbours along 5 (0,0,1) and 6 (0,0,-1), whose values the
     // directions 11..14 reuse (below): gathered here, with the window, where
     // they lie outside it or the lane is a wave's first or last.
+    // A lattice one node wide along x (the 2-D instances) has no x-neighbour
+    // in another lane: it skips all of this (the same for every lane of a
+    // block, whose nodes belong to one lattice), and XR = false compiles it
+    // out (a batch of such lattices only: the registers it holds cost them
+    // occupancy).
+    const bool xreuse = XR && a.dlx.d > 1u;
     double zc[2][K];         // their P (0 if solid)
     Vec2<double> zr[2][2];   // RVZ: their (rho, v), kept in the direction loop
     bool zearly[2] = {false, false};
-    const bool edge_lane = XCH && NW > 1 && (lane == 0u || lane == 63u);
+    const bool edge_lane = XCH && NW > 1 && xreuse && (lane == 0u || lane == 63u);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
 #pragma unroll
@@ -974,7 +981,7 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
         zr[s][1] = Vec2<double>{0.0, 0.0};
         if (valid) {
             const uint32_t q = nb.q[5 + s];
-            zearly[s] = XCH && q != kSolidSlot && ((uint32_t)((int64_t)q - w0) >= (uint32_t)WIN || edge_lane);
+            zearly[s] = XCH && xreuse && q != kSolidSlot && ((uint32_t)((int64_t)q - w0) >= (uint32_t)WIN || edge_lane);
             if (zearly[s]) load_comps<K>(Pn, Np, q, zc[s]);
         }
     }
@@ -1013,11 +1020,11 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
     // wave w serves lane 63 of wave w-1 (directions 11, 13), lane 63 of wave w
     // serves lane 0 of wave w+1 (12, 14)
     if constexpr (XCH && NW > 1) {
-        if (valid && ((lane == 0u && wave > 0u) || (lane == 63u && wave + 1u < (uint32_t)NW))) {
+        if (valid && xreuse && ((lane == 0u && wave > 0u) || (lane == 63u && wave + 1u < (uint32_t)NW))) {
             const uint32_t side = lane == 0u ? 0u : 1u, bnd = lane == 0u ? wave - 1u : wave;
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                double* r = lds + ((bnd * 2u + side) * 2u + (uint32_t)s) * XR;
+                double* r = lds + ((bnd * 2u + side) * 2u + (uint32_t)s) * RW;
 #pragma unroll
                 for (int k = 0; k < K; ++k) r[k] = zc[s][k];
             }
@@ -1055,11 +1062,14 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
             double sh[K];          // moved across the wave while every valid lane is active
             Vec2<double> sr[2];
             if constexpr (zsrc >= 0) {
+                if (xreuse) {
 #pragma unroll
-                for (int k = 0; k < K; ++k) sh[k] = lane_shift<kCX[L]>(zc[zsrc][k]);
-                if constexpr (!stored)
-                    for (int h = 0; h < 2; ++h)
-                        sr[h] = Vec2<double>{lane_shift<kCX[L]>(zr[zsrc][h][0]), lane_shift<kCX[L]>(zr[zsrc][h][1])};
+                    for (int k = 0; k < K; ++k) sh[k] = lane_shift<kCX[L]>(zc[zsrc][k]);
+                    if constexpr (!stored)
+                        for (int h = 0; h < 2; ++h)
+                            sr[h] = Vec2<double>{lane_shift<kCX[L]>(zr[zsrc][h][0]),
+                                                 lane_shift<kCX[L]>(zr[zsrc][h][1])};
+                }
             }
             const uint32_t q = nb.q[L];
             if (q != kSolidSlot) {
@@ -1084,7 +1094,7 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
                         }
                     } else if (how == 2) {  // XCH: these directions' probabilities are stored
                         const double* r = lds + (((right ? wave : wave - 1u) * 2u + (right ? 0u : 1u)) * 2u +
-                                                 (uint32_t)zsrc) * XR;
+                                                 (uint32_t)zsrc) * RW;
 #pragma unroll
                         for (int k = 0; k < K; ++k) pq[k] = r[k];
                     }
@@ -1162,7 +1172,8 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
             store_comps<K>(Pax, Np, i, pax);
         }
     }
-    if constexpr (XCH && NW > 1) __syncthreads();  // every wave's last edge-lane record read is over
+    if constexpr (XCH && NW > 1)
+        if (xreuse) __syncthreads();  // every wave's last edge-lane record read is over
     block_partial<K, BLOCK>(vp, a.partials + (uint64_t)K * tile, lds);
 }
 
