@@ -404,3 +404,41 @@ def test_bandwidth_helpers_measure():
     for g in (lbx.copy_bandwidth(64 << 20, 2), lbx.planes_bandwidth(19, 1 << 20, 2),
               lbx.planes_bandwidth(22, 1 << 20, 2)):
         assert 10.0 < g < 20000.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", [lbx.MP_STORED, lbx.MP_RECOMPUTED, lbx.MP_MIXED])
+@pytest.mark.parametrize("dims", [(63, 6, 7), (65, 5, 6), (130, 3, 4), (2, 10, 11), (1, 30, 31)])
+def test_mp_lane_reuse_shapes_match_restatement(dims, form):
+    """The MP step's reuse of the z-neighbours of the lanes beside (directions
+    11..14 through wave lane moves and edge-lane records) on rows that end
+    inside a wave, cross wave and block boundaries, are longer than the LDS
+    halo, two nodes long (x+1 and x-1 the same node) and one node long (no
+    reuse): three adsorbing tracers in each coefficient form, P, Pa bit for bit
+    and VACF to 1e-12 of |VACF(0)| against the restatement run per tracer."""
+    say("lane reuse shapes", dims, form)
+    rng = np.random.default_rng(dims[0] * 1000 + dims[1] * 10 + dims[2])
+    N = int(np.prod(dims))
+    nat = (rng.random(N) < 0.2).astype(np.int8)
+    nat[rng.integers(N)] = 0
+    fext = [2e-5, -1e-5, 3e-5]
+    ka = [lbx.ka_kd_from_dF(d)[0] for d in (0.0, -1.0, -2.0)]
+    kd = [lbx.KD_DEFAULT] * 3
+    with lbx.Lattice(dims, nat, 0.9) as L:
+        L.init_rest(1.0)
+        L.equilibrate(fext, 1e-12, max_steps=40)
+        n, rho, j = L.get_state()
+        L.set_mp_coefficients(form)
+        L.mp_init_batch(fext, 0.01, ka, kd)
+        m = L.mp_run(30)
+        got = [L.mp_get(t) for t in range(3)]
+    itf = lbref.interfacial(*dims, nat)
+    nt = lbref.tracer_population(dims, nat, rho, j, fext)
+    for t in range(3):
+        mp = lbref.MP(dims, nat, itf, nt, rho, 0.01, ka[t], kd[t])
+        st, vref = mp.run(30)
+        assert vacf_close(m["vacf"][:, t, :], vref[1:], np.abs(vref[0]).max())
+        P, Pa = got[t]
+        assert np.array_equal(P, mp.P)
+        if mp.ads:
+            assert np.array_equal(Pa, mp.Pa)
