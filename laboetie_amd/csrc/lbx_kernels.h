@@ -1079,7 +1079,7 @@ __device__ __forceinline__ void mp_step_tile(const MpStepArgs<W>& a, uint32_t ti
                 int how = 0;          // 0: gather, 1: from the lane beside, 2: from an edge-lane record
                 if constexpr (zsrc >= 0) {
                     constexpr bool right = kCX[L] > 0;
-                    if (right ? adj_r : adj_l) {
+                    if (xreuse && (right ? adj_r : adj_l)) {  // else gathered (sh is not set)
                         if (!(right ? lane == 63u : lane == 0u))
                             how = 1;
                         else if (XCH && (right ? wave + 1u < (uint32_t)NW : wave > 0u))
