@@ -199,6 +199,21 @@ def serving_leg(n_requests=16, seed=20261202):
                        "F log-uniform [1e-7,1e-5], target 1e-9 (cap 10000), 2000 MP steps, dF U[-3,0], FP32-shift, "
                        "48 M-node HBM budget",
            "saturation_req_per_s": round(rate0, 3), "saturation_mlups": round(sat["mlups"], 1)}
+    # the same requests on the plain FP64 path, one after the other: the aggregate rate the
+    # batched FP32-shift form is compared with, and each request's D_eff against which its
+    # served D_eff is read (each form stopped by its own rule)
+    plain = serving.run_one_at_a_time(reqs[:n_requests // 2], lbx.PREC_FP64)
+    d64 = np.array(plain["deff"])
+    d32 = np.array(sat["deff"])
+    err = np.abs(d32 - d64).max(axis=1) / np.abs(d64).max(axis=1)
+    out["plain_fp64_one_at_a_time"] = {"requests": plain["requests"], "mlups": round(plain["mlups"], 1),
+                                       "wall_s": round(plain["wall_s"], 2)}
+    out["saturation_vs_plain_fp64"] = round(sat["mlups"] / plain["mlups"], 3)
+    out["deff_rel_error_vs_plain_fp64"] = {"requests": int(err.size), "median": float(f"{np.median(err):.3e}"),
+                                           "max": float(f"{err.max():.3e}"),
+                                           "count_above_1e-5": int((err > 1e-5).sum()),
+                                           "kinds": [r["kind"] for r in reqs[:n_requests // 2]],
+                                           "per_request": [float(f"{e:.3e}") for e in err]}
     for f in (0.5, 1.0, 2.0):
         rate = f * rate0
         arr = np.cumsum(np.random.default_rng(seed + int(f * 1000)).exponential(1.0 / rate, n_requests))
@@ -275,6 +290,13 @@ def batch_leg(n=1024):
             "instances": int(conv.sum()), "median": float(f"{np.median(ec):.3e}"), "max": float(f"{ec.max():.3e}"),
             "count_above_1e-5": int((ec > 1e-5).sum())}
         out[f"{name}_vs_fp64"] = round(out[name]["mlups"] / out["fp64"]["mlups"], 3)
+    # the plain FP64 path, one instance after the other, on a sample of 16 instances: its
+    # aggregate rate against the batched forms'
+    plain = serving.run_one_at_a_time(reqs[:16], lbx.PREC_FP64)
+    out["plain_fp64_one_at_a_time"] = {"instances": plain["requests"], "mlups": round(plain["mlups"], 1),
+                                       "wall_s": round(plain["wall_s"], 2)}
+    for name in ("fp64", "fp32_shift", "fp32_store"):
+        out[f"{name}_vs_plain_fp64"] = round(out[name]["mlups"] / plain["mlups"], 2)
     print("bench: batch leg done", file=sys.stderr, flush=True)
     return out
 

@@ -76,6 +76,7 @@ template <typename E, typename R = E>
 struct BatchView {
     const BInst* insts;
     const uint32_t* tile_inst;
+    const uint32_t* act;          // the 256-node tiles of the instances in the phase, ascending
     E* f0;
     E* f1;
     R* j;                         // the step's flux (FP32-shift: j32)
@@ -86,6 +87,16 @@ struct BatchView {
     Ctrl* ctrl;                   // [inst]
     uint64_t Np;
 };
+
+// A launch covers only the tiles of the instances in its phase: logical
+// block l of the launch is BLOCK-node tile `active_tile(act, l)`, in the
+// 256-node tile act[l / (256/BLOCK)] (instances in the other phase, or done,
+// cost no blocks).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t active_tile(const uint32_t* act, uint32_t l) {
+    constexpr uint32_t per = kTile / BLOCK;
+    return act[l / per] * per + l % per;
+}
 
 // LB launch k of a round on one BLOCK-node tile (a redo repeats step lb_t0)
 template <int BLOCK, bool Wide, typename E, typename R = E>
@@ -134,7 +145,7 @@ __device__ __forceinline__ void kb_lb_tile(const BatchView<E, R>& v, uint32_t ti
 
 template <int BLOCK, bool Wide, typename E, typename R = E>
 __global__ __launch_bounds__(BLOCK) void kb_lb_step(BatchView<E, R> v, int64_t k) {
-    kb_lb_tile<BLOCK, Wide, E, R>(v, xcd_tile(), k);
+    kb_lb_tile<BLOCK, Wide, E, R>(v, active_tile<BLOCK>(v.act, xcd_tile()), k);
 }
 
 // l2err of each LB instance's last step of the round (one wave per instance slot)
@@ -156,6 +167,7 @@ template <typename W>
 struct BatchMpView {
     const BInst* insts;
     const uint32_t* tile_inst;
+    const uint32_t* act;          // the 256-node tiles of the instances in the phase, ascending
     const W* win;
     const double* rem;
     const W* ustar;
@@ -214,7 +226,7 @@ __device__ __forceinline__ void kb_mp_tile(const BatchMpView<W>& v, uint32_t til
 // reuse of the lanes' z-neighbours, compiled out for batches of 2-D instances)
 template <int BLOCK, bool Wide, typename W, bool XR>
 __global__ __launch_bounds__(BLOCK) void kb_mp_step(BatchMpView<W> v, int64_t k, int32_t reduce_prev) {
-    kb_mp_tile<BLOCK, Wide, W, XR>(v, xcd_tile(), k, reduce_prev);
+    kb_mp_tile<BLOCK, Wide, W, XR>(v, active_tile<BLOCK>(v.act, xcd_tile()), k, reduce_prev);
 }
 
 // VACF of an MP instance's last step of the round: partials of TILE-node
@@ -257,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) void kb_round(BatchView<E, R> lv, BatchMpVie
     const uint32_t per = (ntiles + 7u) / 8u;
     const uint32_t t0 = x * per, t1 = min(ntiles, t0 + per);
     for (int64_t k = 0; k < S_lb; ++k) {
-        for (uint32_t tile = t0 + local; tile < t1; tile += nblk) kb_lb_tile<BLOCK, Wide, E, R>(lv, tile, k);
+        for (uint32_t l = t0 + local; l < t1; l += nblk) kb_lb_tile<BLOCK, Wide, E, R>(lv, active_tile<BLOCK>(lv.act, l), k);
         grid.sync();
     }
     if (S_lb > 0 && !lb_redo_only) {
@@ -266,7 +278,8 @@ __global__ __launch_bounds__(BLOCK) void kb_round(BatchView<E, R> lv, BatchMpVie
         grid.sync();
     }
     for (int64_t k = 0; k < S_mp; ++k) {
-        for (uint32_t tile = t0 + local; tile < t1; tile += nblk) kb_mp_tile<BLOCK, Wide, W, XR>(mv, tile, k, 1);
+        for (uint32_t l = t0 + local; l < t1; l += nblk)
+            kb_mp_tile<BLOCK, Wide, W, XR>(mv, active_tile<BLOCK>(mv.act, l), k, 1);
         grid.sync();
     }
     if (S_mp > 0)
@@ -409,13 +422,17 @@ struct lbxb {
     std::vector<Ctrl> ctrl_h;  // per instance slot: what reset_inst_ctrl uploads (kept until the stream syncs)
     std::vector<HostInst> hi;
     std::vector<uint32_t> tile_inst_h;
+    // the tiles of the instances with LB (0) or MP (1) work in the launches
+    // about to run (built by upload_active; host copy kept until the stream syncs)
+    uint32_t* act[2] = {nullptr, nullptr};
+    std::vector<uint32_t> act_h[2];
+    uint32_t n_act[2] = {0, 0};
     RangeAlloc node_alloc, map_alloc;
-    uint32_t hw_tiles = 0;  // tiles below this may be in use
     // graphs of a round's launches, keyed by (phase, launches)
     struct Graph {
         hipGraphExec_t exec = nullptr;
         int32_t block = 0;
-        uint32_t grid_tiles = 0;
+        uint32_t grid_tiles = 0;  // active 256-node tiles of the phase
         bool xr = false;  // the MP launches' kernel reuses the lanes' z-neighbours
     };
     std::map<std::pair<int, int64_t>, Graph> graph_cache;
@@ -447,6 +464,8 @@ void free_all(lbxb* b) {
     dfree(b->nodes);
     dfree(b->maps);
     dfree(b->tile_inst);
+    dfree(b->act[0]);
+    dfree(b->act[1]);
     dfree(b->insts);
     dfree(b->slots);
     dfree(b->l2ring);
@@ -480,13 +499,38 @@ int reset_inst_ctrl(lbxb* b, uint32_t id, int64_t check_after) {
     return hip_status(hipMemcpyAsync(b->ctrl + id, &h, sizeof(Ctrl), hipMemcpyHostToDevice, b->stream));
 }
 
-uint32_t grid_of(const lbxb* b, int block) { return b->hw_tiles * (kTile / block); }
+uint32_t grid_of(const lbxb* b, int block, int phase /*0 LB, 1 MP*/) { return b->n_act[phase] * (kTile / block); }
+
+// The active tile lists of the work table as it stands (lb_on / mp_on), for
+// the launches that follow; call after the table changes which instances take
+// part and before those launches.
+int upload_active(lbxb* b) {
+    LBX_TRY(hipStreamSynchronize(b->stream));  // the previous lists' uploads have been read
+    for (int ph = 0; ph < 2; ++ph) {
+        std::vector<std::pair<uint32_t, uint32_t>> r;  // (tile_first, ntiles) of the phase's instances
+        for (uint32_t id = 0; id < b->cap_inst; ++id) {
+            const BInst& in = b->tab[id];
+            if (ph == 0 ? in.lb_on : in.mp_on) r.emplace_back(in.tile_first, in.ntiles);
+        }
+        std::sort(r.begin(), r.end());
+        std::vector<uint32_t>& a = b->act_h[ph];
+        a.clear();
+        for (const auto& x : r)
+            for (uint32_t t = 0; t < x.second; ++t) a.push_back(x.first + t);
+        b->n_act[ph] = (uint32_t)a.size();
+        if (!a.empty())
+            LBX_TRY(hipMemcpyAsync(b->act[ph], a.data(), a.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                   b->stream));
+    }
+    return LBX_OK;
+}
 
 template <typename E, typename R = E>
 BatchView<E, R> lb_view(lbxb* b) {
     BatchView<E, R> v;
     v.insts = b->insts;
     v.tile_inst = b->tile_inst;
+    v.act = b->act[0];
     if constexpr (std::is_same<E, float>::value) {
         v.f0 = b->g[0];
         v.f1 = b->g[1];
@@ -510,6 +554,7 @@ BatchMpView<W> mp_view(lbxb* b) {
     BatchMpView<W> v;
     v.insts = b->insts;
     v.tile_inst = b->tile_inst;
+    v.act = b->act[1];
     if constexpr (std::is_same<W, float>::value) {
         v.win = b->win32;
         v.ustar = b->ustar32;
@@ -544,9 +589,9 @@ void with_block(int block, F&& f) {
         f(std::integral_constant<int, 256>{});
 }
 
-// launch LB step k of a round over every allocated tile
+// launch LB step k of a round over the tiles of the instances with LB work
 void launch_lb(lbxb* b, int64_t k, int block) {
-    const uint32_t grid = grid_of(b, block);
+    const uint32_t grid = grid_of(b, block, 0);
     if (!grid) return;
     with_block(block, [&](auto BT) {
         constexpr int BL = decltype(BT)::value;
@@ -567,7 +612,7 @@ void launch_lb(lbxb* b, int64_t k, int block) {
 }
 
 void launch_mp(lbxb* b, int64_t k, int block, int reduce_prev) {
-    const uint32_t grid = grid_of(b, block);
+    const uint32_t grid = grid_of(b, block, 1);
     if (!grid) return;
     const bool xr = b->n_xrows > 0;
     with_block(block, [&](auto BT) {
@@ -624,7 +669,7 @@ int run_phase(lbxb* b, int phase /*0 LB, 1 MP*/, int64_t S) {
     auto it = b->graph_cache.find(key);
     const bool xr = b->n_xrows > 0;
     if (it != b->graph_cache.end() &&
-        (it->second.block != block || it->second.grid_tiles != b->hw_tiles || it->second.xr != xr)) {
+        (it->second.block != block || it->second.grid_tiles != b->n_act[phase] || it->second.xr != xr)) {
         (void)hipGraphExecDestroy(it->second.exec);
         b->graph_cache.erase(it);
         it = b->graph_cache.end();
@@ -644,7 +689,7 @@ int run_phase(lbxb* b, int phase /*0 LB, 1 MP*/, int64_t S) {
         (void)hipGraphDestroy(graph);
         if (ie != hipSuccess) return hip_status(ie);
         gr.block = block;
-        gr.grid_tiles = b->hw_tiles;
+        gr.grid_tiles = b->n_act[phase];
         gr.xr = xr;
         it = b->graph_cache.emplace(key, gr).first;
     }
@@ -656,7 +701,7 @@ int run_phase(lbxb* b, int phase /*0 LB, 1 MP*/, int64_t S) {
 int run_phase_persistent(lbxb* b, int phase, int64_t S, int32_t redo_only) {
     if (S <= 0) return LBX_OK;
     const int block = b->lb_block;
-    const uint32_t ntiles = grid_of(b, block);
+    const uint32_t ntiles = grid_of(b, block, phase);
     if (!ntiles) return LBX_OK;
     const bool xr = b->n_xrows > 0;
     auto go = [&](auto etag, auto rtag, auto btag, auto wide) -> int {
@@ -845,6 +890,8 @@ int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
     if (!st) st = dalloc(&b->nodes, Np);
     if (!st) st = dalloc(&b->maps, (uint64_t)capacity_box);
     if (!st) st = dalloc(&b->tile_inst, (uint64_t)b->ntiles_cap);
+    if (!st) st = dalloc(&b->act[0], (uint64_t)b->ntiles_cap);
+    if (!st) st = dalloc(&b->act[1], (uint64_t)b->ntiles_cap);
     if (!st) st = dalloc(&b->insts, (uint64_t)max_instances);
     if (!st) st = dalloc(&b->slots, (uint64_t)max_instances * 3 * kSlots);
     if (!st) st = dalloc(&b->l2ring, (uint64_t)max_instances * kRing);
@@ -1000,7 +1047,6 @@ int lbxb_add(lbxb* b, int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, 
     }
     b->tab[id] = in;
     b->hi[id] = std::move(h);
-    b->hw_tiles = std::max<uint32_t>(b->hw_tiles, (uint32_t)(tile0 + ntiles));
     if (lx > 1) ++b->n_xrows;
     *id_out = (int32_t)id;
     return LBX_OK;
@@ -1018,10 +1064,6 @@ int lbxb_remove(lbxb* b, int32_t id) {
     if (b->hi[id].lx > 1) --b->n_xrows;
     b->hi[id] = HostInst{};
     in = BInst{};
-    uint32_t hw = 0;
-    for (uint32_t t = 0; t < b->ntiles_cap; ++t)
-        if (b->tile_inst_h[t] != kNoInst) hw = t + 1;
-    b->hw_tiles = hw;
     return hip_status(hipStreamSynchronize(b->stream));
 }
 
@@ -1057,6 +1099,7 @@ int lbxb_round(lbxb* b, int32_t lb_steps, int32_t mp_steps, int32_t* active) {
     }
     if (any_redo) {
         LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
+        LBX_TRY(upload_active(b));
         launch_lb(b, 0, b->lb_block);
         LBX_TRY(hipGetLastError());
         LBX_TRY(hipStreamSynchronize(b->stream));  // the table upload has been read
@@ -1098,6 +1141,7 @@ int lbxb_round(lbxb* b, int32_t lb_steps, int32_t mp_steps, int32_t* active) {
         }
     }
     LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
+    LBX_TRY(upload_active(b));
     if (S_lb > 0) {
         LBX_TRY(hipEventRecord(b->ev[0], b->stream));
         LBX_TRY(b->persistent ? run_phase_persistent(b, 0, S_lb, 0) : run_phase(b, 0, S_lb));
@@ -1306,6 +1350,7 @@ int lbxb_autotune(lbxb* b, int32_t launches, int32_t* lb_block, int32_t* mp_bloc
         }
     }
     LBX_TRY(hipMemcpyAsync(b->insts, b->tab.data(), ni * sizeof(BInst), hipMemcpyHostToDevice, b->stream));
+    LBX_TRY(upload_active(b));
     auto launch = [&](int phase, int block) {
         if (phase == 0) launch_lb(b, 0, block); else launch_mp(b, 0, block, 0);
     };

@@ -107,15 +107,22 @@ def run_batch(reqs, precision, lb_round=128, mp_round=128, autotune=True, launch
         tuned = []
         t0 = time.perf_counter()
         rounds = 0
-        mp_seen = False
+        mp_prev, mp_tuned = 0, 0
         while True:
             active = B.round(lb_round, mp_round)
             rounds += 1
             if autotune and rounds == 2:
                 tuned.append(B.autotune(5))  # LB phase
-            if autotune and not mp_seen and B.stats()["mp_fluid_updates"] > 0:
-                mp_seen = True
-                tuned.append(B.autotune(5))  # once MP has started
+            if autotune:
+                # MP work of this round; tuned again whenever it has more than doubled
+                # since the last tuning (instances of mixed sizes enter MP at different
+                # times, and the block size that suits the first small ones does not
+                # suit the large ones)
+                mp_now = B.stats()["mp_fluid_updates"]
+                d, mp_prev = mp_now - mp_prev, mp_now
+                if d > 2 * mp_tuned:
+                    tuned.append(B.autotune(5))
+                    mp_tuned = d
             if active == 0:
                 break
         wall = time.perf_counter() - t0
@@ -136,8 +143,10 @@ def run_batch(reqs, precision, lb_round=128, mp_round=128, autotune=True, launch
 
 
 def run_one_at_a_time(reqs, precision):
-    """The plain single-lattice path, one request after the other."""
+    """The plain single-lattice path, one request after the other; returns
+    the aggregate rate and each request's D_eff."""
     box = 0
+    deff = []
     t0 = time.perf_counter()
     for r in reqs:
         with lbx.Lattice(r["dims"], r["nature"], r["tau"], precision=precision) as L:
@@ -145,9 +154,10 @@ def run_one_at_a_time(reqs, precision):
             e = L.equilibrate(r["fext"], r["target"], max_steps=r["max_lb"], trace_cap=1)
             L.mp_init(r["fext"], r["Db"], r["ka"], r["kd"])
             m = L.mp_run(r["mp_steps"])
+            deff.append(L.mp_deff().tolist())
             box += int(np.prod(r["dims"])) * (e["steps"] + m["steps"])
     wall = time.perf_counter() - t0
-    return dict(wall_s=wall, mlups=box / wall / 1e6, requests=len(reqs))
+    return dict(wall_s=wall, mlups=box / wall / 1e6, requests=len(reqs), deff=deff)
 
 
 # ------------------------------------------------------------------ serving
@@ -274,7 +284,8 @@ def main(argv=None):
                 out[f"speedup_{p}_batch_vs_fp64_batch"] = round(res[p]["mlups"] / res["fp64"]["mlups"], 3)
         if args.single_sample > 0:
             one = run_one_at_a_time(reqs[:args.single_sample], lbx.PREC_FP64)
-            out["plain_fp64_one_at_a_time"] = {k: round(v, 2) if isinstance(v, float) else v for k, v in one.items()}
+            out["plain_fp64_one_at_a_time"] = {k: round(v, 2) if isinstance(v, float) else v
+                                               for k, v in one.items() if k != "deff"}
             for p in ("fp32_shift", "fp32_store"):
                 if p in res:
                     out[f"speedup_{p}_batch_vs_plain_fp64"] = round(res[p]["mlups"] / one["mlups"], 2)
