@@ -144,7 +144,7 @@ __device__ __forceinline__ void kb_lb_tile(const BatchView<E, R>& v, uint32_t ti
 }
 
 template <int BLOCK, bool Wide, typename E, typename R = E>
-__global__ __launch_bounds__(BLOCK) void kb_lb_step(BatchView<E, R> v, int64_t k) {
+__global__ __launch_bounds__(BLOCK, (kLbMinWaves<E, R>)) void kb_lb_step(BatchView<E, R> v, int64_t k) {
     kb_lb_tile<BLOCK, Wide, E, R>(v, active_tile<BLOCK>(v.act, xcd_tile()), k);
 }
 

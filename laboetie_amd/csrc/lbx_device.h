@@ -98,7 +98,18 @@ __device__ __forceinline__ double collide_pop(double f, double rho, double jx, d
 // rho0.  V = double (the FP32-store form) runs the same operations in FP64 on
 // the FP32-stored deviations.  In FP32 the constants are rounded so that the
 // collision still conserves momentum: a1 from a1_f32, and A, B from
-// fp32_relaxation (A + B = 1).
+// fp32_relaxation (A + B = 1).  In FP64 each sum of a product and a term is
+// one fused multiply-add (madd): fewer operations and registers, so the
+// FP32-store step runs with five waves per SIMD; the restatement of the form
+// fuses the same operations (C fma).
+template <typename V>
+__device__ __forceinline__ V madd(V a, V b, V c) {
+    if constexpr (std::is_same<V, double>::value)
+        return __builtin_fma(a, b, c);
+    else
+        return c + a * b;  // FP32: unfused, the form's own order
+}
+
 template <int L, typename V = float>
 __device__ __forceinline__ V collide_shift(V g, V drho, V jx, V jy, V jz, V ux, V uy, V uz, V fx, V fy, V fz, V A,
                                            V B, V C) {
@@ -112,17 +123,35 @@ __device__ __forceinline__ V collide_shift(V g, V drho, V jx, V jy, V jz, V ux, 
     if constexpr (cx * cy != 0) q = q + sgn<cx * cy>(jx * uy);
     if constexpr (cx * cz != 0) q = q + sgn<cx * cz>(jx * uz);
     if constexpr (cy * cx != 0) q = q + sgn<cy * cx>(jy * ux);
-    q = q + (jy * uy) * Ky;
+    q = madd<V>(jy * uy, Ky, q);
     if constexpr (cy * cz != 0) q = q + sgn<cy * cz>(jy * uz);
     if constexpr (cz * cx != 0) q = q + sgn<cz * cx>(jz * ux);
     if constexpr (cz * cy != 0) q = q + sgn<cz * cy>(jz * uy);
-    q = q + (jz * uz) * Kz;
+    q = madd<V>(jz * uz, Kz, q);
     V dneq = w0 * drho;
-    if constexpr (L != 0) dneq = dneq + w1 * dot3<cx, cy, cz>(jx, jy, jz);
-    dneq = dneq + w2 * q;
-    V G = w1 * ((((V)cx - ux) * fx + ((V)cy - uy) * fy) + ((V)cz - uz) * fz);
-    if constexpr (L != 0) G = G + (((V)2 * w2) * dot3<cx, cy, cz>(ux, uy, uz)) * dot3<cx, cy, cz>(fx, fy, fz);
-    return (A * g + B * dneq) + C * G;
+    if constexpr (L != 0) dneq = madd<V>(w1, dot3<cx, cy, cz>(jx, jy, jz), dneq);
+    dneq = madd<V>(w2, q, dneq);
+    V G = w1 * madd<V>((V)cz - uz, fz, madd<V>((V)cy - uy, fy, ((V)cx - ux) * fx));
+    if constexpr (L != 0)
+        G = madd<V>(((V)2 * w2) * dot3<cx, cy, cz>(ux, uy, uz), dot3<cx, cy, cz>(fx, fy, fz), G);
+    return madd<V>(C, G, madd<V>(B, dneq, A * g));
+}
+
+// u = j / rho of the FP32 storage forms' collision: FP32-shift divides each
+// component; FP32-store (R = double) multiplies by one reciprocal (the form's
+// restatement does the same).
+template <typename R>
+__device__ __forceinline__ void shift_velocity(R jx, R jy, R jz, R full, R& ux, R& uy, R& uz) {
+    if constexpr (std::is_same<R, double>::value) {
+        const R inv = R(1) / full;
+        ux = jx * inv;
+        uy = jy * inv;
+        uz = jz * inv;
+    } else {
+        ux = jx / full;
+        uy = jy / full;
+        uz = jz / full;
+    }
 }
 
 // Division by a runtime-invariant divisor 1 <= d < 2^31 as a multiply-high

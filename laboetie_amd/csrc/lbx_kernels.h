@@ -349,7 +349,8 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
         jsum[2] = (double)jz;
         if constexpr (kShift) {
             const R drho = rho, full = a.rho0 + drho;
-            const R ux = jx / full, uy = jy / full, uz = jz / full;
+            R ux, uy, uz;
+            shift_velocity<R>(jx, jy, jz, full, ux, uy, uz);
             static_for<0, kQ>([&](auto I) {
                 constexpr int L = I;
                 fout.st((uint32_t)L * Np + i, (E)collide_shift<L, R>(f[L], drho, jx, jy, jz, ux, uy, uz, a.fcx,
@@ -388,8 +389,13 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
     }
 }
 
+// The FP32-store step (FP32 storage, FP64 arithmetic) fits five waves per
+// SIMD without spilling; the compiler's own choice leaves it at four.
+template <typename E, typename R>
+constexpr int kLbMinWaves = (std::is_same<E, float>::value && std::is_same<R, double>::value) ? 5 : LBX_LB_WAVES;
+
 template <bool Wide, typename E, typename R = E>
-__global__ __launch_bounds__(kBlock, LBX_LB_WAVES) void k_lb_step(StepArgs<E, R> a) {
+__global__ __launch_bounds__(kBlock, (kLbMinWaves<E, R>)) void k_lb_step(StepArgs<E, R> a) {
     lb_step_tile<kBlock, Wide, E, R>(a, xcd_tile(), blockIdx.x, blockIdx.x == 0);
 }
 
@@ -554,7 +560,8 @@ __global__ __launch_bounds__(kBlock) void k_collide_packed_shift(float* g, const
     jold[i] = jx;  // j_old of the first step
     jold[Np + i] = jy;
     jold[2 * Np + i] = jz;
-    const R ux = jx / full, uy = jy / full, uz = jz / full;
+    R ux, uy, uz;
+    shift_velocity<R>(jx, jy, jz, full, ux, uy, uz);
     static_for<0, kQ>([&](auto I) {
         constexpr int L = I;
         const R v = g[(uint64_t)L * Np + i];

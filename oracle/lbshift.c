@@ -60,23 +60,45 @@ static inline int32_t pbc(int32_t i, int32_t n) {
 #define FABS fabsf
 #define FMAX fmaxf
 #define SFX(x) lbshift_##x
+/* FP32: a product added to a term, unfused; u = j / rho per component */
+#define MADD(a, b, c) ((c) + (a) * (b))
+#define SHIFT_VELOCITY(jx, jy, jz, full, ux, uy, uz) \
+    do {                                            \
+        ux = (jx) / (full);                         \
+        uy = (jy) / (full);                         \
+        uz = (jz) / (full);                         \
+    } while (0)
 #include "lbshift_eq.inc"
 #undef REAL
 #undef FABS
 #undef FMAX
 #undef SFX
+#undef MADD
+#undef SHIFT_VELOCITY
 
 /* the FP32-store form: the same storage, collision in FP64 */
 #define REAL double
 #define FABS fabs
 #define FMAX fmax
 #define SFX(x) lbstore_##x
+/* FP64 (the library's collide_shift<L, double>, madd): one fused multiply-add,
+ * C99 fma (correctly rounded); u = j * (1 / rho) */
+#define MADD(a, b, c) fma((a), (b), (c))
+#define SHIFT_VELOCITY(jx, jy, jz, full, ux, uy, uz) \
+    do {                                            \
+        const double inv_ = 1.0 / (full);           \
+        ux = (jx) * inv_;                           \
+        uy = (jy) * inv_;                           \
+        uz = (jz) * inv_;                           \
+    } while (0)
 #include "lbshift_eq.inc"
 #undef REAL
 #undef STORE_T
 #undef FABS
 #undef FMAX
 #undef SFX
+#undef MADD
+#undef SHIFT_VELOCITY
 
 /* Moment propagation in the FP32-shift form (probabilities stored in FP32),
  * per node in the library's gather order: P(r,t+1) = sum_l P(r+c_l,t)
