@@ -115,17 +115,25 @@ extern "C" int lbx_detect_interfacial(int32_t lx, int32_t ly, int32_t lz, const 
                                       int8_t* interfacial) {
     using namespace lbx;
     if (lx <= 0 || ly <= 0 || lz <= 0 || !nature || !interfacial) return LBX_ERR_ARG;
+    // row by row: the 9 rows (y+dy, z+dz) a node's neighbours lie in, wrapped
+    // once per row, and the x-neighbours wrapped per node (an axis of length 1
+    // wraps onto itself); a node is interfacial if any of the 18 differs
     for (int32_t k = 0; k < lz; ++k)
-        for (int32_t j = 0; j < ly; ++j)
+        for (int32_t j = 0; j < ly; ++j) {
+            const int8_t* rows[3][3];
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz)
+                    rows[dy + 1][dz + 1] = nature + idx(0, wrap(j + dy, ly), wrap(k + dz, lz), lx, ly);
+            const int8_t* c = rows[1][1];
+            int8_t* out = interfacial + idx(0, j, k, lx, ly);
             for (int32_t i = 0; i < lx; ++i) {
-                const int64_t r = idx(i, j, k, lx, ly);
-                int8_t f = 0;
-                for (int l = 1; l < 19 && !f; ++l) {
-                    const int64_t rp = idx(wrap(i + kCX[l], lx), wrap(j + kCY[l], ly), wrap(k + kCZ[l], lz), lx, ly);
-                    if (nature[rp] != nature[r]) f = 1;
-                }
-                interfacial[r] = f;
+                const int32_t xs[3] = {wrap(i - 1, lx), i, wrap(i + 1, lx)};
+                const int8_t v = c[i];
+                int f = 0;
+                for (int l = 1; l < 19; ++l) f |= rows[kCY[l] + 1][kCZ[l] + 1][xs[kCX[l] + 1]] != v;
+                out[i] = (int8_t)f;
             }
+        }
     return LBX_OK;
 }
 
