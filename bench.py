@@ -32,10 +32,12 @@ shortened precision run ("precision_256": the error of every output of the
 FP32 forms against FP64; "roofline_256": both kernels in all three forms)
 runs outside the timed region.  The other configurations run beside it,
 also outside the timed region: the 4x4x34 slit ("slit_4x4x34"), the
-1024-instance batch ("batch_1024", FP64, FP32-shift and FP32-store), a
-bounded serving stream ("serving": p50/p99 latency and MLUPS at 0.5x, 1x and
-2x of saturation, FP32-shift), and, in "cpu_baseline", the reference program
-on a bounded sample of the job and of each of them.
+1024-instance batch ("batch_1024", FP64, FP32-shift and FP32-store, each
+against the FP64 batch and against the plain FP64 path on a 16-instance
+sample), a bounded serving stream ("serving": p50/p99 latency and MLUPS at
+0.5x, 1x and 2x of saturation, FP32-shift; its saturation requests also on the
+plain FP64 path, for the rate and D_eff comparison), and, in "cpu_baseline",
+the reference program on a bounded sample of the job and of each of them.
 """
 import argparse
 import json
