@@ -868,7 +868,10 @@ int lbxb_create(int32_t precision, int64_t capacity_nodes, int64_t capacity_box,
     b->cap_box = (uint64_t)capacity_box;
     b->cap_inst = (uint32_t)max_instances;
     b->ntiles_cap = (uint32_t)(Np / kTile);
-    b->wide = b->wide_auto = kQ * Np * 8 >= (1ull << 32);
+    // 32-bit byte offsets reach every array while the populations (the
+    // largest: 19 per node, FP64 or FP32 by form) stay below 4 GiB
+    b->wide = b->wide_auto =
+        (uint64_t)kQ * Np * (precision == LBX_PREC_FP64 ? sizeof(double) : sizeof(float)) >= (1ull << 32);
     {
         int dev = 0;
         (void)hipGetDevice(&dev);
