@@ -67,6 +67,7 @@ struct lbx_ctx {
     uint64_t Np = 0;  // packed plane stride
     bool wide = false; // 19*Np*8 >= 2^32: 64-bit addressing of the population buffers
     bool wide_auto = false;  // what the size alone selects
+    bool nt = false;  // LB population stores with the streaming hint (sets beyond the last-level cache)
     int32_t prec = LBX_PREC_FP64;
     double tau = 1.0, A = 0, B = 0, C = 0;
     float A32 = 0, B32 = 0;      // FP32-shift collision: A + B = 1 exactly (fp32_relaxation)
@@ -208,6 +209,9 @@ void free_mp(lbx_ctx* c) {
 
 #ifndef LBX_MP_BAND
 #define LBX_MP_BAND 32
+#endif
+#ifndef LBX_NT_MIN_BYTES  // population footprint above which LB stores stream
+#define LBX_NT_MIN_BYTES (256ull << 20)
 #endif
 // MP tile order: bands of kMpBandRows box rows, plane by plane within a band,
 // so that the z-neighbour rows a tile gathers were read by tiles that ran just
@@ -362,18 +366,21 @@ void launch_lb_step(lbx_ctx* c, int64_t t, const double fm[3], const double fc[3
     if (c->prec == LBX_PREC_FP32_SHIFT) {
         StepArgs<float> a = step_args<float>(c, t, fm, fc, check_after, target, write_l2, check_every, check_base);
         a.have_prev = have_prev;
-        auto kern = c->wide ? k_lb_step<true, float> : k_lb_step<false, float>;
+        auto kern = c->wide ? (c->nt ? k_lb_step<true, float, float, true> : k_lb_step<true, float>)
+                            : (c->nt ? k_lb_step<false, float, float, true> : k_lb_step<false, float>);
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
     } else if (c->prec == LBX_PREC_FP32_STORE) {
         StepArgs<float, double> a =
             step_args<float, double>(c, t, fm, fc, check_after, target, write_l2, check_every, check_base);
         a.have_prev = have_prev;
-        auto kern = c->wide ? k_lb_step<true, float, double> : k_lb_step<false, float, double>;
+        auto kern = c->wide ? (c->nt ? k_lb_step<true, float, double, true> : k_lb_step<true, float, double>)
+                            : (c->nt ? k_lb_step<false, float, double, true> : k_lb_step<false, float, double>);
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
     } else {
         StepArgs<double> a = step_args<double>(c, t, fm, fc, check_after, target, write_l2, check_every, check_base);
         a.have_prev = have_prev;
-        auto kern = c->wide ? k_lb_step<true, double> : k_lb_step<false, double>;
+        auto kern = c->wide ? (c->nt ? k_lb_step<true, double, double, true> : k_lb_step<true, double>)
+                            : (c->nt ? k_lb_step<false, double, double, true> : k_lb_step<false, double>);
         hipLaunchKernelGGL(kern, dim3(gt), dim3(kBlock), 0, c->stream, a);
     }
 }
@@ -622,6 +629,10 @@ int lbx_create(int32_t lx, int32_t ly, int32_t lz, const int8_t* nature, double 
     int st = LBX_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) st = LBX_ERR_DEVICE;
     const bool shift = f32_storage(precision);
+    // a step reads one population set and writes the other: once the two
+    // together exceed the 256 MiB last-level cache, no written line survives
+    // there to the next step
+    c->nt = 2 * kQ * c->Np * (shift ? sizeof(float) : sizeof(double)) > (uint64_t)LBX_NT_MIN_BYTES;
     if (!st && !shift) st = dalloc(&c->f[0], kQ * c->Np);
     if (!st && !shift) st = dalloc(&c->f[1], kQ * c->Np);
     if (!st && shift) st = dalloc(&c->g[0], kQ * c->Np);

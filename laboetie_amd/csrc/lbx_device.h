@@ -192,6 +192,8 @@ __device__ __forceinline__ void st(T* base, uint32_t idx, T v) {
 // 0x00020000 is the gfx9-family descriptor word 3 for raw 32-bit-dword data.
 // E is the element type: double (b64 accesses) or float (b32).
 constexpr int kBufWord3 = 0x00020000;
+// the nt bit of a gfx940-family memory instruction's cache-policy field
+constexpr int kNonTemporal = 2;
 template <typename E>
 using Vec2 = E __attribute__((ext_vector_type(2)));
 template <typename E>
@@ -208,12 +210,15 @@ struct Plane {
         else
             return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4u, 0, 0));
     }
+    // CP is the store's cache-policy field: 0 the default, kNonTemporal the
+    // streaming hint (the line is not kept for reuse in the caches)
+    template <int CP = 0>
     __device__ __forceinline__ void st(uint32_t idx, E v) const {
         if constexpr (sizeof(E) == 8)
             __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
-                                                  r, idx * 8u, 0, 0);
+                                                  r, idx * 8u, 0, CP);
         else
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, idx * 4u, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, idx * 4u, 0, CP);
     }
     // elements 2*pidx and 2*pidx+1 in one access (b128 for double, b64 for float)
     __device__ __forceinline__ Vec2<E> ld2(uint32_t pidx) const {
@@ -243,6 +248,7 @@ struct View<false, E> {
     __device__ __forceinline__ View(const E* base, uint64_t n) : p(base, n * sizeof(E)) {}
     __device__ __forceinline__ E ld(uint32_t idx) const { return p.ld(idx); }
     __device__ __forceinline__ void st(uint32_t idx, E v) const { p.st(idx, v); }
+    __device__ __forceinline__ void stnt(uint32_t idx, E v) const { p.template st<kNonTemporal>(idx, v); }
     __device__ __forceinline__ Vec2<E> ld2(uint32_t pidx) const { return p.ld2(pidx); }
     __device__ __forceinline__ void st2(uint32_t pidx, Vec2<E> v) const { p.st2(pidx, v); }
 };
@@ -252,6 +258,7 @@ struct View<true, E> {
     __device__ __forceinline__ View(const E* base, uint64_t) : b(const_cast<E*>(base)) {}
     __device__ __forceinline__ E ld(uint32_t idx) const { return b[(uint64_t)idx]; }
     __device__ __forceinline__ void st(uint32_t idx, E v) const { b[(uint64_t)idx] = v; }
+    __device__ __forceinline__ void stnt(uint32_t idx, E v) const { __builtin_nontemporal_store(v, b + (uint64_t)idx); }
     __device__ __forceinline__ Vec2<E> ld2(uint32_t pidx) const { return ((const Vec2<E>*)b)[(uint64_t)pidx]; }
     __device__ __forceinline__ void st2(uint32_t pidx, Vec2<E> v) const { ((Vec2<E>*)b)[(uint64_t)pidx] = v; }
 };

@@ -227,7 +227,7 @@ __device__ __forceinline__ E pull(const View<Wide, E>& fin, uint32_t Np, uint32_
 // consecutive packed nodes, i = tile*BLOCK + lane, valid below a.Nf), `stripe`
 // picks its l2err slot, and `first_block` is the one block of the lattice that
 // writes the step's control fields and clears the next step's slots.
-template <int BLOCK, bool Wide, typename E, typename R = E>
+template <int BLOCK, bool Wide, typename E, typename R = E, bool NT = false>
 __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t tile, uint32_t stripe,
                                              bool first_block) {
     constexpr bool kShift = std::is_same<E, float>::value;  // deviations stored in FP32
@@ -353,15 +353,22 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
             shift_velocity<R>(jx, jy, jz, full, ux, uy, uz);
             static_for<0, kQ>([&](auto I) {
                 constexpr int L = I;
-                fout.st((uint32_t)L * Np + i, (E)collide_shift<L, R>(f[L], drho, jx, jy, jz, ux, uy, uz, a.fcx,
-                                                                     a.fcy, a.fcz, a.A, a.B, a.C));
+                const E v = (E)collide_shift<L, R>(f[L], drho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy, a.fcz, a.A,
+                                                   a.B, a.C);
+                if constexpr (NT)
+                    fout.stnt((uint32_t)L * Np + i, v);
+                else
+                    fout.st((uint32_t)L * Np + i, v);
             });
         } else {
             const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
             static_for<0, kQ>([&](auto I) {
                 constexpr int L = I;
-                fout.st((uint32_t)L * Np + i,
-                        collide_pop<L>(f[L], rho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy, a.fcz, a.A, a.B, a.C));
+                const double v = collide_pop<L>(f[L], rho, jx, jy, jz, ux, uy, uz, a.fcx, a.fcy, a.fcz, a.A, a.B, a.C);
+                if constexpr (NT)
+                    fout.stnt((uint32_t)L * Np + i, v);
+                else
+                    fout.st((uint32_t)L * Np + i, v);
             });
         }
     }
@@ -394,9 +401,12 @@ __device__ __forceinline__ void lb_step_tile(const StepArgs<E, R>& a, uint32_t t
 template <typename E, typename R>
 constexpr int kLbMinWaves = (std::is_same<E, float>::value && std::is_same<R, double>::value) ? 5 : LBX_LB_WAVES;
 
-template <bool Wide, typename E, typename R = E>
+// NT: the outgoing populations are written with the streaming hint, for a
+// lattice whose two population sets outgrow the last-level cache, where the
+// next step's reads find none of them there anyway.
+template <bool Wide, typename E, typename R = E, bool NT = false>
 __global__ __launch_bounds__(kBlock, (kLbMinWaves<E, R>)) void k_lb_step(StepArgs<E, R> a) {
-    lb_step_tile<kBlock, Wide, E, R>(a, xcd_tile(), blockIdx.x, blockIdx.x == 0);
+    lb_step_tile<kBlock, Wide, E, R, NT>(a, xcd_tile(), blockIdx.x, blockIdx.x == 0);
 }
 
 // l2err of the last step of a launch run, which no later step kernel reads
