@@ -210,9 +210,6 @@ void free_mp(lbx_ctx* c) {
 #ifndef LBX_MP_BAND
 #define LBX_MP_BAND 32
 #endif
-#ifndef LBX_NT_MIN_BYTES  // population footprint above which LB stores stream
-#define LBX_NT_MIN_BYTES (256ull << 20)
-#endif
 // MP tile order: bands of kMpBandRows box rows, plane by plane within a band,
 // so that the z-neighbour rows a tile gathers were read by tiles that ran just
 // before it on the same XCD (0: the packed order).  Each tile still covers

@@ -98,8 +98,9 @@ __device__ __forceinline__ uint32_t active_tile(const uint32_t* act, uint32_t l)
     return act[l / per] * per + l % per;
 }
 
-// LB launch k of a round on one BLOCK-node tile (a redo repeats step lb_t0)
-template <int BLOCK, bool Wide, typename E, typename R = E>
+// LB launch k of a round on one BLOCK-node tile (a redo repeats step lb_t0);
+// NT: population stores with the streaming hint
+template <int BLOCK, bool Wide, typename E, typename R = E, bool NT = false>
 __device__ __forceinline__ void kb_lb_tile(const BatchView<E, R>& v, uint32_t tile, int64_t k) {
     const uint32_t id = v.tile_inst[tile / (kTile / BLOCK)];
     if (id == kNoInst) return;
@@ -140,12 +141,12 @@ __device__ __forceinline__ void kb_lb_tile(const BatchView<E, R>& v, uint32_t ti
     a.check_base = in.check_base;
     a.check_inv = (R)in.check_inv;
     const uint32_t first = in.tile_first * (kTile / BLOCK);
-    lb_step_tile<BLOCK, Wide, E, R>(a, tile, tile - first, tile == first);
+    lb_step_tile<BLOCK, Wide, E, R, NT>(a, tile, tile - first, tile == first);
 }
 
-template <int BLOCK, bool Wide, typename E, typename R = E>
+template <int BLOCK, bool Wide, typename E, typename R = E, bool NT = false>
 __global__ __launch_bounds__(BLOCK, (kLbMinWaves<E, R>)) void kb_lb_step(BatchView<E, R> v, int64_t k) {
-    kb_lb_tile<BLOCK, Wide, E, R>(v, active_tile<BLOCK>(v.act, xcd_tile()), k);
+    kb_lb_tile<BLOCK, Wide, E, R, NT>(v, active_tile<BLOCK>(v.act, xcd_tile()), k);
 }
 
 // l2err of each LB instance's last step of the round (one wave per instance slot)
@@ -589,6 +590,16 @@ void with_block(int block, F&& f) {
         f(std::integral_constant<int, 256>{});
 }
 
+// the LB kernel of one launch: addressing form, and streaming stores when the
+// active tiles' two population sets outgrow the last-level cache (a function
+// of the active tile count, which a cached graph's validity already checks)
+template <int BL, typename E, typename R = E>
+auto lb_kernel(const lbxb* b) {
+    const bool nt = 2ull * kQ * b->n_act[0] * kTile * sizeof(E) > (uint64_t)LBX_NT_MIN_BYTES;
+    if (b->wide) return nt ? kb_lb_step<BL, true, E, R, true> : kb_lb_step<BL, true, E, R>;
+    return nt ? kb_lb_step<BL, false, E, R, true> : kb_lb_step<BL, false, E, R>;
+}
+
 // launch LB step k of a round over the tiles of the instances with LB work
 void launch_lb(lbxb* b, int64_t k, int block) {
     const uint32_t grid = grid_of(b, block, 0);
@@ -597,16 +608,13 @@ void launch_lb(lbxb* b, int64_t k, int block) {
         constexpr int BL = decltype(BT)::value;
         if (b->prec == LBX_PREC_FP32_SHIFT) {
             auto v = lb_view<float>(b);
-            auto kern = b->wide ? kb_lb_step<BL, true, float> : kb_lb_step<BL, false, float>;
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k);
+            hipLaunchKernelGGL((lb_kernel<BL, float>(b)), dim3(grid), dim3(BL), 0, b->stream, v, k);
         } else if (b->prec == LBX_PREC_FP32_STORE) {
             auto v = lb_view<float, double>(b);
-            auto kern = b->wide ? kb_lb_step<BL, true, float, double> : kb_lb_step<BL, false, float, double>;
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k);
+            hipLaunchKernelGGL((lb_kernel<BL, float, double>(b)), dim3(grid), dim3(BL), 0, b->stream, v, k);
         } else {
             auto v = lb_view<double>(b);
-            auto kern = b->wide ? kb_lb_step<BL, true, double> : kb_lb_step<BL, false, double>;
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(BL), 0, b->stream, v, k);
+            hipLaunchKernelGGL((lb_kernel<BL, double>(b)), dim3(grid), dim3(BL), 0, b->stream, v, k);
         }
     });
 }

@@ -404,6 +404,9 @@ constexpr int kLbMinWaves = (std::is_same<E, float>::value && std::is_same<R, do
 // NT: the outgoing populations are written with the streaming hint, for a
 // lattice whose two population sets outgrow the last-level cache, where the
 // next step's reads find none of them there anyway.
+#ifndef LBX_NT_MIN_BYTES  // population footprint (both sets) above which LB stores stream
+#define LBX_NT_MIN_BYTES (256ull << 20)
+#endif
 template <bool Wide, typename E, typename R = E, bool NT = false>
 __global__ __launch_bounds__(kBlock, (kLbMinWaves<E, R>)) void k_lb_step(StepArgs<E, R> a) {
     lb_step_tile<kBlock, Wide, E, R, NT>(a, xcd_tile(), blockIdx.x, blockIdx.x == 0);
