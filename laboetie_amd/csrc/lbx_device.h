@@ -336,10 +336,36 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// v of the lane that DPP control CTRL selects within the lane's row of 16
+// (all 64 lanes active)
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes of a wave in a fixed order, returned to every lane
+// (all 64 lanes active).  Within each row of 16 lanes, register moves only:
+// lane pairs (quad_perm 1,0,3,2), pairs of pairs (quad_perm 2,3,0,1), half
+// rows (row_half_mirror) and rows (row_mirror); each step adds two partial
+// sums in either order, so every lane of a row ends with the same bits.  The
+// four row sums are then read from lanes 0, 16, 32, 48 and added as
+// (r0 + r1) + (r2 + r3).
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
-    return v;  // lane 0 holds the sum, fixed tree
+    v = v + dpp_row<0xB1>(v);
+    v = v + dpp_row<0x4E>(v);
+    v = v + dpp_row<0x141>(v);
+    v = v + dpp_row<0x140>(v);
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // Lane k of the wave receives lane k+1's value (S = +1, DPP wave_shl:1) or
