@@ -330,11 +330,6 @@ __device__ __forceinline__ uint32_t xcd_tile() {
     return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
 
 // v of the lane that DPP control CTRL selects within the lane's row of 16
 // (all 64 lanes active)
@@ -366,6 +361,16 @@ __device__ __forceinline__ double wave_sum(double v) {
     v = v + dpp_row<0x141>(v);
     v = v + dpp_row<0x140>(v);
     return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
+// Maximum over the 64 lanes of a wave (fmax: a NaN lane is passed over),
+// returned to every lane (all 64 lanes active); the same row steps as wave_sum.
+__device__ __forceinline__ double wave_max(double v) {
+    v = fmax(v, dpp_row<0xB1>(v));
+    v = fmax(v, dpp_row<0x4E>(v));
+    v = fmax(v, dpp_row<0x141>(v));
+    v = fmax(v, dpp_row<0x140>(v));
+    return fmax(fmax(read_lane(v, 0), read_lane(v, 16)), fmax(read_lane(v, 32), read_lane(v, 48)));
 }
 
 // Lane k of the wave receives lane k+1's value (S = +1, DPP wave_shl:1) or

@@ -161,13 +161,29 @@ constexpr unsigned long long kNegMark = 0x7FF0000000000000ull;
 // the floor test needs l2err this far below the stage's maximum
 constexpr double kFloorDrop = 1e-2;
 
+// maximum over the 64 lanes of a wave, returned to every lane (all 64 lanes
+// active): DPP row steps, then the four row maxima
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
+    const auto step = [&](auto ctrl) {
+        constexpr int C = decltype(ctrl)::value;
+        const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)v, C, 0xf, 0xf, false);
+        const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), C, 0xf, 0xf, false);
+        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
         v = o > v ? o : v;
+    };
+    step(std::integral_constant<int, 0xB1>{});
+    step(std::integral_constant<int, 0x4E>{});
+    step(std::integral_constant<int, 0x141>{});
+    step(std::integral_constant<int, 0x140>{});
+    unsigned long long m = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 16 * r);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 16 * r);
+        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+        m = o > m ? o : m;
     }
-    return v;
+    return m;
 }
 
 // max over the kSlots slots of one step (called by the first wave of a block)
@@ -175,10 +191,11 @@ __device__ __forceinline__ unsigned long long slots_max(const unsigned long long
     return wave_max_u64(vload(&set[threadIdx.x & (kSlots - 1)]));
 }
 
-// Per-block sums of NV values in a fixed order, written by thread 0 to
-// out[0 .. NV-1].
+// Per-block sums of NV values in a fixed order, written to out[0 .. NV-1]
+// (value a by thread a).
 template <int NV, int BLOCK>
 __device__ __forceinline__ void block_partial(const double (&v)[NV], double* out, double* lds /* [BLOCK/64*NV] */) {
+    static_assert(NV <= BLOCK, "one thread per value");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < NV; ++a) {
@@ -186,12 +203,11 @@ __device__ __forceinline__ void block_partial(const double (&v)[NV], double* out
         if (lane == 0) lds[w * NV + a] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int a = 0; a < NV; ++a) {
-            double s = lds[a];
-            for (int k = 1; k < BLOCK / 64; ++k) s = s + lds[k * NV + a];
-            out[a] = s;
-        }
+    if (threadIdx.x < NV) {  // value a summed over the waves in wave order, by thread a
+        const int a = threadIdx.x;
+        double s = lds[a];
+        for (int k = 1; k < BLOCK / 64; ++k) s = s + lds[k * NV + a];
+        out[a] = s;
     }
     __syncthreads();
 }
