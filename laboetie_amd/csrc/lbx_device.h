@@ -363,6 +363,23 @@ __device__ __forceinline__ double wave_sum(double v) {
     return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
+// wave_sum of NV values at once, level by level, so that the NV independent
+// chains fill each other's DPP wait states; the same additions per value.
+template <int NV>
+__device__ __forceinline__ void wave_sums(double (&v)[NV]) {
+#pragma unroll
+    for (int a = 0; a < NV; ++a) v[a] = v[a] + dpp_row<0xB1>(v[a]);
+#pragma unroll
+    for (int a = 0; a < NV; ++a) v[a] = v[a] + dpp_row<0x4E>(v[a]);
+#pragma unroll
+    for (int a = 0; a < NV; ++a) v[a] = v[a] + dpp_row<0x141>(v[a]);
+#pragma unroll
+    for (int a = 0; a < NV; ++a) v[a] = v[a] + dpp_row<0x140>(v[a]);
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+        v[a] = (read_lane(v[a], 0) + read_lane(v[a], 16)) + (read_lane(v[a], 32) + read_lane(v[a], 48));
+}
+
 // Maximum over the 64 lanes of a wave (fmax: a NaN lane is passed over),
 // returned to every lane (all 64 lanes active); the same row steps as wave_sum.
 __device__ __forceinline__ double wave_max(double v) {

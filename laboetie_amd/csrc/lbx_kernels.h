@@ -197,11 +197,13 @@ template <int NV, int BLOCK>
 __device__ __forceinline__ void block_partial(const double (&v)[NV], double* out, double* lds /* [BLOCK/64*NV] */) {
     static_assert(NV <= BLOCK, "one thread per value");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double s[NV];
 #pragma unroll
-    for (int a = 0; a < NV; ++a) {
-        const double s = wave_sum(v[a]);
-        if (lane == 0) lds[w * NV + a] = s;
-    }
+    for (int a = 0; a < NV; ++a) s[a] = v[a];
+    wave_sums<NV>(s);
+    if (lane == 0)
+#pragma unroll
+        for (int a = 0; a < NV; ++a) lds[w * NV + a] = s[a];
     __syncthreads();
     if (threadIdx.x < NV) {  // value a summed over the waves in wave order, by thread a
         const int a = threadIdx.x;
